@@ -1,0 +1,135 @@
+/*
+ * qpd.h - C ABI of the MI355X kinematic-diffraction library (libqpd.so).
+ *
+ * One call computes the diffraction intensity of a volume:
+ *
+ *   A(k) = S(k) sum_a exp(-i (k + G).R_a) / V [FFT(fE_a)(k)/n - i k . FFT(fE_a u_a)(k)/n]
+ *   fE_a(r) = sum_p w_p(r) f_pa exp(-i qC . u_ap(r)),   u_ap = u(r) + su_pa . P(r)
+ *   I(k) = fftshift(max(|A(k)|^2, 1e-50))
+ *
+ * which is the computation of src/python/diffraction.py:62-113 and of the
+ * source / fft.forward / accumulate / intensity sequence of
+ * src/heffte/qpop_diffraction_mgpu.cu:160-186 in the reference.
+ *
+ * Conventions: every pointer argument named *_dev or documented as device memory
+ * is device memory owned by the caller; the library owns only its plan tables.
+ * Work is enqueued on the stream passed in (a hipStream_t, NULL = default
+ * stream) and is asynchronous: no call synchronises the device except
+ * qpd_plan_create / qpd_plan_destroy.  Functions return QPD_OK (0) or a negative
+ * code; qpd_last_error() gives the message of the calling thread's last error.
+ * A plan may not be used by two host threads at once; distinct plans may run
+ * concurrently on distinct streams.
+ *
+ * Memory layout: a grid is given as three memory dims, dims[0] fastest; axis[d]
+ * is the physical axis (0 = x, 1 = y, 2 = z) of memory dim d.  FerroDyn files
+ * and src/heffte use x fastest: dims = {nx, ny, nz}, axis = {0, 1, 2}.  The
+ * reference's Python arrays are (x, y, z) C order, z fastest: dims = {nz, ny, nx},
+ * axis = {2, 1, 0}.  Fields and the intensity share the grid's layout.
+ */
+#ifndef QPD_H_
+#define QPD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QPD_OK 0
+#define QPD_ERR_ARG (-1)
+#define QPD_ERR_UNSUPPORTED (-2)
+#define QPD_ERR_HIP (-3)
+#define QPD_ERR_NOMEM (-4)
+
+#define QPD_MAX_ATOMS 64
+#define QPD_MAX_PHASES 16
+
+typedef struct qpd_plan qpd_plan;
+
+/* Grid of one volume; see the layout note above.  h[] is the spacing of the
+ * physical axes x, y, z in nm (SizeContext.dx/dy/dz, size_context.py:178-180). */
+typedef struct qpd_grid {
+    int64_t dims[3];
+    int axis[3];
+    double h[3];
+} qpd_grid;
+
+/* The crystal of parameter.atom.in after AtomList.setup (atom_list.py:37-193) and
+ * reciprocal_centre (diffraction.py:36-48).  Host pointers, copied at plan creation. */
+typedef struct qpd_crystal {
+    int n_atom;
+    int n_phase;
+    int n_struc;
+    const double* R;     /* (n_atom, 3) atom positions xAtom . aC, nm           (diffraction.py:94) */
+    const double* f_re;  /* (n_phase, n_atom) form factors, real part            (atom_list.py:102-111) */
+    const double* f_im;  /* (n_phase, n_atom) imaginary part */
+    const double* su;    /* (3, n_struc, n_phase, n_atom) shift per unit P, nm   (diffraction.py:93) */
+    double G[3];         /* reciprocal lattice point, nm^-1 */
+    double qC[3];        /* strain-corrected centre, nm^-1 */
+    double V;            /* cell volume det(aC), nm^3 */
+} qpd_crystal;
+
+/* Per-cell inputs of one volume, device memory, in the grid's layout, n cells each.
+ * Element type "real" is float for a precision-32 plan and double for precision 64. */
+typedef struct qpd_fields {
+    const void* P;                 /* (n_struc, n) real: order parameter, C/m^2     (diffraction.py:89) */
+    const double* U;               /* (3, n) double: displacement in nm, physical x, y, z components,
+                                      u_z reference ramp included                     (cli.py:215-217) */
+    const signed char* phase;      /* (n) phase index of the cell, -1 = does not scatter
+                                                                                      (kernels.cuh:55-66) */
+    const void* fraction;          /* (n_phase, n) real phase fractions, or NULL = one-hot from phase */
+    const void* region;            /* (n) real scattering weight, or NULL = 1 */
+} qpd_fields;
+
+/* Create a plan for one grid, crystal and precision (32: complex FP32 bulk with FP64
+ * phase reduction and FP64-generated twiddles; 64: complex FP64 throughout).
+ * Replaces the per-call setup of diffraction.py:75-94 and the HeFFTe plan of
+ * qpop_diffraction_mgpu.cu:154-157.  Allocates and uploads the plan tables. */
+int qpd_plan_create(qpd_plan** plan, const qpd_grid* grid, const qpd_crystal* crystal, int precision);
+
+/* Release a plan (synchronises the device). */
+int qpd_plan_destroy(qpd_plan* plan);
+
+/* Bytes of device workspace qpd_intensity needs (intermediate spectra). */
+size_t qpd_workspace_bytes(const qpd_plan* plan);
+
+/* Number of kernel launches one qpd_intensity call enqueues. */
+int qpd_launches_per_volume(const qpd_plan* plan);
+
+/* The intensity of one volume: fftshifted max(|A|^2, 1e-50) as double, n values in
+ * the grid's layout, written to I_dev.  work_dev holds qpd_workspace_bytes(plan).
+ * Replaces diffraction.py:96-112 and qpop_diffraction_mgpu.cu:166-186. */
+int qpd_intensity(qpd_plan* plan, const qpd_fields* fields, double* I_dev, void* work_dev, void* stream);
+
+/* Enqueue only pass `pass` (0, 1 or 2) of qpd_intensity: 0 = source + transform along
+ * dims[0], 1 = transform along dims[1], 2 = transform along dims[2] + intensity.  Has no
+ * counterpart in the reference; it exists so that each kernel can be timed on its own.
+ * Running passes 0, 1, 2 in order is exactly qpd_intensity. */
+int qpd_intensity_pass(qpd_plan* plan, const qpd_fields* fields, double* I_dev, void* work_dev, int pass,
+                       void* stream);
+
+/* Algorithmic bytes one launch of pass `pass` moves per cell: each input read once and
+ * each output written once (fields, intensity and source inputs as the plan uses them). */
+double qpd_pass_bytes_per_cell(const qpd_plan* plan, const qpd_fields* fields, int pass);
+
+/* Forward, unnormalised, in-place 3-D DFT (exponent -2 pi i jk/N on every dim) of
+ * n_fields complex fields stored one after another, dims[0] fastest; complex
+ * interleaved float (precision 32) or double (64).  The transform that
+ * xp.fft.fftn performs at diffraction.py:105-106, exposed for testing. */
+int qpd_fft3d(const int64_t dims[3], int precision, void* data_dev, int n_fields, void* stream);
+
+/* Whether this build supports a 1-D transform length (all prime factors <= 17). */
+int qpd_fft_length_supported(int64_t n);
+
+/* Message of the calling thread's last error ("" when none). */
+const char* qpd_last_error(void);
+
+/* ABI version of the library (major * 100 + minor). */
+int qpd_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QPD_H_ */

@@ -1,0 +1,269 @@
+// Device building blocks for gfx950: complex arithmetic, fast division by a
+// run-time constant, FP64-reduced FP32 sincos, and the LDS-resident
+// mixed-radix Stockham FFT used by every pass kernel.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace qpd {
+
+template <typename T>
+struct cx {
+    T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ cx<T> operator+(cx<T> a, cx<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename T>
+__device__ __forceinline__ cx<T> operator-(cx<T> a, cx<T> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename T>
+__device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> b) {
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename T>
+__device__ __forceinline__ cx<T> cscale(cx<T> a, T s) { return {a.x * s, a.y * s}; }
+// -i * a
+template <typename T>
+__device__ __forceinline__ cx<T> negi(cx<T> a) { return {a.y, -a.x}; }
+
+// Division by a run-time constant d: q = floor(n / d) for n * d < 2^32.
+// m = floor((2^32 - 1) / d) + 1, q = umulhi(n, m); d == 1 is flagged by m == 0.
+struct FastDiv {
+    uint32_t d;
+    uint32_t m;
+};
+
+__host__ inline FastDiv make_fastdiv(uint32_t d) {
+    FastDiv f;
+    f.d = d;
+    f.m = (d <= 1) ? 0u : (uint32_t)(0xFFFFFFFFu / d + 1u);
+    return f;
+}
+
+__device__ __forceinline__ uint32_t fdiv(uint32_t n, FastDiv f) {
+    return f.m == 0u ? n : __umulhi(n, f.m);
+}
+
+// sin and cos of an FP64 phase t (radians), FP32 results.  The argument is
+// reduced to quarter turns in FP64 (r = t * 2/pi, q = rint(r)), so phases of
+// thousands of radians keep FP32 accuracy; the remainder |x| <= pi/4 goes
+// through the Taylor series of sin (to x^9) and cos (to x^10), whose
+// truncation errors (< 2e-9 and < 2e-10) are below FP32 rounding.
+__device__ __forceinline__ void sincos_fp64_reduced(double t, float& s, float& c) {
+    const double two_over_pi = 0.636619772367581343075535053490057448;
+    const double half_pi = 1.570796326794896619231321691639751442;
+    double r = t * two_over_pi;
+    double q = rint(r);
+    float x = (float)((r - q) * half_pi);
+    float x2 = x * x;
+    float ps = fmaf(x2, 1.0f / 362880.0f, -1.0f / 5040.0f);
+    ps = fmaf(x2, ps, 1.0f / 120.0f);
+    ps = fmaf(x2, ps, -1.0f / 6.0f);
+    ps = fmaf(x2, ps, 1.0f);
+    float sx = x * ps;
+    float pc = fmaf(x2, -1.0f / 3628800.0f, 1.0f / 40320.0f);
+    pc = fmaf(x2, pc, -1.0f / 720.0f);
+    pc = fmaf(x2, pc, 1.0f / 24.0f);
+    pc = fmaf(x2, pc, -0.5f);
+    float cx_ = fmaf(x2, pc, 1.0f);
+    int quadrant = ((int)q) & 3;
+    float s0 = (quadrant & 1) ? cx_ : sx;
+    float c0 = (quadrant & 1) ? sx : cx_;
+    s = (quadrant & 2) ? -s0 : s0;
+    c = ((quadrant + 1) & 2) ? -c0 : c0;
+}
+
+__device__ __forceinline__ void phase_sincos(double t, float& s, float& c) { sincos_fp64_reduced(t, s, c); }
+__device__ __forceinline__ void phase_sincos(double t, double& s, double& c) { sincos(t, &s, &c); }
+
+// ---------------------------------------------------------------------------
+// 1-D transform plan of one axis: N = prod R[s]; stage s has sub-length Ns
+// (product of the earlier radices) and twiddle stride tws = N / (Ns R) into the
+// table W[m] = exp(-2 pi i m / N).  The stage list lives in device memory and
+// is read with uniform (scalar) loads.
+constexpr int kMaxStages = 16;
+constexpr int kMaxRegs = 16;  // complex values a thread holds during one stage
+
+struct Stage {
+    int R;
+    int Ns;
+    int tws;
+    uint32_t ns_m;  // FastDiv multiplier of Ns
+    int NR;         // N / R, read per stage so that nothing stage-specific is hoisted out of the stage loop
+    int pad[3];
+};
+
+// Host-side description of an axis (the device sees N, nst and the Stage list).
+struct AxisFFT {
+    int N;
+    int nst;
+    Stage st[kMaxStages];
+};
+
+template <typename T>
+__device__ __forceinline__ void dft2(cx<T>* v) {
+    cx<T> a = v[0], b = v[1];
+    v[0] = a + b;
+    v[1] = a - b;
+}
+
+template <typename T>
+__device__ __forceinline__ void dft4(cx<T>& v0, cx<T>& v1, cx<T>& v2, cx<T>& v3) {
+    cx<T> t0 = v0 + v2, t1 = v0 - v2, t2 = v1 + v3, t3 = negi(v1 - v3);
+    v0 = t0 + t2;
+    v2 = t0 - t2;
+    v1 = t1 + t3;
+    v3 = t1 - t3;
+}
+
+template <typename T>
+__device__ __forceinline__ void dft8(cx<T>* v) {
+    const T h = (T)0.707106781186547524400844362104849039;  // sqrt(1/2)
+    cx<T> e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+    cx<T> o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+    dft4(e0, e1, e2, e3);
+    dft4(o0, o1, o2, o3);
+    // o_k *= exp(-2 pi i k / 8)
+    o1 = {h * (o1.x + o1.y), h * (o1.y - o1.x)};
+    o2 = negi(o2);
+    o3 = {h * (o3.y - o3.x), -h * (o3.x + o3.y)};
+    v[0] = e0 + o0;
+    v[4] = e0 - o0;
+    v[1] = e1 + o1;
+    v[5] = e1 - o1;
+    v[2] = e2 + o2;
+    v[6] = e2 - o2;
+    v[3] = e3 + o3;
+    v[7] = e3 - o3;
+}
+
+// Odd-length DFT y[t] = sum_s v[s] exp(-2 pi i t s / R) using the symmetric pairs
+// a_s = v[s] + v[R-s], b_s = v[s] - v[R-s]; cr[m], sr[m] = cos, sin(2 pi m / R).
+template <int R, typename T>
+__device__ __forceinline__ void dft_odd(cx<T>* v, const T* cr, const T* sr) {
+    constexpr int H = (R - 1) / 2;
+    cx<T> a[H + 1], b[H + 1];
+    cx<T> y0 = v[0];
+#pragma unroll
+    for (int s = 1; s <= H; ++s) {
+        a[s] = v[s] + v[R - s];
+        b[s] = v[s] - v[R - s];
+        y0 = y0 + a[s];
+    }
+    cx<T> out[R];
+    out[0] = y0;
+#pragma unroll
+    for (int t = 1; t <= H; ++t) {
+        cx<T> A = v[0];
+        cx<T> B = {(T)0, (T)0};
+#pragma unroll
+        for (int s = 1; s <= H; ++s) {
+            const int m = (t * s) % R;
+            const int mm = m <= H ? m : R - m;
+            const T c = cr[mm];
+            const T sn = m <= H ? sr[mm] : -sr[mm];
+            A.x += a[s].x * c;
+            A.y += a[s].y * c;
+            B.x += b[s].x * sn;
+            B.y += b[s].y * sn;
+        }
+        out[t] = {A.x + B.y, A.y - B.x};
+        out[R - t] = {A.x - B.y, A.y + B.x};
+    }
+#pragma unroll
+    for (int t = 0; t < R; ++t) v[t] = out[t];
+}
+
+template <int R, typename T>
+__device__ __forceinline__ void dft(cx<T>* v, const T* cr, const T* sr) {
+    if constexpr (R == 2) {
+        dft2(v);
+    } else if constexpr (R == 4) {
+        dft4(v[0], v[1], v[2], v[3]);
+    } else if constexpr (R == 8) {
+        dft8(v);
+    } else {
+        dft_odd<R>(v, cr, sr);
+    }
+}
+
+// One Stockham stage, in place in LDS.  The tile holds L lines of N points,
+// point j of line l at tile[j * stride + l]; lines [0, Lc) are transformed.
+// Every thread first loads all its butterflies (at most kMaxRegs values), the
+// block synchronises, then the butterflies are twiddled, transformed and
+// written back in autosort order.
+template <int R, typename T>
+__device__ __forceinline__ void fft_stage(cx<T>* __restrict__ tile, const cx<T>* __restrict__ W, int NR,
+                                          int stride, int Lc, FastDiv divLc, int Ns, FastDiv divNs, int tws,
+                                          int tid, int nthr) {
+    constexpr int MB = (kMaxRegs / R) > 0 ? (kMaxRegs / R) : 1;
+    const int total = NR * Lc;
+    T cr[R / 2 + 1], sr[R / 2 + 1];
+    if constexpr ((R & 1) == 1) {
+#pragma unroll
+        for (int m = 1; m <= R / 2; ++m) {
+            cx<T> w = W[m * NR];
+            cr[m] = w.x;
+            sr[m] = -w.y;
+        }
+    }
+    cx<T> v[MB][R];
+#pragma unroll
+    for (int b = 0; b < MB; ++b) {
+        const int e = tid + b * nthr;
+        if (e < total) {
+            const int j = (int)fdiv((uint32_t)e, divLc);
+            const int line = e - j * Lc;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[b][r] = tile[(j + r * NR) * stride + line];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < MB; ++b) {
+        const int e = tid + b * nthr;
+        if (e < total) {
+            const int j = (int)fdiv((uint32_t)e, divLc);
+            const int line = e - j * Lc;
+            const int q = (int)fdiv((uint32_t)j, divNs);
+            const int k = j - q * Ns;
+            if (Ns > 1) {
+                const int base = k * tws;
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], W[r * base]);
+            }
+            dft<R>(v[b], cr, sr);
+            const int o = q * Ns * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) tile[(o + r * Ns) * stride + line] = v[b][r];
+        }
+    }
+    __syncthreads();
+}
+
+// Forward transform of lines [0, Lc) of an LDS tile (see fft_stage), natural order in and out.
+template <typename T>
+__device__ __forceinline__ void fft_tile(cx<T>* tile, int nst, const Stage* __restrict__ stages,
+                                         const cx<T>* __restrict__ W, int stride, int Lc, FastDiv divLc, int tid,
+                                         int nthr) {
+    for (int s = 0; s < nst; ++s) {
+        const Stage sd = stages[s];
+        const FastDiv dn = {(uint32_t)sd.Ns, sd.ns_m};
+        const int Ns = sd.Ns, tws = sd.tws, NR = sd.NR;
+        switch (sd.R) {
+            case 2: fft_stage<2, T>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
+            case 3: fft_stage<3, T>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
+            case 4: fft_stage<4, T>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
+            case 5: fft_stage<5, T>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
+            case 7: fft_stage<7, T>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
+            case 8: fft_stage<8, T>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
+            case 11: fft_stage<11, T>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
+            case 13: fft_stage<13, T>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
+            case 17: fft_stage<17, T>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
+            default: break;  // the host accepts no other radix
+        }
+    }
+}
+
+}  // namespace qpd

@@ -1,0 +1,340 @@
+// The three pass kernels of one diffraction volume (gfx950).
+//
+// The reference forms, per atom a, four fields fE_a, fE_a u_x, fE_a u_y, fE_a u_z,
+// transforms each with a 3-D FFT and adds phi_a(k) [FE - i k . FU] to the amplitude
+// (src/python/diffraction.py:97-109, src/heffte/kernels.cuh:55-111).  phi_a(k) and
+// the voxel shape factor S(k) factor into one term per axis, and -i k_d acts on one
+// axis only, so both commute with the transforms along the other axes.  A volume is
+// therefore three passes, one per memory dim, each "FFT along the dim, multiply by a
+// per-index coefficient, sum into fewer outputs":
+//
+//   pass 0 (contiguous dim):  inputs computed from P, U and the weights (the source);
+//                             per atom [E, U_d0, U_d1, U_d2] -> per class of atoms with
+//                             equal (R_d1, R_d2): [A, B_d1, B_d2]
+//   pass 1 (middle dim):      per class [A, B_d1, B_d2] -> per class of equal R_d2: [C, D]
+//   pass 2 (outer dim):       per class [C, D] -> amplitude -> fftshifted max(|.|^2, 1e-50)
+//
+// In every pass a chunk of M+1 inputs feeds the group's M outputs:
+//   out_0 += c(k) F[in_0] + c(k) (-i k) F[in_1],  out_j += c(k) F[in_{j+1}] (j >= 1),
+// with c(k) a per-chunk coefficient table (exp(-i k R) s(k), and exp(-i G.R_a)/(V n)
+// in pass 0).  A plain transform is the case M+1 = 1 with c = 1.
+//
+// Offsets inside one field are 32-bit (the host rejects grids of 2^31 cells or more);
+// field base pointers are block-uniform 64-bit values.
+#pragma once
+
+#include "qpd_device.hpp"
+
+namespace qpd {
+
+constexpr int kAtomTab = 11;  // per (atom, phase): Re f, Im f, su[d][k] (d, k = 0..2)
+
+template <typename T>
+struct PassArgs {
+    int N;                    // transform length along this pass
+    int nst;                  // its number of stages
+    const Stage* stages;      // its stage list (device memory)
+    const cx<T>* W;           // twiddles exp(-2 pi i m / N), N entries
+    const cx<T>* coef;        // coefficient tables, [table][N]
+    const T* k;               // wavenumber of each index along this dim, nm^-1
+    const int* chunk_src;     // source of each chunk: atom (pass 0) or input class
+    const int* chunk_coef;    // coefficient table of each chunk, -1 = identity
+    const int* group_first;   // chunks of group g: [group_first[g], group_first[g+1])
+    int ngroups;
+    int nin;                  // inputs per chunk (M + 1, or 1 for a plain transform)
+    int nout;                 // outputs per group (M)
+    int nf;                   // inputs resident in LDS at once
+    uint32_t n0, n1, n2;      // memory extents, n0 fastest
+    uint32_t ncell;
+    int nl;                   // rows (contiguous pass) or TX (strided pass) per block
+    FastDiv divN;             // division by N (element -> row)
+    FastDiv divL1, divL2, divL3, divL4;  // division by nfc * nl, nfc = 1..4
+    // strided pass: lines along dim `dim` (1 or 2)
+    int dim;
+    int log2TX;
+    uint32_t ntiles0;
+    uint32_t S;               // element stride along the pass
+    uint32_t OS;              // stride of the outer index
+    // data
+    const cx<T>* in;
+    cx<T>* out;
+    double* I;
+    // source (pass 0 only)
+    const T* P;
+    const double* U;
+    const signed char* phase;
+    const T* fraction;
+    const T* region;
+    const T* atomtab;         // [atom][phase][kAtomTab]
+    double qC0, qC1, qC2;
+    int n_phase;
+    int ax0, ax1, ax2;        // physical axis of memory dims 0, 1, 2
+};
+
+// v[i] for a run-time i in 0..2 without indexing a private array.
+__device__ __forceinline__ double pick3(double v0, double v1, double v2, int i) {
+    return i == 0 ? v0 : (i == 1 ? v1 : v2);
+}
+
+template <typename T>
+__device__ __forceinline__ FastDiv lines_div(const PassArgs<T>& a, int nfc) {
+    return nfc == 1 ? a.divL1 : (nfc == 2 ? a.divL2 : (nfc == 3 ? a.divL3 : a.divL4));
+}
+
+// ---------------------------------------------------------------------------
+// Source of one cell for one atom (kernels.cuh:55-86; diffraction.py:100-104):
+//   E = sum_p w_p f_pa e^{-i qC.u_ap},  U_j = sum_p w_p f_pa e^{-i qC.u_ap} u_ap[ax_j],
+//   u_ap = U + su[:, :, p, a] . P.
+// The phase qC.u_ap is formed in FP64; see phase_sincos.
+template <typename T>
+struct CellInputs {
+    T P0, P1, P2;
+    double U0, U1, U2;
+    T rg;
+    int ph;
+};
+
+template <typename T>
+__device__ __forceinline__ void cell_source(const PassArgs<T>& a, const CellInputs<T>& ci, uint32_t off, int atom,
+                                            cx<T>& E, cx<T>& V0, cx<T>& V1, cx<T>& V2) {
+    E = V0 = V1 = V2 = {(T)0, (T)0};
+    if (ci.ph < 0) return;
+    const int p_lo = a.fraction ? 0 : ci.ph;
+    const int p_hi = a.fraction ? a.n_phase : ci.ph + 1;
+    for (int p = p_lo; p < p_hi; ++p) {
+        T w = ci.rg;
+        if (a.fraction) w *= a.fraction[(size_t)p * a.ncell + off];
+        if (w == (T)0) continue;
+        const T* tab = a.atomtab + (atom * a.n_phase + p) * kAtomTab;
+        const double ux = ci.U0 + (double)(tab[2] * ci.P0 + tab[3] * ci.P1 + tab[4] * ci.P2);
+        const double uy = ci.U1 + (double)(tab[5] * ci.P0 + tab[6] * ci.P1 + tab[7] * ci.P2);
+        const double uz = ci.U2 + (double)(tab[8] * ci.P0 + tab[9] * ci.P1 + tab[10] * ci.P2);
+        const double t = a.qC0 * ux + a.qC1 * uy + a.qC2 * uz;
+        T sn, cs;
+        phase_sincos(t, sn, cs);
+        const T fr = tab[0] * w, fi = tab[1] * w;
+        // w f e^{-i t} = w (fr + i fi)(cs - i sn)
+        const cx<T> e = {fr * cs + fi * sn, fi * cs - fr * sn};
+        E = E + e;
+        V0 = V0 + cscale(e, (T)pick3(ux, uy, uz, a.ax0));
+        V1 = V1 + cscale(e, (T)pick3(ux, uy, uz, a.ax1));
+        V2 = V2 + cscale(e, (T)pick3(ux, uy, uz, a.ax2));
+    }
+}
+
+// Input jj of a chunk, already multiplied by the chunk coefficient, into the outputs:
+// 0 -> out0, 1 -> out0 times -i k, j >= 2 -> out_{j-1}.
+template <typename T, int NOUT>
+__device__ __forceinline__ void accumulate_term(int jj, cx<T> t, T kv, cx<T>* acc) {
+    if (jj == 0) {
+        acc[0] = acc[0] + t;
+    } else if (jj == 1) {
+        acc[0] = acc[0] + cscale(negi(t), kv);
+    } else if constexpr (NOUT >= 2) {
+        if (jj == 2) {
+            acc[1] = acc[1] + t;
+        } else if constexpr (NOUT >= 3) {
+            acc[2] = acc[2] + t;
+        }
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ cx<T> chunk_coef(const PassArgs<T>& a, int ct, int idx) {
+    return ct >= 0 ? a.coef[ct * a.N + idx] : cx<T>{(T)1, (T)0};
+}
+
+// ---------------------------------------------------------------------------
+// Pass along the contiguous dim: block = nl consecutive rows of n0 points.
+// LDS tile: point idx of line (f, row) at tile[idx * (nf*nl) + f*nl + row].
+// Element e = tid + m*nthr of a field's tile is (row, idx) = divmod(e, N).
+template <typename T, int EPT, int NOUT, bool SOURCE>
+__global__ __launch_bounds__(512) void pass_contig(PassArgs<T> a) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T>* tile = reinterpret_cast<cx<T>*>(smem_raw);
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int N = a.N;
+    const int nl = a.nl;
+    const uint32_t nrows = a.n1 * a.n2;
+    const uint32_t row0 = blockIdx.x * (uint32_t)nl;
+    const int tileN = nl * N;
+    const int stride = a.nf * nl;
+    const uint32_t cell0 = row0 * (uint32_t)N;
+
+    CellInputs<T> ci[SOURCE ? EPT : 1];
+    if constexpr (SOURCE) {
+#pragma unroll
+        for (int m = 0; m < EPT; ++m) {
+            const int e = tid + m * nthr;
+            const int row = (int)fdiv((uint32_t)e, a.divN);
+            const bool valid = e < tileN && row0 + row < nrows;
+            ci[m].ph = -1;
+            ci[m].rg = (T)1;
+            ci[m].P0 = ci[m].P1 = ci[m].P2 = (T)0;
+            ci[m].U0 = ci[m].U1 = ci[m].U2 = 0.0;
+            if (valid) {
+                const uint32_t off = cell0 + (uint32_t)e;
+                ci[m].ph = a.phase[off];
+                if (a.region) ci[m].rg = a.region[off];
+                ci[m].P0 = a.P[off];
+                ci[m].P1 = a.P[a.ncell + off];
+                ci[m].P2 = a.P[(size_t)2 * a.ncell + off];
+                ci[m].U0 = a.U[off];
+                ci[m].U1 = a.U[a.ncell + off];
+                ci[m].U2 = a.U[(size_t)2 * a.ncell + off];
+            }
+        }
+    }
+
+    for (int g = 0; g < a.ngroups; ++g) {
+        cx<T> acc[EPT][NOUT];
+#pragma unroll
+        for (int m = 0; m < EPT; ++m)
+#pragma unroll
+            for (int o = 0; o < NOUT; ++o) acc[m][o] = {(T)0, (T)0};
+
+        for (int c = a.group_first[g]; c < a.group_first[g + 1]; ++c) {
+            const int src = a.chunk_src[c];
+            const int ct = a.chunk_coef[c];
+            for (int j0 = 0; j0 < a.nin; j0 += a.nf) {
+                const int nfc = min(a.nf, a.nin - j0);
+                const cx<T>* inbase = SOURCE ? nullptr : a.in + ((size_t)src * a.nin + j0) * a.ncell + cell0;
+#pragma unroll
+                for (int m = 0; m < EPT; ++m) {
+                    const int e = tid + m * nthr;
+                    if (e >= tileN) continue;
+                    const int row = (int)fdiv((uint32_t)e, a.divN);
+                    const int idx = e - row * N;
+                    cx<T>* dst = tile + idx * stride + row;
+                    if constexpr (SOURCE) {
+                        cx<T> v0, v1, v2, v3;
+                        cell_source(a, ci[m], cell0 + (uint32_t)e, src, v0, v1, v2, v3);
+                        for (int f = 0; f < nfc; ++f) {
+                            const int jj = j0 + f;
+                            dst[f * nl] = jj == 0 ? v0 : (jj == 1 ? v1 : (jj == 2 ? v2 : v3));
+                        }
+                    } else {
+                        const bool valid = row0 + row < nrows;
+                        for (int f = 0; f < nfc; ++f)
+                            dst[f * nl] = valid ? inbase[(size_t)f * a.ncell + e] : cx<T>{(T)0, (T)0};
+                    }
+                }
+                __syncthreads();
+                fft_tile(tile, a.nst, a.stages, a.W, stride, nfc * nl, lines_div(a, nfc), tid, nthr);
+#pragma unroll
+                for (int m = 0; m < EPT; ++m) {
+                    const int e = tid + m * nthr;
+                    if (e >= tileN) continue;
+                    const int row = (int)fdiv((uint32_t)e, a.divN);
+                    const int idx = e - row * N;
+                    const cx<T> cf = chunk_coef(a, ct, idx);
+                    const T kv = a.k[idx];
+                    const cx<T>* srcp = tile + idx * stride + row;
+                    for (int f = 0; f < nfc; ++f) accumulate_term<T, NOUT>(j0 + f, cmul(cf, srcp[f * nl]), kv, acc[m]);
+                }
+                __syncthreads();
+            }
+        }
+        cx<T>* obase = a.out + (size_t)g * NOUT * a.ncell + cell0;
+#pragma unroll
+        for (int m = 0; m < EPT; ++m) {
+            const int e = tid + m * nthr;
+            if (e >= tileN) continue;
+            const int row = (int)fdiv((uint32_t)e, a.divN);
+            if (row0 + row >= nrows) continue;
+#pragma unroll
+            for (int q = 0; q < NOUT; ++q) obase[(size_t)q * a.ncell + e] = acc[m][q];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Pass along a strided dim (1 or 2): block = TX consecutive i0 x one outer index.
+// LDS tile: point idx of line (f, tx) at tile[idx * (nf*TX) + f*TX + tx].  A thread
+// keeps one column tx = tid % TX and the points idx = tid / TX + m * (nthr / TX).
+// INTENSITY: the single output is the amplitude; write fftshifted max(|A|^2, 1e-50).
+template <typename T, int EPT, int NOUT, bool INTENSITY>
+__global__ __launch_bounds__(512) void pass_strided(PassArgs<T> a) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T>* tile = reinterpret_cast<cx<T>*>(smem_raw);
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int N = a.N;
+    const int TX = 1 << a.log2TX;
+    const int stride = a.nf * TX;
+    const uint32_t t0 = blockIdx.x % a.ntiles0;
+    const uint32_t outer = blockIdx.x / a.ntiles0;
+    const int tx = tid & (TX - 1);
+    const int r0 = tid >> a.log2TX;
+    const int rstep = nthr >> a.log2TX;
+    const uint32_t i0 = t0 * (uint32_t)TX + (uint32_t)tx;
+    const bool col_ok = i0 < a.n0;
+    const uint32_t colbase = i0 + outer * a.OS;  // offset of point idx = 0 of this column
+
+    for (int g = 0; g < a.ngroups; ++g) {
+        cx<T> acc[EPT][NOUT];
+#pragma unroll
+        for (int m = 0; m < EPT; ++m)
+#pragma unroll
+            for (int o = 0; o < NOUT; ++o) acc[m][o] = {(T)0, (T)0};
+
+        for (int c = a.group_first[g]; c < a.group_first[g + 1]; ++c) {
+            const int src = a.chunk_src[c];
+            const int ct = a.chunk_coef[c];
+            for (int j0 = 0; j0 < a.nin; j0 += a.nf) {
+                const int nfc = min(a.nf, a.nin - j0);
+                const cx<T>* inbase = a.in + ((size_t)src * a.nin + j0) * a.ncell;
+#pragma unroll
+                for (int m = 0; m < EPT; ++m) {
+                    const int idx = r0 + m * rstep;
+                    if (idx >= N) continue;
+                    const uint32_t off = colbase + (uint32_t)idx * a.S;
+                    cx<T>* dst = tile + idx * stride + tx;
+                    for (int f = 0; f < nfc; ++f)
+                        dst[f * TX] = col_ok ? inbase[(size_t)f * a.ncell + off] : cx<T>{(T)0, (T)0};
+                }
+                __syncthreads();
+                fft_tile(tile, a.nst, a.stages, a.W, stride, nfc * TX, lines_div(a, nfc), tid, nthr);
+#pragma unroll
+                for (int m = 0; m < EPT; ++m) {
+                    const int idx = r0 + m * rstep;
+                    if (idx >= N) continue;
+                    const cx<T> cf = chunk_coef(a, ct, idx);
+                    const T kv = a.k[idx];
+                    const cx<T>* srcp = tile + idx * stride + tx;
+                    for (int f = 0; f < nfc; ++f) accumulate_term<T, NOUT>(j0 + f, cmul(cf, srcp[f * TX]), kv, acc[m]);
+                }
+                __syncthreads();
+            }
+        }
+        if (!col_ok) continue;
+        if constexpr (INTENSITY) {
+            // fftshift: index i of an axis of length n goes to (i + n/2) mod n
+            const uint32_t s0 = (i0 + a.n0 / 2) >= a.n0 ? i0 + a.n0 / 2 - a.n0 : i0 + a.n0 / 2;
+            const uint32_t nA = a.dim == 2 ? a.n2 : a.n1;  // axis of idx
+            const uint32_t nB = a.dim == 2 ? a.n1 : a.n2;  // axis of outer
+            const uint32_t sB = (outer + nB / 2) >= nB ? outer + nB / 2 - nB : outer + nB / 2;
+#pragma unroll
+            for (int m = 0; m < EPT; ++m) {
+                const int idx = r0 + m * rstep;
+                if (idx >= N) continue;
+                const uint32_t sA = ((uint32_t)idx + nA / 2) >= nA ? idx + nA / 2 - nA : idx + nA / 2;
+                const uint32_t s1 = a.dim == 2 ? sB : sA;
+                const uint32_t s2 = a.dim == 2 ? sA : sB;
+                const double re = (double)acc[m][0].x, im = (double)acc[m][0].y;
+                a.I[(size_t)s0 + (size_t)a.n0 * ((size_t)s1 + (size_t)a.n1 * s2)] = fmax(re * re + im * im, 1e-50);
+            }
+        } else {
+            cx<T>* obase = a.out + (size_t)g * NOUT * a.ncell;
+#pragma unroll
+            for (int m = 0; m < EPT; ++m) {
+                const int idx = r0 + m * rstep;
+                if (idx >= N) continue;
+                const uint32_t off = colbase + (uint32_t)idx * a.S;
+#pragma unroll
+                for (int q = 0; q < NOUT; ++q) obase[(size_t)q * a.ncell + off] = acc[m][q];
+            }
+        }
+    }
+}
+
+}  // namespace qpd

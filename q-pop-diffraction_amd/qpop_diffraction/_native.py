@@ -1,0 +1,185 @@
+"""ctypes binding of libqpd.so, the HIP/gfx950 library of the diffraction path.
+
+The library is built in-tree (``qpop_diffraction/lib/libqpd.so``; ``QPD_LIB``
+overrides the path).  Device buffers are PyTorch-ROCm tensors passed by
+``data_ptr()``; the stream is ``torch.cuda.current_stream().cuda_stream``.
+Nothing here falls back to the CPU: a missing library or GPU raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(os.environ.get("QPD_LIB", Path(__file__).resolve().parent / "lib" / "libqpd.so"))
+
+# Entry points declared in include/qpd.h
+EXPORTS = (
+    "qpd_plan_create", "qpd_plan_destroy", "qpd_workspace_bytes", "qpd_launches_per_volume",
+    "qpd_intensity", "qpd_intensity_pass", "qpd_pass_bytes_per_cell", "qpd_fft3d", "qpd_fft_length_supported", "qpd_last_error", "qpd_version",
+)
+
+c_double3 = ctypes.c_double * 3
+
+
+class Grid(ctypes.Structure):
+    _fields_ = [("dims", ctypes.c_int64 * 3), ("axis", ctypes.c_int * 3), ("h", c_double3)]
+
+
+class Crystal(ctypes.Structure):
+    _fields_ = [
+        ("n_atom", ctypes.c_int), ("n_phase", ctypes.c_int), ("n_struc", ctypes.c_int),
+        ("R", ctypes.POINTER(ctypes.c_double)), ("f_re", ctypes.POINTER(ctypes.c_double)),
+        ("f_im", ctypes.POINTER(ctypes.c_double)), ("su", ctypes.POINTER(ctypes.c_double)),
+        ("G", c_double3), ("qC", c_double3), ("V", ctypes.c_double),
+    ]
+
+
+class Fields(ctypes.Structure):
+    _fields_ = [("P", ctypes.c_void_p), ("U", ctypes.c_void_p), ("phase", ctypes.c_void_p),
+                ("fraction", ctypes.c_void_p), ("region", ctypes.c_void_p)]
+
+
+class QPDError(RuntimeError):
+    """A libqpd call returned an error code."""
+
+
+_lib = None
+
+
+def load_library():
+    """Load libqpd.so once; raise ImportError when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ImportError(f"libqpd.so not found at {LIB_PATH}; build it with `python __graft_entry__.py build` "
+                          "or `python q-pop-diffraction_amd/build_lib.py`")
+    lib = ctypes.CDLL(str(LIB_PATH))
+    vp, i32, i64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64
+    lib.qpd_plan_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(Grid), ctypes.POINTER(Crystal), i32]
+    lib.qpd_plan_create.restype = i32
+    lib.qpd_plan_destroy.argtypes = [vp]
+    lib.qpd_plan_destroy.restype = i32
+    lib.qpd_workspace_bytes.argtypes = [vp]
+    lib.qpd_workspace_bytes.restype = ctypes.c_size_t
+    lib.qpd_launches_per_volume.argtypes = [vp]
+    lib.qpd_launches_per_volume.restype = i32
+    lib.qpd_intensity.argtypes = [vp, ctypes.POINTER(Fields), vp, vp, vp]
+    lib.qpd_intensity.restype = i32
+    lib.qpd_intensity_pass.argtypes = [vp, ctypes.POINTER(Fields), vp, vp, i32, vp]
+    lib.qpd_intensity_pass.restype = i32
+    lib.qpd_pass_bytes_per_cell.argtypes = [vp, ctypes.POINTER(Fields), i32]
+    lib.qpd_pass_bytes_per_cell.restype = ctypes.c_double
+    lib.qpd_fft3d.argtypes = [ctypes.POINTER(i64), i32, vp, i32, vp]
+    lib.qpd_fft3d.restype = i32
+    lib.qpd_fft_length_supported.argtypes = [i64]
+    lib.qpd_fft_length_supported.restype = i32
+    lib.qpd_last_error.argtypes = []
+    lib.qpd_last_error.restype = ctypes.c_char_p
+    lib.qpd_version.argtypes = []
+    lib.qpd_version.restype = i32
+    _lib = lib
+    return lib
+
+
+def check(rc):
+    if rc != 0:
+        msg = load_library().qpd_last_error().decode(errors="replace")
+        raise QPDError(f"libqpd error {rc}: {msg}")
+
+
+def _dptr(a):
+    return ctypes.cast(a.ctypes.data, ctypes.POINTER(ctypes.c_double))
+
+
+class Plan:
+    """A libqpd plan: one grid layout, one crystal, one precision.
+
+    dims are the memory extents (dims[0] fastest) and axis[d] the physical axis
+    (0 = x, 1 = y, 2 = z) of memory dim d; h the x, y, z spacings in nm.
+    """
+
+    def __init__(self, dims, axis, h, R, f, su, G, qC, V, precision=32):
+        lib = load_library()
+        self.dims = tuple(int(d) for d in dims)
+        self.axis = tuple(int(a) for a in axis)
+        self.precision = int(precision)
+        self.n = int(np.prod(self.dims))
+        self._R = np.ascontiguousarray(R, dtype=np.float64)
+        f = np.asarray(f, dtype=complex)
+        self._fre = np.ascontiguousarray(f.real, dtype=np.float64)
+        self._fim = np.ascontiguousarray(f.imag, dtype=np.float64)
+        self._su = np.ascontiguousarray(su, dtype=np.float64)
+        n_atom = self._R.shape[0]
+        n_phase = f.shape[0]
+        if self._su.shape != (3, 3, n_phase, n_atom):
+            raise ValueError(f"su must be (3, 3, nPhase, nAtom), got {self._su.shape}")
+        self.n_phase = n_phase
+        self.n_atom = n_atom
+        grid = Grid((ctypes.c_int64 * 3)(*self.dims), (ctypes.c_int * 3)(*self.axis), c_double3(*h))
+        cr = Crystal(n_atom, n_phase, 3, _dptr(self._R), _dptr(self._fre), _dptr(self._fim), _dptr(self._su),
+                     c_double3(*G), c_double3(*qC), float(V))
+        handle = ctypes.c_void_p()
+        check(lib.qpd_plan_create(ctypes.byref(handle), ctypes.byref(grid), ctypes.byref(cr), self.precision))
+        self._h = handle
+        self.workspace_bytes = int(lib.qpd_workspace_bytes(handle))
+        self.launches = int(lib.qpd_launches_per_volume(handle))
+
+    def intensity(self, P, U, phase, I, work, fraction=None, region=None, stream=None):
+        """Enqueue one volume: device tensors in, fftshifted |A|^2 (float64) into I."""
+        fields = Fields(P.data_ptr(), U.data_ptr(), phase.data_ptr(),
+                        fraction.data_ptr() if fraction is not None else None,
+                        region.data_ptr() if region is not None else None)
+        check(load_library().qpd_intensity(self._h, ctypes.byref(fields), I.data_ptr(), work.data_ptr(),
+                                           stream if stream is not None else current_stream()))
+
+    def _fields(self, P, U, phase, fraction, region):
+        return Fields(P.data_ptr(), U.data_ptr(), phase.data_ptr(),
+                      fraction.data_ptr() if fraction is not None else None,
+                      region.data_ptr() if region is not None else None)
+
+    def run_pass(self, which, P, U, phase, I, work, fraction=None, region=None, stream=None):
+        """Enqueue one of the three passes only (for timing each kernel)."""
+        fields = self._fields(P, U, phase, fraction, region)
+        check(load_library().qpd_intensity_pass(self._h, ctypes.byref(fields), I.data_ptr(), work.data_ptr(),
+                                                int(which), stream if stream is not None else current_stream()))
+
+    def pass_bytes_per_cell(self, which, P, U, phase, fraction=None, region=None):
+        fields = self._fields(P, U, phase, fraction, region)
+        return float(load_library().qpd_pass_bytes_per_cell(self._h, ctypes.byref(fields), int(which)))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().qpd_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def current_stream():
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def fft3d(x, precision=None, stream=None):
+    """In-place forward unnormalised 3-D DFT of a complex device tensor (..., d2, d1, d0)."""
+    import torch
+    if x.dtype not in (torch.complex64, torch.complex128) or not x.is_contiguous() or not x.is_cuda:
+        raise TypeError("fft3d needs a contiguous complex64/complex128 CUDA tensor")
+    prec = 32 if x.dtype == torch.complex64 else 64
+    if precision is not None and precision != prec:
+        raise TypeError("precision does not match the tensor dtype")
+    d2, d1, d0 = x.shape[-3:]
+    nf = int(np.prod(x.shape[:-3])) if x.dim() > 3 else 1
+    dims = (ctypes.c_int64 * 3)(d0, d1, d2)
+    check(load_library().qpd_fft3d(dims, prec, ctypes.c_void_p(x.data_ptr()), nf,
+                                   stream if stream is not None else current_stream()))
+    return x

@@ -1,0 +1,137 @@
+#!/usr/bin/env python3
+"""``qpop-diffraction``: the drop-in command line (reference: src/python/cli.py:66-94,149-225).
+
+Reads input.json, system_setting.json (with materials/ and data/materialField_0.h5
+beside it) and parameter.atom.in, and writes {output_file}.h5 (/intensity shaped
+(nz, ny, nx), /q_x, /q_y, /q_z) and {output_file}.xdmf, as the reference does.
+
+The fields stay in FerroDyn's file order (x fastest) from the HDF5 reader to the
+GPU and back to the writer: the plan is built for memory dims (nx, ny, nz) with
+physical axes (0, 1, 2), so the intensity comes out already shaped (nz, ny, nx).
+``--mgpu`` (the reference's multi-GPU two-step) is not offered: one MI355X holds
+the grids that program was written for.
+"""
+
+import argparse
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+from .atom_list import AtomList
+from .constants import L0, OS0
+from .diffraction import Engine, Volume, default_precision, q_axes
+from .h5lite import open_file
+from .io_utils import crop_slices, read_field, read_material_field, read_vector, write_diffraction_output
+from .json_input import load_input_json
+from .size_context import SizeContext
+
+
+def _parse_args(argv=None) -> argparse.Namespace:
+    parser = argparse.ArgumentParser(
+        description="Q-POP Diffraction Simulation for Q-POP-FerroDyn output (MI355X)",
+        formatter_class=argparse.ArgumentDefaultsHelpFormatter,
+    )
+    parser.add_argument("--input", "-i", default="input.json", metavar="JSON",
+                        help="Path to input.json (P_Lattice/Displacement paths, slice limits, output name).")
+    parser.add_argument("--system", "-s", default="system_setting.json", metavar="JSON",
+                        help="Path to the FerroDyn system_setting.json of the run.")
+    parser.add_argument("--atom", "-a", default="parameter.atom.in", metavar="FILE",
+                        help="Path to parameter.atom.in.")
+    parser.add_argument("--precision", type=int, choices=(32, 64), default=None,
+                        help="Working precision: 32 (complex FP32, FP64 phase reduction) or 64. "
+                             "Default: QPOP_DIFFRACTION_PRECISION or 32.")
+    return parser.parse_args(argv)
+
+
+def prepare_run(args):
+    """Everything before the GPU call: config, crystal, fields in file order (cli.py:149-217).
+
+    Returns (params, atoms, fields dict, step, output base name) or raises.
+    """
+    cfg = load_input_json(args.input)
+    lims = (cfg["x_limits"], cfg["y_limits"], cfg["z_limits"])
+
+    params = SizeContext()
+    fe_ids = params.read_json(args.system)
+    full = (params.nx, params.ny, params.nz)
+    dz = params.lz / params.nz
+    sl = crop_slices(*full, *lims)
+
+    atoms = AtomList()
+    if not atoms.read(args.atom, params.nPhase, params.nStruc) or not atoms.setup():
+        return None
+
+    real = np.float32 if args.precision == 32 else np.float64
+    print(f"Loading P_Lattice file: {cfg['platt_file']}")
+    P = read_vector(cfg["platt_file"], ("P_Lattice", "PLatt"), *full, *lims, dtype=real)
+    if OS0 != 1.0:
+        P = P / OS0
+    with open_file(cfg["platt_file"], "r") as fh:
+        step = int(fh.attrs["Timestep"])
+
+    if cfg["displacement_file"] is not None:
+        print(f"Loading Displacement file: {cfg['displacement_file']}")
+        U = read_vector(cfg["displacement_file"], ("Displacement",), *full, *lims, dtype=np.float64) / L0
+    else:
+        print("No displacement_file provided — using zero displacement field.")
+        U = np.zeros((3,) + P.shape[1:], dtype=np.float64)
+
+    # phases: cells of non-FE materials do not scatter (cli.py:189-200)
+    mat_file = Path(args.system).parent / "data" / "materialField_0.h5"
+    print(f"Loading material field: {mat_file}")
+    mat = read_material_field(mat_file, *full)  # (nz, ny, nx)
+    zyx = (sl[2], sl[1], sl[0])
+    phase_of = np.full(len(params.names), -1, dtype=np.int8)
+    phase_of[fe_ids] = np.arange(len(fe_ids), dtype=np.int8)
+    phase = phase_of[mat[zyx]]
+    fraction = None
+    if cfg["phase_file"] is not None:
+        print(f"Loading phase fractions: {cfg['phase_file']}")
+        with open_file(cfg["phase_file"], "r") as fh:
+            fraction = np.stack([read_field(fh, f"phase_{p + 1}", *full, sl, real) for p in range(params.nPhase)])
+    region = None
+    if cfg["region_file"] is not None:
+        print(f"Loading scattering weight: {cfg['region_file']}")
+        with open_file(cfg["region_file"], "r") as fh:
+            region = read_field(fh, "region", *full, sl, real)
+
+    # u_z reference ramp over the full stack, then cropped (cli.py:202-217)
+    print(f"\nTimestep {step}:")
+    dc = params.reference_strain(np.diag(atoms.aC), params.temperature(step))[mat]
+    uz_ref = ((np.cumsum(dc, axis=0) - 0.5 * dc) * dz)[zyx]
+    U[2] += uz_ref
+
+    print("\nApplying domain slice:")
+    params.apply_slice(*lims)
+    params.setup()
+    fields = {"P": P, "U": U, "phase": phase, "fraction": fraction, "region": region}
+    return params, atoms, fields, step, cfg["output_file"]
+
+
+def main(argv=None) -> int:
+    """Run one diffraction calculation; 0 on success, 1 when parameter.atom.in is unusable."""
+    args = _parse_args(argv)
+    if args.precision is None:
+        args.precision = default_precision()
+    prepared = prepare_run(args)
+    if prepared is None:
+        return 1
+    params, atoms, f, step, out_base = prepared
+
+    print(f"\nComputing diffraction (MI355X, complex FP{args.precision})")
+    t0 = time.perf_counter()
+    engine = Engine(params, atoms)
+    dims, axis = (params.nx, params.ny, params.nz), (0, 1, 2)
+    vol = Volume(dims, axis, f["P"], f["U"], f["phase"], f["fraction"], f["region"], args.precision)
+    I = engine.intensity(vol).cpu().numpy().reshape(params.nz, params.ny, params.nx)
+    print(f"  GPU path {time.perf_counter() - t0:.3f} s for {params.n} cells")
+    qC = engine.tables[4]
+    write_diffraction_output(out_base, I, *q_axes(params, qC), step, zyx=True)
+    print("\nDiffraction simulation completed.")
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

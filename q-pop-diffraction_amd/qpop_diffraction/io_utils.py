@@ -1,0 +1,180 @@
+"""FerroDyn field input and diffraction output (reference: src/python/io_utils.py:16-141,186-207).
+
+FerroDyn stores each field component as a dataset shaped (nz, ny, nx), x fastest,
+in SI units.  Two families of loaders are offered:
+
+* the reference's own, which return the cropped field in (x, y, z) C order,
+  float64, flattened (``load_platt_hdf5`` and friends) - the input layout of
+  ``diffraction()``;
+* file-order loaders (``read_field``, ``read_vector``) that keep (z, y, x) order
+  and the dtype asked for.  The GPU path consumes that layout directly, so the
+  command-line run never transposes a field on the host.
+
+Output: ``{base}.h5`` with ``/intensity`` shaped (nz, ny, nx) float64, the axes
+``/q_x``, ``/q_y``, ``/q_z`` (nm^-1), attributes ``Timestep`` and ``q_units``,
+and the XDMF 3DRectMesh sidecar ``{base}.xdmf``.
+"""
+
+from __future__ import annotations
+
+import os
+
+import numpy as np
+
+from .constants import L0
+from .h5lite import open_file, read_dataset
+
+
+def crop_slices(nx, ny, nz, x_lim, y_lim, z_lim):
+    """Inclusive 0-based limits (None = whole axis) as (x, y, z) slices."""
+    out = []
+    for lim, n in ((x_lim, nx), (y_lim, ny), (z_lim, nz)):
+        out.append(slice(0, n) if lim is None else slice(lim[0], lim[1] + 1))
+    return tuple(out)
+
+
+def _checked(fh, name, nx, ny, nz):
+    shape = tuple(fh[name].shape)
+    if shape != (nz, ny, nx):
+        raise ValueError(
+            f"Dataset '{name}' in {fh.filename} has shape {shape}, expected "
+            f"(nz, ny, nx) = ({nz}, {ny}, {nx}) from system_setting.json.  Files "
+            f"written before 2026-05-26 are (nx, ny, nz): convert them with "
+            f"qpop_monitor.migrate_hdf5_layout.")
+
+
+def read_field(fh, name, nx, ny, nz, sl, dtype=np.float64):
+    """One component, cropped, in file order (z, y, x), converted to dtype."""
+    _checked(fh, name, nx, ny, nz)
+    return np.ascontiguousarray(read_dataset(fh, name, (sl[2], sl[1], sl[0]), dtype=dtype))
+
+
+def _prefix(fh, prefixes):
+    for p in prefixes:
+        if f"{p}_x" in fh:
+            return p
+    raise StopIteration(f"none of {prefixes} found in {fh.filename}")
+
+
+def read_vector(filepath, prefixes, nx, ny, nz, x_lim=None, y_lim=None, z_lim=None, dtype=np.float64):
+    """``{prefix}_x/y/z`` cropped, as (3, nz_s, ny_s, nx_s) in file order."""
+    sl = crop_slices(nx, ny, nz, x_lim, y_lim, z_lim)
+    with open_file(filepath, "r") as fh:
+        prefix = _prefix(fh, prefixes)
+        return np.stack([read_field(fh, f"{prefix}_{c}", nx, ny, nz, sl, dtype) for c in "xyz"])
+
+
+def _xyz(field_zyx):
+    return field_zyx.transpose(2, 1, 0)
+
+
+def load_platt_hdf5(filepath, nx, ny, nz, x_lim=None, y_lim=None, z_lim=None):
+    """P in C/m^2, cropped, (x, y, z) C order flattened as [Px | Py | Pz] (io_utils.py:54-60)."""
+    v = read_vector(filepath, ("P_Lattice", "PLatt"), nx, ny, nz, x_lim, y_lim, z_lim)
+    return np.stack([_xyz(c) for c in v]).ravel()
+
+
+def load_displacement_hdf5(filepath, nx, ny, nz, x_lim=None, y_lim=None, z_lim=None):
+    """u in nm (files hold metres), (x, y, z) C order flattened as [ux | uy | uz] (io_utils.py:63-69)."""
+    v = read_vector(filepath, ("Displacement",), nx, ny, nz, x_lim, y_lim, z_lim)
+    return np.stack([_xyz(c) for c in v]).ravel() / L0
+
+
+def load_phase_hdf5(filepath, nPhase, nx, ny, nz, x_lim=None, y_lim=None, z_lim=None):
+    """Phase fractions phase_1 .. phase_<nPhase>, cropped, as (nPhase, nx_s, ny_s, nz_s) (io_utils.py:72-78)."""
+    sl = crop_slices(nx, ny, nz, x_lim, y_lim, z_lim)
+    with open_file(filepath, "r") as fh:
+        return np.stack([_xyz(read_field(fh, f"phase_{p + 1}", nx, ny, nz, sl)) for p in range(nPhase)])
+
+
+def load_region_hdf5(filepath, nx, ny, nz, x_lim=None, y_lim=None, z_lim=None):
+    """Scattering weight ``region``, cropped, as (nx_s, ny_s, nz_s) (io_utils.py:81-86)."""
+    sl = crop_slices(nx, ny, nz, x_lim, y_lim, z_lim)
+    with open_file(filepath, "r") as fh:
+        return _xyz(read_field(fh, "region", nx, ny, nz, sl))
+
+
+def read_material_field(filepath, nx, ny, nz):
+    """materialField, uncropped, in file order (z, y, x), as int64."""
+    full = crop_slices(nx, ny, nz, None, None, None)
+    with open_file(filepath, "r") as fh:
+        return read_field(fh, "materialField", nx, ny, nz, full, np.float64).astype(np.int64)
+
+
+def load_material_field(filepath, nx, ny, nz):
+    """materialField uncropped, integer, (x, y, z) order (io_utils.py:89-92)."""
+    return _xyz(read_material_field(filepath, nx, ny, nz))
+
+
+# -- output -------------------------------------------------------------------
+
+def write_diffraction_axes(base_name, qx, qy, qz, step):
+    """``{base}.h5`` without /intensity (attributes and q axes) plus the XDMF sidecar (io_utils.py:108-141)."""
+    nx, ny, nz = len(qx), len(qy), len(qz)
+    print(f"Writing diffraction output → {base_name}.h5")
+    with open_file(f"{base_name}.h5", "w") as fh:
+        fh.attrs["Timestep"] = step
+        fh.attrs["q_units"] = "nm^-1"
+        fh["q_x"] = np.asarray(qx, dtype=np.float64)
+        fh["q_y"] = np.asarray(qy, dtype=np.float64)
+        fh["q_z"] = np.asarray(qz, dtype=np.float64)
+
+    h5 = os.path.basename(f"{base_name}.h5")
+
+    def item(dims, path):
+        return f'<DataItem Dimensions="{dims}" NumberType="Float" Precision="8" Format="HDF">{h5}:/{path}</DataItem>'
+
+    xdmf = "\n".join([
+        '<?xml version="1.0" ?>',
+        '<Xdmf Version="3.0">',
+        '  <Domain>',
+        '    <Grid Name="DiffractionPattern" GridType="Uniform">',
+        f'      <Topology TopologyType="3DRectMesh" Dimensions="{nz} {ny} {nx}"/>',
+        '      <Geometry GeometryType="VXVYVZ">',
+        f'        {item(nx, "q_x")}',
+        f'        {item(ny, "q_y")}',
+        f'        {item(nz, "q_z")}',
+        '      </Geometry>',
+        '      <Attribute Name="Intensity" AttributeType="Scalar" Center="Node">',
+        f'        {item(f"{nz} {ny} {nx}", "intensity")}',
+        '      </Attribute>',
+        '    </Grid>',
+        '  </Domain>',
+        '</Xdmf>',
+        '',
+    ])
+    with open(f"{base_name}.xdmf", "w", encoding="utf-8") as fh:
+        fh.write(xdmf)
+    print(f"Writing ParaView sidecar  → {base_name}.xdmf")
+
+
+def write_diffraction_output(base_name, I, qx, qy, qz, step, zyx=False):
+    """Write ``{base}.h5`` and ``{base}.xdmf`` (io_utils.py:95-105).
+
+    I is (nx, ny, nz) as ``diffraction()`` returns it, or already (nz, ny, nx)
+    when ``zyx`` is true; /intensity is stored (nz, ny, nx).
+    """
+    write_diffraction_axes(base_name, qx, qy, qz, step)
+    data = I if zyx else np.ascontiguousarray(I.transpose(2, 1, 0))
+    with open_file(f"{base_name}.h5", "a") as fh:
+        fh["intensity"] = np.asarray(data, dtype=np.float64)
+
+
+def load_diffraction_hdf5(filepath) -> dict:
+    """Read a diffraction.h5 back: intensity as (1, nx, ny, nz) and the q grid (io_utils.py:186-207)."""
+    with open_file(filepath, "r") as fh:
+        I = read_dataset(fh, "intensity", dtype=np.float64).transpose(2, 1, 0)
+        q = [read_dataset(fh, f"q_{c}", dtype=np.float64) for c in "xyz"]
+        units = fh.attrs["q_units"]
+    if isinstance(units, bytes):
+        units = units.decode()
+    shape = I.shape
+    grids = [np.broadcast_to(qd.reshape([-1 if d == i else 1 for i in range(3)]), shape) for d, qd in enumerate(q)]
+    return {
+        "dims": shape,
+        "intensity": I[np.newaxis],
+        "log10_intensity": np.log10(I)[np.newaxis],
+        "q_x": q[0], "q_y": q[1], "q_z": q[2],
+        "q_vector": np.stack(grids),
+        "q_units": str(units),
+    }

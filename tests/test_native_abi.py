@@ -1,0 +1,58 @@
+"""libqpd.so loads and exports every entry point include/qpd.h declares (CPU, no compute)."""
+
+import ctypes
+import re
+from pathlib import Path
+
+import pytest
+
+from qpop_diffraction import _native
+
+HEADER = Path(__file__).resolve().parents[1] / "include" / "qpd.h"
+
+
+def declared():
+    text = HEADER.read_text()
+    return sorted(set(re.findall(r"^(?:int|size_t|double|const char\*)\s+(qpd_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_header_declares_the_abi():
+    names = declared()
+    assert set(_native.EXPORTS) == set(names), (names, _native.EXPORTS)
+
+
+@pytest.mark.skipif(not _native.LIB_PATH.exists(), reason="libqpd.so not built")
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(str(_native.LIB_PATH))
+    for name in declared():
+        assert hasattr(lib, name), name
+
+
+@pytest.mark.skipif(not _native.LIB_PATH.exists(), reason="libqpd.so not built")
+def test_host_only_entry_points():
+    lib = _native.load_library()
+    assert lib.qpd_version() >= 100
+    for n, ok in [(1, 1), (2, 1), (104, 1), (4000, 1), (68, 1), (19, 0), (2 * 23, 0), (65536, 1), (65536 * 2, 0)]:
+        assert lib.qpd_fft_length_supported(n) == ok, n
+    assert lib.qpd_last_error() is not None
+
+
+@pytest.mark.skipif(not _native.LIB_PATH.exists(), reason="libqpd.so not built")
+def test_plan_rejects_bad_arguments():
+    """Argument checks run on the host before any device call."""
+    lib = _native.load_library()
+    handle = ctypes.c_void_p()
+    grid = _native.Grid((ctypes.c_int64 * 3)(4, 4, 4), (ctypes.c_int * 3)(0, 0, 1), _native.c_double3(1, 1, 1))
+    cr = _native.Crystal()
+    cr.n_atom, cr.n_phase, cr.n_struc = 1, 1, 3
+    rc = lib.qpd_plan_create(ctypes.byref(handle), ctypes.byref(grid), ctypes.byref(cr), 32)
+    assert rc == -1 and b"permutation" in lib.qpd_last_error()
+    grid.axis[1] = 1
+    grid.axis[2] = 2
+    rc = lib.qpd_plan_create(ctypes.byref(handle), ctypes.byref(grid), ctypes.byref(cr), 16)
+    assert rc == -1 and b"precision" in lib.qpd_last_error()
+    rc = lib.qpd_plan_create(ctypes.byref(handle), ctypes.byref(grid), ctypes.byref(cr), 32)
+    assert rc == -1 and b"crystal" in lib.qpd_last_error()  # null tables
+    cr.n_struc = 2
+    rc = lib.qpd_plan_create(ctypes.byref(handle), ctypes.byref(grid), ctypes.byref(cr), 32)
+    assert rc == -2
