@@ -9,6 +9,8 @@
 #include <complex>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
+#include <type_traits>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -17,6 +19,7 @@
 
 #include "qpd.h"
 #include "qpd_kernels.hpp"
+#include "qpd_fast.hpp"
 
 using namespace qpd;
 
@@ -102,6 +105,16 @@ double shape_factor(double k, double h) {
     return (double)(sinl(x) / x);
 }
 
+// The compile-time-length kernels are used unless QPD_GENERIC_FFT=1 (for comparison runs).
+bool fast_enabled() {
+    static int v = -1;
+    if (v < 0) {
+        const char* e = std::getenv("QPD_GENERIC_FFT");
+        v = (e && e[0] == '1') ? 0 : 1;
+    }
+    return v == 1;
+}
+
 int max_lds_bytes() {
     static int cached = -1;
     if (cached < 0) {
@@ -118,6 +131,7 @@ int max_lds_bytes() {
 // ---------------------------------------------------------------------------
 // Launch shape of one pass.
 struct Launch {
+    int fast;        // 1: compile-time-length kernel of qpd_fast.hpp (FP32), 0: generic
     int contiguous;  // 1: pass_contig, 0: pass_strided
     int nl;          // rows (contig) or TX (strided) per block
     int nf;          // inputs per LDS fill
@@ -203,6 +217,7 @@ struct qpd_plan {
     int axis[3];
     int64_t ncell;
     int n_phase;
+    int n_atom;
     int nY, nZ;  // classes after pass 0 and pass 1
     size_t csize;
     AxisFFT fft[3];
@@ -411,11 +426,89 @@ hipError_t launch_one(void (*kernel)(A), const Launch& L, hipStream_t st, const 
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Compile-time-length kernels (FP32).  Roles: 0 contiguous + source (4 inputs, 3
+// outputs), 1 contiguous plain transform, 2 strided 3 -> 2, 3 strided 2 -> intensity,
+// 4 strided plain transform.  G = rows or columns per block.
+constexpr int kMaxAtomTab = 1024;  // atom-table entries the source pass keeps in LDS
+using S512 = Sizes<512, 8, 8, 8>;
+using S256 = Sizes<256, 4, 8, 8>;
+using S128 = Sizes<128, 4, 4, 8>;
+using S104 = Sizes<104, 8, 13>;
+using S64 = Sizes<64, 8, 8>;
+
+constexpr int fast_g(int role, int tpl) {
+    // source pass: 4 G TPL threads, about 512; plain contiguous: about 256 threads;
+    // strided: 16 columns (128-byte rows of FP32 complex) when the block stays within
+    // 512 threads, else 8
+    return role == 0 ? (tpl >= 128 ? 1 : 128 / tpl)
+                     : role == 1 ? (tpl >= 256 ? 1 : 256 / tpl) : (16 * tpl <= 512 ? 16 : 8);
+}
+
+template <class SZ>
+bool fast_shape_sz(int role, int64_t lines, Launch& L) {
+    const int G = fast_g(role, SZ::TPL);
+    const int F = role == 0 ? 4 : 1;  // inputs held in the tile at once
+    L.fast = 1;
+    L.contiguous = role <= 1;
+    L.nl = G;
+    L.nf = F;
+    L.nthr = (role == 0 ? 4 : 1) * G * SZ::TPL;
+    L.ept = 0;
+    // two stage buffers + the twiddle table (+ the atom table of the source pass)
+    const size_t tile = role <= 1 ? (size_t)F * G * SZ::PITCH : (size_t)SZ::N * (F * G + 1);
+    L.lds = (2 * tile + SZ::N) * 8 + (role == 0 ? 4 * kMaxAtomTab : 0);
+    (void)lines;
+    return L.nthr <= 1024 && L.lds <= 160 * 1024;
+}
+
+bool fast_shape(int64_t N, int role, Launch& L) {
+    switch (N) {
+        case 512: return fast_shape_sz<S512>(role, 0, L);
+        case 256: return fast_shape_sz<S256>(role, 0, L);
+        case 128: return fast_shape_sz<S128>(role, 0, L);
+        case 104: return fast_shape_sz<S104>(role, 0, L);
+        case 64: return fast_shape_sz<S64>(role, 0, L);
+        default: return false;
+    }
+}
+
+template <class SZ>
+hipError_t launch_fast_sz(int role, const Launch& L, const PassArgs<float>& a, hipStream_t st) {
+    constexpr int T = SZ::TPL;
+    switch (role) {
+        case 0: return launch_one(fast_source<float, SZ, fast_g(0, T)>, L, st, a);
+        case 1: return launch_one(fast_contig<float, SZ, fast_g(1, T), 1, false>, L, st, a);
+        case 2: return launch_one(fast_strided<float, SZ, fast_g(2, T), 3, false>, L, st, a);
+        case 3: return launch_one(fast_strided<float, SZ, fast_g(3, T), 2, true>, L, st, a);
+        case 4: return launch_one(fast_strided<float, SZ, fast_g(4, T), 1, false>, L, st, a);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_fast(int N, int role, const Launch& L, const PassArgs<float>& a, hipStream_t st) {
+    switch (N) {
+        case 512: return launch_fast_sz<S512>(role, L, a, st);
+        case 256: return launch_fast_sz<S256>(role, L, a, st);
+        case 128: return launch_fast_sz<S128>(role, L, a, st);
+        case 104: return launch_fast_sz<S104>(role, L, a, st);
+        case 64: return launch_fast_sz<S64>(role, L, a, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
 // Kernel instance for a pass: contiguous passes compute the source (3 outputs) or
 // transform fields (1 output); strided passes have 2 outputs, or 1 when they
 // write the intensity or run a plain transform.
 template <typename T>
 hipError_t launch_pass(const Launch& L, const PassArgs<T>& a, bool source, bool intensity, hipStream_t st) {
+    if (L.fast) {
+        if constexpr (std::is_same<T, float>::value) {
+            const int role = L.contiguous ? (source ? 0 : 1) : (intensity ? 3 : (a.nout == 2 ? 2 : 4));
+            return launch_fast(a.N, role, L, a, st);
+        }
+        return hipErrorInvalidValue;
+    }
     if (L.contiguous) {
         if (source) {
             if (a.nout != 3 || L.ept > 4) return hipErrorInvalidValue;
@@ -435,12 +528,34 @@ hipError_t launch_pass(const Launch& L, const PassArgs<T>& a, bool source, bool 
 }
 
 // Launch shapes of the three passes of a plan with the given inputs per chunk.
-int plan_launches(const AxisFFT fft[3], const int64_t dims[3], const int nin[3], size_t csize, Launch out[3]) {
+int plan_launches(const AxisFFT fft[3], const int64_t dims[3], const int nin[3], size_t csize, Launch out[3],
+                  bool intensity_last = false, int atomtab_entries = 0) {
     const int hard = max_lds_bytes();
     const int soft = std::min(hard, 65536);
     for (int d = 0; d < 3; ++d) {
         const bool contig = d == 0;
         const int64_t lines = contig ? dims[1] * dims[2] : 0;
+        if (csize == 8 && fast_enabled()) {
+            int role = -1;
+            if (contig)
+                role = (nin[d] == 4 && atomtab_entries <= kMaxAtomTab) ? 0 : (nin[d] == 1 ? 1 : -1);
+            else if (nin[d] == 3)
+                role = 2;
+            else if (nin[d] == 2 && intensity_last && d == 2)
+                role = 3;
+            else if (nin[d] == 1)
+                role = 4;
+            if (role >= 0 && fast_shape(dims[d], role, out[d]) && (int)out[d].lds <= hard) {
+                if (contig) {
+                    out[d].blocks = (lines + out[d].nl - 1) / out[d].nl;
+                } else {
+                    const int64_t tiles0 = (dims[0] + out[d].nl - 1) / out[d].nl;
+                    out[d].blocks = tiles0 * (d == 1 ? dims[2] : dims[1]);
+                }
+                continue;
+            }
+        }
+        out[d].fast = 0;
         const int ept_max = contig ? (nin[d] > 1 ? 4 : 8) : 8;
         if (!choose_launch(fft[d], nin[d], contig, csize, lines, soft, hard, ept_max, out[d]))
             return fail(QPD_ERR_UNSUPPORTED, "no launch shape fits dim " + std::to_string(d) + " of length " +
@@ -500,6 +615,7 @@ int run_intensity(qpd_plan* p, const qpd_fields* f, double* I, void* work, hipSt
             a.qC1 = p->qC[1];
             a.qC2 = p->qC[2];
             a.n_phase = p->n_phase;
+            a.n_atom_tab = p->n_atom * p->n_phase * kAtomTab;
             a.ax0 = p->axis[0];
             a.ax1 = p->axis[1];
             a.ax2 = p->axis[2];
@@ -653,10 +769,11 @@ int qpd_plan_create(qpd_plan** plan, const qpd_grid* grid, const qpd_crystal* c,
         }
     }
     p->n_phase = c->n_phase;
+    p->n_atom = c->n_atom;
     p->csize = precision == 32 ? sizeof(cx<float>) : sizeof(cx<double>);
     for (int i = 0; i < 3; ++i) p->qC[i] = c->qC[i];
     const int nin[3] = {4, 3, 2};
-    int rc = plan_launches(p->fft, p->dims, nin, p->csize, p->launch);
+    int rc = plan_launches(p->fft, p->dims, nin, p->csize, p->launch, true, c->n_atom * c->n_phase * kAtomTab);
     if (rc == QPD_OK) rc = precision == 32 ? build_tables<float>(p, grid, c) : build_tables<double>(p, grid, c);
     if (rc != QPD_OK) {
         if (p->dev) (void)hipFree(p->dev);

@@ -1,0 +1,444 @@
+// Register-staged Stockham passes with compile-time transform lengths (gfx950).
+//
+// Same three passes and the same chunk / group / coefficient structure as
+// qpd_kernels.hpp, for lengths whose radix sequence is known at compile time.
+// A line of N points is handled by TPL = max_s(N / R_s) threads; thread j owns
+// butterfly j of every stage.  The first stage takes its inputs x[j + r N/R_0]
+// straight from global memory (or from the source computation) into registers,
+// the last stage leaves its outputs X[j + r N/R_last] in registers, where they are
+// multiplied by the chunk coefficient and accumulated; only the hand-offs between
+// stages go through LDS.  Each thread transforms the F inputs of a chunk at once,
+// so its twiddles are loaded once per stage and reused F times.
+//
+// LDS layouts (8-byte complex points, 32 four-byte banks):
+//   contiguous pass: line-major, point idx of line l at l * PITCH + idx + idx / 8
+//                    (the padding breaks the stride-R pattern of stage outputs);
+//   strided pass:    point-major, point idx of line l at idx * (L + 1) + l, with the
+//                    lanes of a wave walking consecutive columns (lines).
+#pragma once
+
+#include "qpd_kernels.hpp"
+
+namespace qpd {
+
+template <int N_, int R0_, int R1_, int R2_ = 1, int R3_ = 1>
+struct Sizes {
+    static constexpr int N = N_;
+    static constexpr int NST = R3_ > 1 ? 4 : (R2_ > 1 ? 3 : (R1_ > 1 ? 2 : 1));
+    static constexpr int R[4] = {R0_, R1_, R2_, R3_};
+    static constexpr int rad(int s) { return s == 0 ? R0_ : s == 1 ? R1_ : s == 2 ? R2_ : R3_; }
+    static constexpr int nr(int s) { return N_ / rad(s); }
+    static constexpr int ns(int s) { return s == 0 ? 1 : ns(s - 1) * rad(s - 1); }
+    static constexpr int cmax(int a, int b) { return a > b ? a : b; }
+    static constexpr int TPL = cmax(cmax(nr(0), NST > 1 ? nr(1) : 0), cmax(NST > 2 ? nr(2) : 0, NST > 3 ? nr(3) : 0));
+    static constexpr int RMAX = cmax(cmax(R0_, R1_), cmax(R2_, R3_));
+    static constexpr int RL = rad(NST - 1);
+    static constexpr int NRL = N_ / RL;
+    static constexpr int PITCH = N_ + N_ / 8 + 1;  // contiguous layout: padded line pitch
+    static_assert(R0_ * (R1_ > 1 ? R1_ : 1) * (R2_ > 1 ? R2_ : 1) * (R3_ > 1 ? R3_ : 1) == N_, "radices must multiply to N");
+};
+
+template <bool STRIDED, int PITCH, int LP>
+__device__ __forceinline__ int lds_addr(int line, int idx) {
+    if constexpr (STRIDED)
+        return idx * LP + line;
+    else
+        return line * PITCH + idx + (idx >> 3);
+}
+
+// Stage s on F lines held in registers v[f][r] (inputs of butterfly j already loaded):
+// twiddle (shared by the F lines), then DFT_R.
+template <typename T, class SZ, int S, int F>
+__device__ __forceinline__ void stage_compute(cx<T> (&v)[F][SZ::RMAX], int j, const cx<T>* __restrict__ W) {
+    constexpr int R = SZ::rad(S);
+    constexpr int NS = SZ::ns(S);
+    constexpr int NR = SZ::nr(S);
+    T cr[R / 2 + 1], sr[R / 2 + 1];
+    if constexpr ((R & 1) == 1) {
+#pragma unroll
+        for (int m = 1; m <= R / 2; ++m) {
+            const cx<T> w = W[m * NR];
+            cr[m] = w.x;
+            sr[m] = -w.y;
+        }
+    }
+    if constexpr (NS > 1) {
+        constexpr int TWS = SZ::N / (NS * R);
+        const int k = j % NS;
+        cx<T> w[R];
+#pragma unroll
+        for (int r = 1; r < R; ++r) w[r] = W[r * k * TWS];
+#pragma unroll
+        for (int f = 0; f < F; ++f)
+#pragma unroll
+            for (int r = 1; r < R; ++r) v[f][r] = cmul(v[f][r], w[r]);
+    }
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+        cx<T> t[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) t[r] = v[f][r];
+        dft<R>(t, cr, sr);
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[f][r] = t[r];
+    }
+}
+
+// Write the outputs of stage S (butterfly j) to LDS at their Stockham positions.
+template <typename T, class SZ, int S, int F, bool STRIDED, int LP>
+__device__ __forceinline__ void stage_write(cx<T>* tile, const cx<T> (&v)[F][SZ::RMAX], int j, int line0, int dline) {
+    constexpr int R = SZ::rad(S), NS = SZ::ns(S);
+    const int o = (j / NS) * NS * R + (j % NS);
+#pragma unroll
+    for (int f = 0; f < F; ++f)
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            tile[lds_addr<STRIDED, SZ::PITCH, LP>(line0 + f * dline, o + r * NS)] = v[f][r];
+}
+
+// Read the inputs x[j + r N/R_S] of stage S from LDS.
+template <typename T, class SZ, int S, int F, bool STRIDED, int LP>
+__device__ __forceinline__ void stage_read(const cx<T>* tile, cx<T> (&v)[F][SZ::RMAX], int j, int line0, int dline) {
+    constexpr int R = SZ::rad(S), NR = SZ::nr(S);
+#pragma unroll
+    for (int f = 0; f < F; ++f)
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[f][r] = tile[lds_addr<STRIDED, SZ::PITCH, LP>(line0 + f * dline, j + r * NR)];
+}
+
+// Stages 0 .. NST-1 of the F lines of butterfly j, with two LDS buffers X, Y for the
+// hand-offs between stages.  LOAD0: the stage-0 inputs are in X in natural order
+// (written before a barrier); otherwise they are already in v.  Stage outputs go
+// to the buffers in turn (Y, X, Y, ... with LOAD0; X, Y, X, ... without), with a
+// barrier between each write and the read that follows; the last stage leaves
+// X[j + r NRL] in v.  No barrier is pending on return.  The next call may write
+// the buffer this call read last only after a barrier, so callers alternate the
+// roles of X and Y between calls when the number of writes (chunk_writes) is odd.
+template <class SZ, bool LOAD0>
+constexpr int chunk_writes() {
+    return SZ::NST - 1 + (LOAD0 ? 1 : 0);
+}
+
+template <typename T, class SZ, int F, bool STRIDED, int LP, bool LOAD0>
+__device__ __forceinline__ void transform(cx<T>* X, cx<T>* Y, cx<T> (&v)[F][SZ::RMAX], int j, int line0, int dline,
+                                          const cx<T>* __restrict__ W) {
+    cx<T>* rd = X;  // buffer holding the current stage's inputs
+    cx<T>* wr = LOAD0 ? Y : X;
+    const bool a0 = j < SZ::nr(0);
+    if constexpr (LOAD0) {
+        if (a0) stage_read<T, SZ, 0, F, STRIDED, LP>(rd, v, j, line0, dline);
+    }
+    if (a0) stage_compute<T, SZ, 0, F>(v, j, W);
+    if constexpr (SZ::NST > 1) {
+        if (a0) stage_write<T, SZ, 0, F, STRIDED, LP>(wr, v, j, line0, dline);
+        __syncthreads();
+        rd = wr;
+        wr = (wr == X) ? Y : X;
+        const bool a1 = j < SZ::nr(1);
+        if (a1) {
+            stage_read<T, SZ, 1, F, STRIDED, LP>(rd, v, j, line0, dline);
+            stage_compute<T, SZ, 1, F>(v, j, W);
+        }
+        if constexpr (SZ::NST > 2) {
+            if (a1) stage_write<T, SZ, 1, F, STRIDED, LP>(wr, v, j, line0, dline);
+            __syncthreads();
+            rd = wr;
+            wr = (wr == X) ? Y : X;
+            const bool a2 = j < SZ::nr(2);
+            if (a2) {
+                stage_read<T, SZ, 2, F, STRIDED, LP>(rd, v, j, line0, dline);
+                stage_compute<T, SZ, 2, F>(v, j, W);
+            }
+            if constexpr (SZ::NST > 3) {
+                if (a2) stage_write<T, SZ, 2, F, STRIDED, LP>(wr, v, j, line0, dline);
+                __syncthreads();
+                rd = wr;
+                const bool a3 = j < SZ::nr(3);
+                if (a3) {
+                    stage_read<T, SZ, 3, F, STRIDED, LP>(rd, v, j, line0, dline);
+                    stage_compute<T, SZ, 3, F>(v, j, W);
+                }
+            }
+        }
+    }
+}
+
+// Copy the twiddle table W[0, N) into LDS (caller synchronises before use).
+template <typename T, int N>
+__device__ __forceinline__ void load_twiddles(cx<T>* dst, const cx<T>* __restrict__ src, int tid, int nthr) {
+    for (int m = tid; m < N; m += nthr) dst[m] = src[m];
+}
+
+// Multiply the last-stage outputs by the chunk coefficient and add them to the
+// group's outputs (input 0 -> out 0, input 1 -> out 0 times -i k, input f -> out f-1).
+template <typename T, class SZ, int F, int NOUT>
+__device__ __forceinline__ void accumulate_regs(cx<T> (&acc)[NOUT][SZ::RL], const cx<T> (&v)[F][SZ::RMAX], int j,
+                                                const PassArgs<T>& a, int ct) {
+#pragma unroll
+    for (int r = 0; r < SZ::RL; ++r) {
+        const int idx = j + r * SZ::NRL;
+        const cx<T> cf = chunk_coef(a, ct, idx);
+        const cx<T> t0 = cmul(cf, v[0][r]);
+        acc[0][r] = acc[0][r] + t0;
+        if constexpr (F > 1) {
+            const T kv = a.k[idx];
+            acc[0][r] = acc[0][r] + cscale(negi(cmul(cf, v[1][r])), kv);
+        }
+#pragma unroll
+        for (int f = 2; f < F; ++f) acc[f - 1][r] = acc[f - 1][r] + cmul(cf, v[f][r]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Contiguous pass: block = G rows of N points; thread (row g = tid / TPL, j = tid % TPL).
+// SOURCE: the F = 4 inputs of a chunk are the source values of one atom; otherwise
+// they are F consecutive input fields (F = nin).
+template <typename T, class SZ, int G, int F, bool SOURCE>
+__global__ __launch_bounds__(G* SZ::TPL) void fast_contig(PassArgs<T> a) {
+    static_assert(!SOURCE, "the source pass uses fast_source");
+    constexpr int N = SZ::N, TPL = SZ::TPL, R0 = SZ::rad(0), NR0 = SZ::nr(0);
+    constexpr int NOUT = F > 1 ? F - 1 : 1;
+    constexpr int TILE = F * G * SZ::PITCH;
+    constexpr bool ALT = (chunk_writes<SZ, false>() & 1) == 1;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T>* bufA = reinterpret_cast<cx<T>*>(smem_raw);
+    cx<T>* bufB = bufA + TILE;
+    cx<T>* Wl = bufB + TILE;
+    const int tid = threadIdx.x;
+    const int g = tid / TPL, j = tid % TPL;
+    const uint32_t nrows = a.n1 * a.n2;
+    const uint32_t row = blockIdx.x * G + g;
+    const bool row_ok = row < nrows;
+    const uint32_t cell0 = row * (uint32_t)N;
+    load_twiddles<T, N>(Wl, a.W, tid, G * TPL);
+    __syncthreads();
+    int par = 0;
+    for (int grp = 0; grp < a.ngroups; ++grp) {
+        cx<T> acc[NOUT][SZ::RL];
+#pragma unroll
+        for (int o = 0; o < NOUT; ++o)
+#pragma unroll
+            for (int r = 0; r < SZ::RL; ++r) acc[o][r] = {(T)0, (T)0};
+        for (int c = a.group_first[grp]; c < a.group_first[grp + 1]; ++c) {
+            const int src = a.chunk_src[c];
+            const int ct = a.chunk_coef[c];
+            cx<T> v[F][SZ::RMAX];
+            if (j < NR0) {
+                const cx<T>* in = a.in + (size_t)src * a.nin * a.ncell + cell0 + j;
+#pragma unroll
+                for (int r = 0; r < R0; ++r)
+#pragma unroll
+                    for (int f = 0; f < F; ++f)
+                        v[f][r] = row_ok ? in[(size_t)f * a.ncell + r * NR0] : cx<T>{(T)0, (T)0};
+            }
+            transform<T, SZ, F, false, 0, false>(par ? bufB : bufA, par ? bufA : bufB, v, j, g, G, Wl);
+            if (ALT) par ^= 1;
+            if (j < SZ::NRL) accumulate_regs<T, SZ, F, NOUT>(acc, v, j, a, ct);
+        }
+        if (row_ok && j < SZ::NRL) {
+            cx<T>* o = a.out + (size_t)grp * NOUT * a.ncell + cell0 + j;
+#pragma unroll
+            for (int q = 0; q < NOUT; ++q)
+#pragma unroll
+                for (int r = 0; r < SZ::RL; ++r) o[(size_t)q * a.ncell + r * SZ::NRL] = acc[q][r];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Contiguous pass with the source (pass 0): block = G rows of N cells, 4 * G * TPL
+// threads.  For each atom the block first computes the four source values of its
+// cells (about R0 / 4 cells per thread, inputs cached in registers) into the tile
+// in natural order; then thread (f, row, j) transforms line (f, row) and keeps the
+// contribution of input f to its output: f = 0 and f = 1 (times -i k) both feed
+// output 0 and are summed through LDS once per group; f = 2, 3 feed outputs 1, 2.
+template <typename T, class SZ, int G>
+__global__ __launch_bounds__(4 * G * SZ::TPL) void fast_source(PassArgs<T> a) {
+    constexpr int N = SZ::N, TPL = SZ::TPL, NRL = SZ::NRL, RL = SZ::RL;
+    constexpr int NTHR = 4 * G * TPL;
+    constexpr int CPT = (G * N + NTHR - 1) / NTHR;  // cells per thread in the source phase
+    constexpr int TILE = 4 * G * SZ::PITCH;
+    constexpr bool ALT = (chunk_writes<SZ, true>() & 1) == 1;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T>* bufA = reinterpret_cast<cx<T>*>(smem_raw);
+    cx<T>* bufB = bufA + TILE;
+    cx<T>* Wl = bufB + TILE;
+    T* tabl = reinterpret_cast<T*>(Wl + N);  // atom table (n_atom * n_phase * kAtomTab)
+    const int tid = threadIdx.x;
+    const int f = tid / (G * TPL);
+    const int g = (tid / TPL) % G;
+    const int j = tid % TPL;
+    const uint32_t nrows = a.n1 * a.n2;
+    const uint32_t row0 = blockIdx.x * G;
+    const uint32_t cell0 = row0 * (uint32_t)N;
+    const uint32_t row = row0 + g;
+    load_twiddles<T, N>(Wl, a.W, tid, NTHR);
+    const int ntab = a.n_atom_tab;
+    for (int m = tid; m < ntab; m += NTHR) tabl[m] = a.atomtab[m];
+
+    // source cells of this thread: block cell e = tid + m * NTHR -> (row e / N, x e % N)
+    CellInputs<T> ci[CPT];
+#pragma unroll
+    for (int m = 0; m < CPT; ++m) {
+        const int e = tid + m * NTHR;
+        ci[m].ph = -1;
+        ci[m].rg = (T)1;
+        ci[m].P0 = ci[m].P1 = ci[m].P2 = (T)0;
+        ci[m].U0 = ci[m].U1 = ci[m].U2 = 0.0;
+        if (e < G * N && row0 + e / N < nrows) {
+            const uint32_t off = cell0 + e;
+            ci[m].ph = a.phase[off];
+            if (a.region) ci[m].rg = a.region[off];
+            ci[m].P0 = a.P[off];
+            ci[m].P1 = a.P[a.ncell + off];
+            ci[m].P2 = a.P[(size_t)2 * a.ncell + off];
+            ci[m].U0 = a.U[off];
+            ci[m].U1 = a.U[a.ncell + off];
+            ci[m].U2 = a.U[(size_t)2 * a.ncell + off];
+        }
+    }
+    __syncthreads();
+
+    int par = 0;
+    for (int grp = 0; grp < a.ngroups; ++grp) {
+        cx<T> acc[RL];
+#pragma unroll
+        for (int r = 0; r < RL; ++r) acc[r] = {(T)0, (T)0};
+        for (int c = a.group_first[grp]; c < a.group_first[grp + 1]; ++c) {
+            const int src = a.chunk_src[c];
+            const int ct = a.chunk_coef[c];
+            cx<T>* X = par ? bufB : bufA;
+            cx<T>* Y = par ? bufA : bufB;
+            if (ALT) par ^= 1;
+#pragma unroll
+            for (int m = 0; m < CPT; ++m) {
+                const int e = tid + m * NTHR;
+                if (e < G * N) {
+                    cx<T> v0, v1, v2, v3;
+                    cell_source(a, tabl, ci[m], cell0 + e, src, v0, v1, v2, v3);
+                    const int rr = e / N, x = e - rr * N;
+                    X[lds_addr<false, SZ::PITCH, 0>(0 * G + rr, x)] = v0;
+                    X[lds_addr<false, SZ::PITCH, 0>(1 * G + rr, x)] = v1;
+                    X[lds_addr<false, SZ::PITCH, 0>(2 * G + rr, x)] = v2;
+                    X[lds_addr<false, SZ::PITCH, 0>(3 * G + rr, x)] = v3;
+                }
+            }
+            __syncthreads();
+            cx<T> v[1][SZ::RMAX];
+            transform<T, SZ, 1, false, 0, true>(X, Y, v, j, f * G + g, G, Wl);
+            if (j < NRL) {
+#pragma unroll
+                for (int r = 0; r < RL; ++r) {
+                    const int idx = j + r * NRL;
+                    cx<T> t = cmul(chunk_coef(a, ct, idx), v[0][r]);
+                    if (f == 1) t = cscale(negi(t), a.k[idx]);
+                    acc[r] = acc[r] + t;
+                }
+            }
+        }
+        // out0 = (f = 0) + (f = 1): the f = 1 threads pass their sums through LDS
+        cx<T>* ex = bufA;
+        __syncthreads();
+        if (f == 1 && j < NRL) {
+#pragma unroll
+            for (int r = 0; r < RL; ++r) ex[lds_addr<false, SZ::PITCH, 0>(g, j + r * NRL)] = acc[r];
+        }
+        __syncthreads();
+        if (row < nrows && j < NRL && f != 1) {
+            const int q = f == 0 ? 0 : f - 1;
+            cx<T>* o = a.out + ((size_t)grp * 3 + q) * a.ncell + cell0 + (uint32_t)g * N + j;
+#pragma unroll
+            for (int r = 0; r < RL; ++r) {
+                cx<T> val = acc[r];
+                if (f == 0) val = val + ex[lds_addr<false, SZ::PITCH, 0>(g, j + r * NRL)];
+                o[r * NRL] = val;
+            }
+        }
+        __syncthreads();
+        par = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Strided pass: block = G consecutive i0 (columns) x one outer index; thread
+// (column g = tid % G, j = tid / G).  INTENSITY: write fftshifted max(|A|^2, 1e-50).
+template <typename T, class SZ, int G, int NIN, bool INTENSITY>
+__global__ __launch_bounds__(G* SZ::TPL) void fast_strided(PassArgs<T> a) {
+    // the NIN inputs of a chunk are transformed one after another (one line per thread)
+    constexpr int N = SZ::N, R0 = SZ::rad(0), NR0 = SZ::nr(0), LP = G + 1;
+    constexpr int NOUT = NIN > 1 ? NIN - 1 : 1;
+    constexpr int TILE = N * LP;
+    constexpr bool ALT = (chunk_writes<SZ, false>() & 1) == 1;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T>* bufA = reinterpret_cast<cx<T>*>(smem_raw);
+    cx<T>* bufB = bufA + TILE;
+    cx<T>* Wl = bufB + TILE;
+    const int tid = threadIdx.x;
+    const int g = tid % G, j = tid / G;
+    const uint32_t t0 = blockIdx.x % a.ntiles0;
+    const uint32_t outer = blockIdx.x / a.ntiles0;
+    const uint32_t i0 = t0 * G + g;
+    const bool col_ok = i0 < a.n0;
+    const uint32_t colbase = i0 + outer * a.OS;
+    load_twiddles<T, N>(Wl, a.W, tid, G * SZ::TPL);
+    __syncthreads();
+    int par = 0;
+    for (int grp = 0; grp < a.ngroups; ++grp) {
+        cx<T> acc[NOUT][SZ::RL];
+#pragma unroll
+        for (int o = 0; o < NOUT; ++o)
+#pragma unroll
+            for (int r = 0; r < SZ::RL; ++r) acc[o][r] = {(T)0, (T)0};
+        for (int c = a.group_first[grp]; c < a.group_first[grp + 1]; ++c) {
+            const int src = a.chunk_src[c];
+            const int ct = a.chunk_coef[c];
+#pragma unroll
+            for (int f = 0; f < NIN; ++f) {
+                cx<T> v[1][SZ::RMAX];
+                if (j < NR0) {
+                    const cx<T>* in = a.in + ((size_t)src * a.nin + f) * a.ncell + colbase;
+#pragma unroll
+                    for (int r = 0; r < R0; ++r)
+                        v[0][r] = col_ok ? in[(uint32_t)(j + r * NR0) * a.S] : cx<T>{(T)0, (T)0};
+                }
+                transform<T, SZ, 1, true, LP, false>(par ? bufB : bufA, par ? bufA : bufB, v, j, g, 0, Wl);
+                if (ALT) par ^= 1;
+                if (j < SZ::NRL) {
+#pragma unroll
+                    for (int r = 0; r < SZ::RL; ++r) {
+                        const int idx = j + r * SZ::NRL;
+                        cx<T> t = cmul(chunk_coef(a, ct, idx), v[0][r]);
+                        if (f == 1) t = cscale(negi(t), a.k[idx]);
+                        const int q = f == 0 ? 0 : (f == 1 ? 0 : f - 1);
+                        acc[q][r] = acc[q][r] + t;
+                    }
+                }
+            }
+        }
+        if (!col_ok || j >= SZ::NRL) continue;
+        if constexpr (INTENSITY) {
+            const uint32_t s0 = (i0 + a.n0 / 2) >= a.n0 ? i0 + a.n0 / 2 - a.n0 : i0 + a.n0 / 2;
+            const uint32_t nA = a.dim == 2 ? a.n2 : a.n1;
+            const uint32_t nB = a.dim == 2 ? a.n1 : a.n2;
+            const uint32_t sB = (outer + nB / 2) >= nB ? outer + nB / 2 - nB : outer + nB / 2;
+#pragma unroll
+            for (int r = 0; r < SZ::RL; ++r) {
+                const uint32_t idx = j + r * SZ::NRL;
+                const uint32_t sA = (idx + nA / 2) >= nA ? idx + nA / 2 - nA : idx + nA / 2;
+                const uint32_t s1 = a.dim == 2 ? sB : sA;
+                const uint32_t s2 = a.dim == 2 ? sA : sB;
+                const double re = (double)acc[0][r].x, im = (double)acc[0][r].y;
+                a.I[(size_t)s0 + (size_t)a.n0 * ((size_t)s1 + (size_t)a.n1 * s2)] = fmax(re * re + im * im, 1e-50);
+            }
+        } else {
+            cx<T>* o = a.out + (size_t)grp * NOUT * a.ncell + colbase;
+#pragma unroll
+            for (int q = 0; q < NOUT; ++q)
+#pragma unroll
+                for (int r = 0; r < SZ::RL; ++r)
+                    o[(size_t)q * a.ncell + (uint32_t)(j + r * SZ::NRL) * a.S] = acc[q][r];
+        }
+    }
+}
+
+}  // namespace qpd

@@ -68,6 +68,7 @@ struct PassArgs {
     const T* atomtab;         // [atom][phase][kAtomTab]
     double qC0, qC1, qC2;
     int n_phase;
+    int n_atom_tab;           // entries of atomtab (n_atom * n_phase * kAtomTab)
     int ax0, ax1, ax2;        // physical axis of memory dims 0, 1, 2
 };
 
@@ -95,8 +96,8 @@ struct CellInputs {
 };
 
 template <typename T>
-__device__ __forceinline__ void cell_source(const PassArgs<T>& a, const CellInputs<T>& ci, uint32_t off, int atom,
-                                            cx<T>& E, cx<T>& V0, cx<T>& V1, cx<T>& V2) {
+__device__ __forceinline__ void cell_source(const PassArgs<T>& a, const T* atomtab, const CellInputs<T>& ci,
+                                            uint32_t off, int atom, cx<T>& E, cx<T>& V0, cx<T>& V1, cx<T>& V2) {
     E = V0 = V1 = V2 = {(T)0, (T)0};
     if (ci.ph < 0) return;
     const int p_lo = a.fraction ? 0 : ci.ph;
@@ -105,7 +106,7 @@ __device__ __forceinline__ void cell_source(const PassArgs<T>& a, const CellInpu
         T w = ci.rg;
         if (a.fraction) w *= a.fraction[(size_t)p * a.ncell + off];
         if (w == (T)0) continue;
-        const T* tab = a.atomtab + (atom * a.n_phase + p) * kAtomTab;
+        const T* tab = atomtab + (atom * a.n_phase + p) * kAtomTab;
         const double ux = ci.U0 + (double)(tab[2] * ci.P0 + tab[3] * ci.P1 + tab[4] * ci.P2);
         const double uy = ci.U1 + (double)(tab[5] * ci.P0 + tab[6] * ci.P1 + tab[7] * ci.P2);
         const double uz = ci.U2 + (double)(tab[8] * ci.P0 + tab[9] * ci.P1 + tab[10] * ci.P2);
@@ -208,7 +209,7 @@ __global__ __launch_bounds__(512) void pass_contig(PassArgs<T> a) {
                     cx<T>* dst = tile + idx * stride + row;
                     if constexpr (SOURCE) {
                         cx<T> v0, v1, v2, v3;
-                        cell_source(a, ci[m], cell0 + (uint32_t)e, src, v0, v1, v2, v3);
+                        cell_source(a, a.atomtab, ci[m], cell0 + (uint32_t)e, src, v0, v1, v2, v3);
                         for (int f = 0; f < nfc; ++f) {
                             const int jj = j0 + f;
                             dst[f * nl] = jj == 0 ? v0 : (jj == 1 ? v1 : (jj == 2 ? v2 : v3));
