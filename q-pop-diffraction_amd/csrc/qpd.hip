@@ -375,6 +375,11 @@ int build_tables(qpd_plan* p, const qpd_grid* g, const qpd_crystal* c) {
             for (int d = 0; d < 3; ++d)
                 for (int kk = 0; kk < 3; ++kk)
                     at.push_back(c->su[((d * c->n_struc + kk) * c->n_phase + ph) * c->n_atom + a]);
+            for (int kk = 0; kk < 3; ++kk) {  // g_k = sum_d qC_d su[d][k], in FP64
+                double gk = 0.0;
+                for (int d = 0; d < 3; ++d) gk += c->qC[d] * c->su[((d * c->n_struc + kk) * c->n_phase + ph) * c->n_atom + a];
+                at.push_back(gk);
+            }
         }
     offAtom = tb.add_real<T>(at);
 
@@ -438,15 +443,15 @@ using S104 = Sizes<104, 8, 13>;
 using S64 = Sizes<64, 8, 8>;
 
 constexpr int fast_g(int role, int tpl) {
-    // source pass: 4 G TPL threads, about 512; plain contiguous: about 256 threads;
+    // source pass: 4 G TPL threads, about 256; plain contiguous: about 256 threads;
     // strided: 16 columns (128-byte rows of FP32 complex) when the block stays within
     // 512 threads, else 8
-    return role == 0 ? (tpl >= 128 ? 1 : 128 / tpl)
+    return role == 0 ? (tpl >= 64 ? 1 : 64 / tpl)
                      : role == 1 ? (tpl >= 256 ? 1 : 256 / tpl) : (16 * tpl <= 512 ? 16 : 8);
 }
 
 template <class SZ>
-bool fast_shape_sz(int role, int64_t lines, Launch& L) {
+bool fast_shape_sz(int role, int atomtab_entries, Launch& L) {
     const int G = fast_g(role, SZ::TPL);
     const int F = role == 0 ? 4 : 1;  // inputs held in the tile at once
     L.fast = 1;
@@ -457,18 +462,17 @@ bool fast_shape_sz(int role, int64_t lines, Launch& L) {
     L.ept = 0;
     // two stage buffers + the twiddle table (+ the atom table of the source pass)
     const size_t tile = role <= 1 ? (size_t)F * G * SZ::PITCH : (size_t)SZ::N * (F * G + 1);
-    L.lds = (2 * tile + SZ::N) * 8 + (role == 0 ? 4 * kMaxAtomTab : 0);
-    (void)lines;
+    L.lds = (2 * tile + SZ::N) * 8 + (role == 0 ? 4 * atomtab_entries : 0);
     return L.nthr <= 1024 && L.lds <= 160 * 1024;
 }
 
-bool fast_shape(int64_t N, int role, Launch& L) {
+bool fast_shape(int64_t N, int role, int atomtab_entries, Launch& L) {
     switch (N) {
-        case 512: return fast_shape_sz<S512>(role, 0, L);
-        case 256: return fast_shape_sz<S256>(role, 0, L);
-        case 128: return fast_shape_sz<S128>(role, 0, L);
-        case 104: return fast_shape_sz<S104>(role, 0, L);
-        case 64: return fast_shape_sz<S64>(role, 0, L);
+        case 512: return fast_shape_sz<S512>(role, atomtab_entries, L);
+        case 256: return fast_shape_sz<S256>(role, atomtab_entries, L);
+        case 128: return fast_shape_sz<S128>(role, atomtab_entries, L);
+        case 104: return fast_shape_sz<S104>(role, atomtab_entries, L);
+        case 64: return fast_shape_sz<S64>(role, atomtab_entries, L);
         default: return false;
     }
 }
@@ -545,7 +549,7 @@ int plan_launches(const AxisFFT fft[3], const int64_t dims[3], const int nin[3],
                 role = 3;
             else if (nin[d] == 1)
                 role = 4;
-            if (role >= 0 && fast_shape(dims[d], role, out[d]) && (int)out[d].lds <= hard) {
+            if (role >= 0 && fast_shape(dims[d], role, atomtab_entries, out[d]) && (int)out[d].lds <= hard) {
                 if (contig) {
                     out[d].blocks = (lines + out[d].nl - 1) / out[d].nl;
                 } else {

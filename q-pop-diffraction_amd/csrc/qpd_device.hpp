@@ -74,6 +74,30 @@ __device__ __forceinline__ void sincos_fp64_reduced(double t, float& s, float& c
     c = ((quadrant + 1) & 2) ? -c0 : c0;
 }
 
+// sin and cos of a phase of a few radians given in FP32: quarter-turn reduction in
+// FP32 (absolute error about 1e-7 rad for |t| < 4), then the same series.
+__device__ __forceinline__ void sincos_small(float t, float& s, float& c) {
+    const float r = t * 0.636619772367581343f;
+    const float q = rintf(r);
+    const float x = fmaf(q, -1.57079637050628662f, t) + q * 4.37113883e-8f;  // t - q pi/2, two-part pi/2
+    const float x2 = x * x;
+    float ps = fmaf(x2, 1.0f / 362880.0f, -1.0f / 5040.0f);
+    ps = fmaf(x2, ps, 1.0f / 120.0f);
+    ps = fmaf(x2, ps, -1.0f / 6.0f);
+    ps = fmaf(x2, ps, 1.0f);
+    const float sx = x * ps;
+    float pc = fmaf(x2, -1.0f / 3628800.0f, 1.0f / 40320.0f);
+    pc = fmaf(x2, pc, -1.0f / 720.0f);
+    pc = fmaf(x2, pc, 1.0f / 24.0f);
+    pc = fmaf(x2, pc, -0.5f);
+    const float cx_ = fmaf(x2, pc, 1.0f);
+    const int quadrant = ((int)q) & 3;
+    const float s0 = (quadrant & 1) ? cx_ : sx;
+    const float c0 = (quadrant & 1) ? sx : cx_;
+    s = (quadrant & 2) ? -s0 : s0;
+    c = ((quadrant + 1) & 2) ? -c0 : c0;
+}
+
 __device__ __forceinline__ void phase_sincos(double t, float& s, float& c) { sincos_fp64_reduced(t, s, c); }
 __device__ __forceinline__ void phase_sincos(double t, double& s, double& c) { sincos(t, &s, &c); }
 

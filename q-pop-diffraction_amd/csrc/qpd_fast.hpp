@@ -17,6 +17,8 @@
 //                    lanes of a wave walking consecutive columns (lines).
 #pragma once
 
+#include <type_traits>
+
 #include "qpd_kernels.hpp"
 
 namespace qpd {
@@ -37,6 +39,66 @@ struct Sizes {
     static constexpr int PITCH = N_ + N_ / 8 + 1;  // contiguous layout: padded line pitch
     static_assert(R0_ * (R1_ > 1 ? R1_ : 1) * (R2_ > 1 ? R2_ : 1) * (R3_ > 1 ? R3_ : 1) == N_, "radices must multiply to N");
 };
+
+// Per-cell inputs of the FP32 source with the phase split exp(-i qC.u_a) =
+// exp(-i qC.U) exp(-i g_a.P): e0 = exp(-i qC.U) is formed once per cell with the
+// FP64-reduced sincos; per atom only g_a.P (a few radians) remains.
+struct CellInputsF {
+    float P0, P1, P2;
+    float u0, u1, u2;
+    cx<float> e0;
+    float rg;
+    int ph;
+};
+
+__device__ __forceinline__ void cell_load_f(const PassArgs<float>& a, uint32_t off, bool valid, CellInputsF& c) {
+    c.ph = -1;
+    c.rg = 1.0f;
+    c.P0 = c.P1 = c.P2 = 0.0f;
+    c.u0 = c.u1 = c.u2 = 0.0f;
+    c.e0 = {1.0f, 0.0f};
+    if (!valid) return;
+    c.ph = a.phase[off];
+    if (a.region) c.rg = a.region[off];
+    c.P0 = a.P[off];
+    c.P1 = a.P[a.ncell + off];
+    c.P2 = a.P[(size_t)2 * a.ncell + off];
+    const double U0 = a.U[off], U1 = a.U[a.ncell + off], U2 = a.U[(size_t)2 * a.ncell + off];
+    c.u0 = (float)U0;
+    c.u1 = (float)U1;
+    c.u2 = (float)U2;
+    float sn, cs;
+    sincos_fp64_reduced(a.qC0 * U0 + a.qC1 * U1 + a.qC2 * U2, sn, cs);
+    c.e0 = {cs, -sn};
+}
+
+// Source of one cell for one atom from CellInputsF (same values as cell_source).
+__device__ __forceinline__ void cell_source_f(const PassArgs<float>& a, const float* atomtab, const CellInputsF& ci,
+                                              uint32_t off, int atom, cx<float>& E, cx<float>& V0, cx<float>& V1,
+                                              cx<float>& V2) {
+    E = V0 = V1 = V2 = {0.0f, 0.0f};
+    if (ci.ph < 0) return;
+    const int p_lo = a.fraction ? 0 : ci.ph;
+    const int p_hi = a.fraction ? a.n_phase : ci.ph + 1;
+    for (int p = p_lo; p < p_hi; ++p) {
+        float w = ci.rg;
+        if (a.fraction) w *= a.fraction[(size_t)p * a.ncell + off];
+        if (w == 0.0f) continue;
+        const float* tab = atomtab + (atom * a.n_phase + p) * kAtomTab;
+        const float ux = ci.u0 + (tab[2] * ci.P0 + tab[3] * ci.P1 + tab[4] * ci.P2);
+        const float uy = ci.u1 + (tab[5] * ci.P0 + tab[6] * ci.P1 + tab[7] * ci.P2);
+        const float uz = ci.u2 + (tab[8] * ci.P0 + tab[9] * ci.P1 + tab[10] * ci.P2);
+        float sn, cs;
+        sincos_small(tab[11] * ci.P0 + tab[12] * ci.P1 + tab[13] * ci.P2, sn, cs);
+        // w f e0 e^{-i s}
+        const cx<float> fw = {tab[0] * w, tab[1] * w};
+        const cx<float> e = cmul(cmul(fw, ci.e0), cx<float>{cs, -sn});
+        E = E + e;
+        V0 = V0 + cscale(e, a.ax0 == 0 ? ux : (a.ax0 == 1 ? uy : uz));
+        V1 = V1 + cscale(e, a.ax1 == 0 ? ux : (a.ax1 == 1 ? uy : uz));
+        V2 = V2 + cscale(e, a.ax2 == 0 ? ux : (a.ax2 == 1 ? uy : uz));
+    }
+}
 
 template <bool STRIDED, int PITCH, int LP>
 __device__ __forceinline__ int lds_addr(int line, int idx) {
@@ -277,25 +339,12 @@ __global__ __launch_bounds__(4 * G * SZ::TPL) void fast_source(PassArgs<T> a) {
     for (int m = tid; m < ntab; m += NTHR) tabl[m] = a.atomtab[m];
 
     // source cells of this thread: block cell e = tid + m * NTHR -> (row e / N, x e % N)
-    CellInputs<T> ci[CPT];
+    static_assert(std::is_same<T, float>::value, "the fast source pass is FP32");
+    CellInputsF ci[CPT];
 #pragma unroll
     for (int m = 0; m < CPT; ++m) {
         const int e = tid + m * NTHR;
-        ci[m].ph = -1;
-        ci[m].rg = (T)1;
-        ci[m].P0 = ci[m].P1 = ci[m].P2 = (T)0;
-        ci[m].U0 = ci[m].U1 = ci[m].U2 = 0.0;
-        if (e < G * N && row0 + e / N < nrows) {
-            const uint32_t off = cell0 + e;
-            ci[m].ph = a.phase[off];
-            if (a.region) ci[m].rg = a.region[off];
-            ci[m].P0 = a.P[off];
-            ci[m].P1 = a.P[a.ncell + off];
-            ci[m].P2 = a.P[(size_t)2 * a.ncell + off];
-            ci[m].U0 = a.U[off];
-            ci[m].U1 = a.U[a.ncell + off];
-            ci[m].U2 = a.U[(size_t)2 * a.ncell + off];
-        }
+        cell_load_f(a, cell0 + e, e < G * N && row0 + e / N < nrows, ci[m]);
     }
     __syncthreads();
 
@@ -315,7 +364,7 @@ __global__ __launch_bounds__(4 * G * SZ::TPL) void fast_source(PassArgs<T> a) {
                 const int e = tid + m * NTHR;
                 if (e < G * N) {
                     cx<T> v0, v1, v2, v3;
-                    cell_source(a, tabl, ci[m], cell0 + e, src, v0, v1, v2, v3);
+                    cell_source_f(a, tabl, ci[m], cell0 + e, src, v0, v1, v2, v3);
                     const int rr = e / N, x = e - rr * N;
                     X[lds_addr<false, SZ::PITCH, 0>(0 * G + rr, x)] = v0;
                     X[lds_addr<false, SZ::PITCH, 0>(1 * G + rr, x)] = v1;

@@ -27,7 +27,8 @@
 
 namespace qpd {
 
-constexpr int kAtomTab = 11;  // per (atom, phase): Re f, Im f, su[d][k] (d, k = 0..2)
+// per (atom, phase): Re f, Im f, su[d][k] (d, k = 0..2), g[k] = sum_d qC_d su[d][k]
+constexpr int kAtomTab = 14;
 
 template <typename T>
 struct PassArgs {
