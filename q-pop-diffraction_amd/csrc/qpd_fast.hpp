@@ -431,6 +431,21 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided(PassArgs<T> a) {
     const uint32_t colbase = i0 + outer * a.OS;
     load_twiddles<T, N>(Wl, a.W, tid, G * SZ::TPL);
     __syncthreads();
+    // inputs are consumed in the order (chunk, input); on long lines the next one is
+    // prefetched into registers while the current one is transformed
+    constexpr bool PF = SZ::TPL >= 32;
+    const int nitems = a.group_first[a.ngroups] * NIN;
+    auto load_item = [&](int it, cx<T>(&dst)[1][SZ::RMAX]) {
+        const int c = it / NIN, f = it - (it / NIN) * NIN;
+        const cx<T>* in = a.in + ((size_t)a.chunk_src[c] * a.nin + f) * a.ncell + colbase;
+        if (j < NR0) {
+#pragma unroll
+            for (int r = 0; r < R0; ++r) dst[0][r] = col_ok ? in[(uint32_t)(j + r * NR0) * a.S] : cx<T>{(T)0, (T)0};
+        }
+    };
+    cx<T> nxt[1][SZ::RMAX];
+    if constexpr (PF) load_item(0, nxt);
+    int it = 0;
     int par = 0;
     for (int grp = 0; grp < a.ngroups; ++grp) {
         cx<T> acc[NOUT][SZ::RL];
@@ -439,16 +454,16 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided(PassArgs<T> a) {
 #pragma unroll
             for (int r = 0; r < SZ::RL; ++r) acc[o][r] = {(T)0, (T)0};
         for (int c = a.group_first[grp]; c < a.group_first[grp + 1]; ++c) {
-            const int src = a.chunk_src[c];
             const int ct = a.chunk_coef[c];
 #pragma unroll
-            for (int f = 0; f < NIN; ++f) {
+            for (int f = 0; f < NIN; ++f, ++it) {
                 cx<T> v[1][SZ::RMAX];
-                if (j < NR0) {
-                    const cx<T>* in = a.in + ((size_t)src * a.nin + f) * a.ncell + colbase;
+                if constexpr (PF) {
 #pragma unroll
-                    for (int r = 0; r < R0; ++r)
-                        v[0][r] = col_ok ? in[(uint32_t)(j + r * NR0) * a.S] : cx<T>{(T)0, (T)0};
+                    for (int r = 0; r < SZ::RMAX; ++r) v[0][r] = nxt[0][r];
+                    if (it + 1 < nitems) load_item(it + 1, nxt);
+                } else {
+                    load_item(it, v);
                 }
                 transform<T, SZ, 1, true, LP, false>(par ? bufB : bufA, par ? bufA : bufB, v, j, g, 0, Wl);
                 if (ALT) par ^= 1;
