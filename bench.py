@@ -37,6 +37,12 @@ def parse():
     ap.add_argument("--grid", type=int, nargs=3, default=(512, 512, 104), metavar=("NX", "NY", "NZ"))
     ap.add_argument("--cpu-crop", type=int, default=256, help="lateral size of the CPU-baseline crop (0: skip)")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched 48 snapshots x 4 reflections leg")
+    ap.add_argument("--batch-streams", type=int, default=4)
+    ap.add_argument("--no-serve", action="store_true", help="skip the served Poisson-stream leg")
+    ap.add_argument("--serve-requests", type=int, default=200)
+    ap.add_argument("--serve-rate", type=float, default=20.0, help="mean arrivals per second")
+    ap.add_argument("--serve-slots", type=int, default=4)
     return ap.parse_args()
 
 
@@ -95,6 +101,147 @@ def cpu_baseline(w, crop):
             "sample": f"FP64 NumPy oracle (restatement of the reference diffraction()) on the "
                       f"{cx}x{cy}x{nz} corner crop of the same volume, {dt:.1f} s; host has {os.cpu_count()} "
                       f"cores, numpy.fft and the elementwise steps use 1"}
+
+
+def max_dlog10(I, ref):
+    """max |log10 I - log10 ref| over voxels >= 1e-10 of the peak, and max |dI| / peak (device tensors)."""
+    import torch
+    peak = ref.max()
+    mask = ref >= 1e-10 * peak
+    dl = (torch.log10(I[mask]) - torch.log10(ref[mask])).abs().max().item()
+    return dl, ((I - ref).abs().max() / peak).item()
+
+
+def batch_leg(args):
+    """48 twin-stack snapshots x 4 reflections: one graph replay per batch, against one job at a time."""
+    import torch
+    from qpop_diffraction import workloads
+    from qpop_diffraction.batch import Batch, PlanCache, Reflection, plain_job, resident_job
+    grids = workloads.batch_grids(48, seed=3)
+    log(f"batch: generating 48 snapshots {grids[:4]} ...")
+    snaps = [workloads.twin_snapshot(("twin", i), L, L, d, seed=100 + i) for i, (L, d) in enumerate(grids)]
+    refls = [Reflection(h, workloads.pto_atoms(h)) for h in workloads.REFLECTIONS]
+    t0 = time.perf_counter()
+    b = Batch(snaps, refls, n_streams=args.batch_streams, graph=True)
+    t_plan = time.perf_counter() - t0
+    b.capture()
+    vox = sum(j.plan.n for j in b.jobs)
+    log(f"batch: {len(b.jobs)} jobs, {len(b.plans)} plans built in {t_plan:.2f} s, capture {b.stats.capture_s:.2f} s")
+    reps = 5
+    sec = time_steps(b.run, reps, 1)
+    # the same jobs one at a time: resident fields, cached plans, one stream, waiting for each
+    work = torch.empty(max(j.plan.workspace_bytes for j in b.jobs), dtype=torch.uint8, device="cuda")
+
+    def serial():
+        for j in b.jobs:
+            resident_job(b.plans, snaps[j.snapshot], j.reflection, refls[j.reflection], j.out, work)
+            torch.cuda.synchronize()
+    sec_serial = time_steps(serial, 2, 1)
+    # the plain form: what one drop-in call does per job (plan, buffers, intensity to host)
+    plain_job(snaps[0], refls[0])
+    t0 = time.perf_counter()
+    for j in b.jobs:
+        plain_job(snaps[j.snapshot], refls[j.reflection])
+    sec_plain = time.perf_counter() - t0
+    # accuracy: FP32 jobs against FP64 ones on the largest and smallest snapshots, every reflection
+    order = sorted(range(len(snaps)), key=lambda i: grids[i][0] ** 2 * grids[i][1])
+    worst = 0.0
+    plans64 = PlanCache(64)
+    b.run()
+    torch.cuda.synchronize()
+    for i in (order[0], order[-1]):
+        L, d = grids[i]
+        s64 = workloads.twin_snapshot(("twin", i), L, L, d, seed=100 + i, precision=64)
+        w64 = None
+        for r in range(len(refls)):
+            out64 = torch.empty(L * L * d, dtype=torch.float64, device="cuda")
+            plan = plans64.get(s64, r, refls[r])
+            if w64 is None or w64.numel() < plan.workspace_bytes:
+                w64 = torch.empty(plan.workspace_bytes, dtype=torch.uint8, device="cuda")
+            resident_job(plans64, s64, r, refls[r], out64, w64)
+            worst = max(worst, max_dlog10(b.result(i, r), out64)[0])
+        del s64, w64
+    n_jobs = len(b.jobs)
+    res = {
+        "workload": "48 two-variant 90-degree twin-stack snapshots (lateral 128-512 step 64, depth 24-104 from "
+                    f"{list(workloads.DEPTHS)}) x reflections 002, 103, 202, 113 = {n_jobs} volumes, FP32",
+        "grids": [list(g) for g in grids],
+        "volumes": n_jobs, "voxels": vox, "streams": b.n_streams, "plans": len(b.plans),
+        "graph_replay_ms": sec * 1e3, "volumes_per_s": n_jobs / sec, "voxels_per_s": vox / sec,
+        "serial_resident_volumes_per_s": n_jobs / sec_serial,
+        "plain_volumes_per_s": n_jobs / sec_plain,
+        "speedup_vs_plain": sec_plain / sec, "speedup_vs_serial_resident": sec_serial / sec,
+        "plain_form": "one job at a time as one drop-in diffraction() call does it on resident fields: plan "
+                      "built and destroyed, workspace and output allocated, float64 intensity copied to host, "
+                      "synchronous",
+        "max_abs_dlog10_I_vs_fp64": worst,
+        "dlog10_sample": "every reflection of the smallest and the largest snapshot",
+    }
+    log(f"batch: {res['volumes_per_s']:.1f} vol/s graph, {res['serial_resident_volumes_per_s']:.1f} serial, "
+        f"{res['plain_volumes_per_s']:.1f} plain; dlog10 {worst:.2e}")
+    del b, snaps
+    torch.cuda.empty_cache()
+    return res
+
+
+def serve_leg(args):
+    """Poisson stream of jobs over 16 resident hexagonal-twin snapshots per grid, 200 GB budget."""
+    import numpy as np
+    import torch
+    from qpop_diffraction import workloads
+    from qpop_diffraction.batch import PlanCache, Reflection, plain_job, resident_job
+    from qpop_diffraction.serve import Server, latency_summary, poisson_requests
+    grids = workloads.SERVE_GRIDS
+    refls = [Reflection(h, workloads.pto_atoms(h)) for h in workloads.REFLECTIONS]
+    store = workloads.serve_store()
+    log("serve: loading 16 snapshots per grid")
+    store.preload([(g, s) for g in grids for s in range(16)])
+    srv = Server(store, refls, n_slots=args.serve_slots, hbm_budget=200e9)
+    warm = poisson_requests(4 * args.serve_slots * len(grids), 0.0, grids, 16, 4, seed=99)
+    srv.run(warm, realtime=False)
+    reqs = poisson_requests(args.serve_requests, args.serve_rate, grids, 16, 4, seed=5)
+    log(f"serve: {len(reqs)} requests at {args.serve_rate}/s")
+    done = srv.run(reqs, realtime=True)
+    online = latency_summary(done, srv.elapsed_s)
+    log(f"serve: p50 {online['p50_ms']:.2f} ms p99 {online['p99_ms']:.2f} ms")
+    done_c = srv.run(reqs, realtime=False)
+    cap = latency_summary(done_c, srv.elapsed_s)
+    t0 = time.perf_counter()
+    for r in reqs:
+        plain_job(store.acquire(r.key), refls[r.reflection])
+        store.release(r.key)
+    sec_plain = time.perf_counter() - t0
+    # accuracy: one FP64 job per grid
+    worst = 0.0
+    plans64 = PlanCache(64)
+    for g in grids:
+        s32 = store.acquire((g, 0))
+        s64 = workloads.hex_snapshot((g, 0), *g, seed=1000, precision=64)
+        n = int(np.prod(g))
+        o32 = torch.empty(n, dtype=torch.float64, device="cuda")
+        o64 = torch.empty(n, dtype=torch.float64, device="cuda")
+        p32 = srv.plans.get(s32, 1, refls[1])
+        p64 = plans64.get(s64, 1, refls[1])
+        w = torch.empty(max(p32.workspace_bytes, p64.workspace_bytes), dtype=torch.uint8, device="cuda")
+        resident_job(srv.plans, s32, 1, refls[1], o32, w)
+        resident_job(plans64, s64, 1, refls[1], o64, w)
+        worst = max(worst, max_dlog10(o32, o64)[0])
+        store.release((g, 0))
+        del s64, w
+    res = {
+        "workload": f"Poisson stream, mean {args.serve_rate} req/s, {len(reqs)} jobs; grid uniform from "
+                    "128x128x32, 256x256x64, 512x512x104; snapshot id uniform from 16 three-variant hexagonal-twin "
+                    "fields per grid (resident); reflection uniform from 002, 103, 202, 113; HBM budget 200 GB; "
+                    f"{args.serve_slots} slots; latency = arrival to float64 intensity in pinned host memory",
+        "online": online,
+        "capacity_volumes_per_s": cap["volumes_per_s"],
+        "plain_volumes_per_s": len(reqs) / sec_plain,
+        "speedup_vs_plain": sec_plain / cap["elapsed_s"],
+        "peak_hbm_bytes": srv.peak_bytes, "deferred_admissions": srv.deferred,
+        "max_abs_dlog10_I_vs_fp64": worst, "dlog10_sample": "snapshot 0 of each grid, reflection 103",
+    }
+    log(f"serve: capacity {cap['volumes_per_s']:.1f} vol/s vs plain {res['plain_volumes_per_s']:.1f}")
+    return res
 
 
 def main():
@@ -161,14 +308,15 @@ def main():
         out64 = torch.empty(n, dtype=torch.float64, device="cuda")
         log("timing FP64")
         sec64 = time_steps(lambda: engine.intensity(vol64, out64), max(3, args.steps // 4), 1)
-        peak = out64.max()
-        mask = out64 >= 1e-10 * peak
-        dl = (torch.log10(I32[mask]) - torch.log10(out64[mask])).abs().max().item()
-        relpk = ((I32 - out64).abs().max() / peak).item()
+        dl, relpk = max_dlog10(I32, out64)
         result["ms_per_volume_fp64"] = sec64 * 1e3
         result["accuracy_fp32_vs_fp64"] = {"max_abs_dlog10_I": dl, "max_abs_dI_over_peak": relpk,
                                            "floor": "voxels >= 1e-10 of the peak"}
         del vol64, out64, w64
+    if not args.no_batch:
+        result["batched"] = batch_leg(args)
+    if not args.no_serve:
+        result["served"] = serve_leg(args)
     if args.cpu_crop > 0:
         log("CPU baseline")
         result["cpu_baseline"] = cpu_baseline(w, args.cpu_crop)

@@ -421,10 +421,26 @@ void fill_common(PassArgs<T>& a, const AxisFFT& ax, const Stage* stages_dev, con
     a.log2TX = lg;
 }
 
+// Raise a kernel's dynamic LDS limit once per (kernel, device) and size, so that launches
+// recorded into a graph make no attribute calls after the first eager launch.
+hipError_t allow_lds(const void* kernel, int lds) {
+    static std::mutex mu;
+    static std::map<std::pair<const void*, int>, int> granted;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> lock(mu);
+    int& have = granted[{kernel, dev}];
+    if (have >= lds) return hipSuccess;
+    e = hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    if (e == hipSuccess) have = lds;
+    return e;
+}
+
 template <typename A>
 hipError_t launch_one(void (*kernel)(A), const Launch& L, hipStream_t st, const A& a) {
     if (L.lds > 65536) {
-        hipError_t e = hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.lds);
+        hipError_t e = allow_lds((const void*)kernel, (int)L.lds);
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL(kernel, dim3((unsigned)L.blocks), dim3((unsigned)L.nthr), L.lds, st, a);

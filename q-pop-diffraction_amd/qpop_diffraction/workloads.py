@@ -186,3 +186,104 @@ def host_inputs(w):
     oPhase = np.stack([(ph == p).astype(np.float64) for p in range(n_phase)]).ravel()
     region = (ph >= 0).astype(np.float64).ravel()
     return oPhase, P, U, region
+
+
+# -- snapshots and reflections of the batched and served workloads ------------
+
+REFLECTIONS = ((0, 0, 2), (1, 0, 3), (2, 0, 2), (1, 1, 3))
+LATERALS = (128, 192, 256, 320, 384, 448, 512)
+# depths 24-104 chosen so that every radix of the transforms appears: 2, 3, 4, 5, 7, 8, 11, 13, 17
+DEPTHS = (24, 30, 34, 40, 44, 52, 56, 68, 84, 88, 104)
+
+
+def pto_atoms(hkl, n_phase=1, a=synth.A_PTO):
+    """The 5-atom PbTiO3-like cell with the pattern centred on reflection hkl."""
+    return atoms_from_text(synth.atom_file(synth.q00_of(hkl, a), a=a, n_phase=n_phase), n_phase)
+
+
+@dataclass
+class Snapshot:
+    """Device-resident fields of one snapshot (FerroDyn order, x fastest)."""
+    key: tuple
+    params: SizeContext
+    dims: tuple
+    axis: tuple
+    P: object
+    U: object
+    phase: object
+
+    def nbytes(self):
+        return sum(t.numel() * t.element_size() for t in (self.P, self.U, self.phase))
+
+
+def twin_snapshot(key, nx, ny, nz, seed, precision=32, a=synth.A_PTO, device="cuda"):
+    """Two-variant 90-degree twin stack along x: P || x <-> P || z, period 8-32 cells, |P| 0.5-0.7."""
+    import torch
+    rng = np.random.default_rng(seed)
+    period = int(rng.integers(8, 33))
+    mag = float(rng.uniform(0.5, 0.7))
+    variant = ((torch.arange(nx, device=device) // max(1, period // 2)) % 2).view(1, 1, nx).expand(nz, ny, nx)
+    gen = torch.Generator(device=device).manual_seed(seed)
+    P = 0.01 * torch.randn((3, nz, ny, nx), generator=gen, device=device, dtype=torch.float64)
+    P[0] += mag * (variant == 0)
+    P[2] += mag * (variant == 1)
+    real = torch.float32 if precision == 32 else torch.float64
+    n = nx * ny * nz
+    return Snapshot(key, box(nx, ny, nz, a, 1), (nx, ny, nz), (0, 1, 2), P.to(real).reshape(3, n).contiguous(),
+                    torch.zeros((3, n), dtype=torch.float64, device=device),
+                    torch.zeros(n, dtype=torch.int8, device=device))
+
+
+def hex_snapshot(key, nx, ny, nz, seed, precision=32, a=synth.A_PTO, device="cuda"):
+    """Three in-plane variants P = 0.6 (cos t, sin t, 0), t = 0/120/240 degrees, in periodic Voronoi domains."""
+    import torch
+    gen = torch.Generator(device=device).manual_seed(seed)
+    gid = voronoi((nz, ny, nx), 12, gen, device) % 3
+    theta = torch.deg2rad(torch.tensor([0.0, 120.0, 240.0], device=device, dtype=torch.float64))[gid]
+    P = torch.stack([0.6 * torch.cos(theta), 0.6 * torch.sin(theta), torch.zeros_like(theta)])
+    real = torch.float32 if precision == 32 else torch.float64
+    n = nx * ny * nz
+    return Snapshot(key, box(nx, ny, nz, a, 1), (nx, ny, nz), (0, 1, 2), P.to(real).reshape(3, n).contiguous(),
+                    torch.zeros((3, n), dtype=torch.float64, device=device),
+                    torch.zeros(n, dtype=torch.int8, device=device))
+
+
+def batch_grids(n_snapshots=48, seed=3):
+    """Seeded (lateral, depth) of each snapshot of the batched workload."""
+    rng = np.random.default_rng(seed)
+    return [(int(rng.choice(LATERALS)), int(rng.choice(DEPTHS))) for _ in range(n_snapshots)]
+
+
+@dataclass
+class SnapshotSpec:
+    """What a plan and the HBM budget need to know of a snapshot before its fields exist."""
+    params: SizeContext
+    dims: tuple
+    axis: tuple
+    nbytes: int
+
+
+def snapshot_spec(nx, ny, nz, precision=32, a=synth.A_PTO):
+    n = nx * ny * nz
+    real = 4 if precision == 32 else 8
+    return SnapshotSpec(box(nx, ny, nz, a, 1), (nx, ny, nz), (0, 1, 2), n * (3 * real + 3 * 8 + 1))
+
+
+SERVE_GRIDS = ((128, 128, 32), (256, 256, 64), (512, 512, 104))
+
+
+def serve_store(precision=32, seed0=1000):
+    """SnapshotStore of the served workload: key (grid, id) -> hexagonal-twin snapshot of seed seed0 + id."""
+    from .serve import SnapshotStore
+    specs = {}
+
+    def spec(key):
+        if key[0] not in specs:
+            specs[key[0]] = snapshot_spec(*key[0], precision=precision)
+        return specs[key[0]]
+
+    def load(key):
+        grid, sid = key
+        return hex_snapshot(key, *grid, seed=seed0 + int(sid), precision=precision)
+
+    return SnapshotStore(spec, load)
