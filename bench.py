@@ -121,9 +121,14 @@ def batch_leg(args):
     log(f"batch: generating 48 snapshots {grids[:4]} ...")
     snaps = [workloads.twin_snapshot(("twin", i), L, L, d, seed=100 + i) for i, (L, d) in enumerate(grids)]
     refls = [Reflection(h, workloads.pto_atoms(h)) for h in workloads.REFLECTIONS]
+    from qpop_diffraction.autotune import ScheduleCache
     t0 = time.perf_counter()
     b = Batch(snaps, refls, n_streams=args.batch_streams, graph=True)
     t_plan = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    b.tune(ScheduleCache(None))
+    t_tune = time.perf_counter() - t0
+    log(f"batch: tuned {len(b.tuning)} plans in {t_tune:.1f} s")
     b.capture()
     vox = sum(j.plan.n for j in b.jobs)
     log(f"batch: {len(b.jobs)} jobs, {len(b.plans)} plans built in {t_plan:.2f} s, capture {b.stats.capture_s:.2f} s")
@@ -167,6 +172,7 @@ def batch_leg(args):
                     f"{list(workloads.DEPTHS)}) x reflections 002, 103, 202, 113 = {n_jobs} volumes, FP32",
         "grids": [list(g) for g in grids],
         "volumes": n_jobs, "voxels": vox, "streams": b.n_streams, "plans": len(b.plans),
+        "plan_build_s": t_plan, "autotune_s": t_tune,
         "graph_replay_ms": sec * 1e3, "volumes_per_s": n_jobs / sec, "voxels_per_s": vox / sec,
         "serial_resident_volumes_per_s": n_jobs / sec_serial,
         "plain_volumes_per_s": n_jobs / sec_plain,
@@ -197,6 +203,17 @@ def serve_leg(args):
     log("serve: loading 16 snapshots per grid")
     store.preload([(g, s) for g in grids for s in range(16)])
     srv = Server(store, refls, n_slots=args.serve_slots, hbm_budget=200e9)
+    from qpop_diffraction.autotune import ScheduleCache, tune
+    cache = ScheduleCache(None)
+    for g in grids:
+        snap = store.acquire((g, 0))
+        for r in range(len(refls)):
+            plan = srv.plans.get(snap, r, refls[r])
+            o = torch.empty(plan.n, dtype=torch.float64, device="cuda")
+            w = torch.empty(plan.workspace_bytes, dtype=torch.uint8, device="cuda")
+            tune(plan, snap.P, snap.U, snap.phase, o, w, cache=cache)
+            del o, w
+        store.release((g, 0))
     warm = poisson_requests(4 * args.serve_slots * len(grids), 0.0, grids, 16, 4, seed=99)
     srv.run(warm, realtime=False)
     reqs = poisson_requests(args.serve_requests, args.serve_rate, grids, 16, 4, seed=5)
@@ -268,6 +285,9 @@ def main():
     plan = engine.plan(vol.dims, vol.axis, 32)
     work = engine.workspace(plan)
     out = torch.empty(n, dtype=torch.float64, device="cuda")
+    from qpop_diffraction.autotune import ScheduleCache
+    tuned = engine.tune(vol, cache=ScheduleCache(None), out=out)
+    log("schedules: " + "; ".join(f"pass {p['pass']}: {p['chosen']}" for p in tuned["passes"]))
 
     log(f"timing FP32: {args.warmup} warmup + {args.steps} steps")
     sec = time_steps(lambda: engine.intensity(vol, out), args.steps, args.warmup)
@@ -296,7 +316,8 @@ def main():
                                f"hkl 002, one volume per step", "grid": [nx, ny, nz], "precision": "fp32",
                    "launches_per_volume": plan.launches},
         "ms_per_volume_fp32": sec * 1e3,
-        "kernels": [{"pass": p, "ms": passes[p], "bytes_per_cell": bpc[p], "GB_s": gbs[p]} for p in range(3)],
+        "kernels": [{"pass": p, "ms": passes[p], "bytes_per_cell": bpc[p], "GB_s": gbs[p],
+                     "schedule": tuned["passes"][p]["chosen"]} for p in range(3)],
         "roofline": {"bound": "hbm", "kernel": f"pass {dom}", "achieved": gbs[dom], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": gbs[dom] / HBM_PEAK_GBS, "traffic": None,
                      "bytes_per_launch": bpc[dom] * n},

@@ -94,8 +94,30 @@ int qpd_plan_destroy(qpd_plan* plan);
 /* Bytes of device workspace qpd_intensity needs (intermediate spectra). */
 size_t qpd_workspace_bytes(const qpd_plan* plan);
 
-/* Number of kernel launches one qpd_intensity call enqueues. */
+/* Number of kernel launches one qpd_intensity call enqueues (a split pass launches twice). */
 int qpd_launches_per_volume(const qpd_plan* plan);
+
+/* Launch schedules.  Each pass of a plan can run one of several schedules: a
+ * compile-time-length kernel of this build (FP32), a long strided length split into
+ * two launches of registered lengths N1 x N2 (N2-point sub-transforms with the twiddle
+ * in place, then N1-point transforms with the pass's accumulation), or the generic
+ * run-time-radix kernel.  Schedule 0 is the default; an autotuner times the others
+ * and selects one.  Selecting is a host-side change that applies to calls enqueued
+ * afterwards (a captured graph keeps the launches it recorded).  No counterpart in the
+ * reference, which delegates its transforms to cuFFT / HeFFTe (qpop_diffraction_mgpu.cu:
+ * 154-176); these entry points replace those libraries' plan-time algorithm choice. */
+int qpd_plan_schedules(const qpd_plan* plan, int pass);      /* how many schedules pass has */
+int qpd_plan_schedule_name(const qpd_plan* plan, int pass, int index, char* buf, int cap);
+                                                             /* name into buf; returns its length */
+int qpd_plan_set_schedule(qpd_plan* plan, int pass, int index);
+int qpd_plan_get_schedule(const qpd_plan* plan, int pass);   /* the selected index */
+
+/* Lengths with a compile-time kernel for a role (0 source pass, 1 contiguous transform,
+ * 2 strided 3 -> 2 pass, 3 strided pass to the intensity, 4 strided transform, 5 first
+ * step of a split length, 6 and 7 roles 2 and 3 as the second step of a split length),
+ * ascending: writes up to cap of them and returns how many there are.  No counterpart
+ * in the reference. */
+int qpd_fast_lengths(int role, int64_t* lengths, int cap);
 
 /* The intensity of one volume: fftshifted max(|A|^2, 1e-50) as double, n values in
  * the grid's layout, written to I_dev.  work_dev holds qpd_workspace_bytes(plan).

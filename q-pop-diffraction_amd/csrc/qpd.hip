@@ -19,7 +19,7 @@
 
 #include "qpd.h"
 #include "qpd_kernels.hpp"
-#include "qpd_fast.hpp"
+#include "qpd_launch.hpp"
 
 using namespace qpd;
 
@@ -128,18 +128,6 @@ int max_lds_bytes() {
     return cached;
 }
 
-// ---------------------------------------------------------------------------
-// Launch shape of one pass.
-struct Launch {
-    int fast;        // 1: compile-time-length kernel of qpd_fast.hpp (FP32), 0: generic
-    int contiguous;  // 1: pass_contig, 0: pass_strided
-    int nl;          // rows (contig) or TX (strided) per block
-    int nf;          // inputs per LDS fill
-    int nthr;
-    int ept;         // elements per thread, 4 / 8 / 16
-    size_t lds;
-    int64_t blocks;
-};
 
 bool stage_fits(const AxisFFT& ax, int lines, int nthr) {
     for (int s = 0; s < ax.nst; ++s) {
@@ -193,6 +181,7 @@ bool choose_launch(const AxisFFT& ax, int nin, bool contiguous, size_t csize, in
                 if ((elems + nthr - 1) / nthr > ept_max) continue;
                 while (nthr < 512 && !stage_fits(ax, nf * p, nthr)) nthr += 64;
                 if (!stage_fits(ax, nf * p, nthr)) continue;
+                L.fe = nullptr;
                 L.contiguous = contiguous;
                 L.nl = p;
                 L.nf = nf;
@@ -221,11 +210,14 @@ struct qpd_plan {
     int nY, nZ;  // classes after pass 0 and pass 1
     size_t csize;
     AxisFFT fft[3];
-    Launch launch[3];
+    PassPlan pass[3];                 // the schedule each pass runs (one of cand)
+    std::vector<PassPlan> cand[3];    // schedules each pass can run, the default first
+    int sel[3] = {0, 0, 0};
     int ngroups[3];
     // device tables
     void* dev = nullptr;          // one allocation holding every table below
     const void* W[3];
+    std::map<int, const void*> Wlen;  // twiddles exp(-2 pi i m / L) of the split step lengths L
     const Stage* stages[3];
     const void* k[3];
     const void* coef[3];
@@ -266,6 +258,7 @@ template <typename T>
 int build_tables(qpd_plan* p, const qpd_grid* g, const qpd_crystal* c) {
     TableBuilder tb;
     size_t offW[3], offK[3], offC[3], offSrc[3], offCoef[3], offGrp[3], offSt[3], offAtom;
+    std::map<int, size_t> offWs;
     const int64_t n = p->ncell;
     const double h_of[3] = {g->h[0], g->h[1], g->h[2]};
 
@@ -307,6 +300,15 @@ int build_tables(qpd_plan* p, const qpd_grid* g, const qpd_crystal* c) {
         }
         offW[d] = tb.add_cplx<T>(wr, wi);
         offK[d] = tb.add_real<T>(kk);
+        for (const PassPlan& c : p->cand[d]) {
+            if (c.nlaunch != 2) continue;
+            for (int L : {c.N1, c.N2}) {
+                if (offWs.count(L)) continue;
+                std::vector<double> sr(L), si(L);
+                for (int m = 0; m < L; ++m) unit_root(m, L, sr[m], si[m]);
+                offWs[L] = tb.add_cplx<T>(sr, si);
+            }
+        }
         offSt[d] = tb.add(p->fft[d].st, sizeof(Stage) * kMaxStages);
 
         // coefficient tables and chunk / group lists
@@ -396,6 +398,7 @@ int build_tables(qpd_plan* p, const qpd_grid* g, const qpd_crystal* c) {
         p->group_first[d] = reinterpret_cast<const int*>(base + offGrp[d]);
     }
     p->atomtab = base + offAtom;
+    for (auto& kv : offWs) p->Wlen[kv.first] = base + kv.second;
     return QPD_OK;
 }
 
@@ -419,102 +422,31 @@ void fill_common(PassArgs<T>& a, const AxisFFT& ax, const Stage* stages_dev, con
     int lg = 0;
     while ((1 << lg) < L.nl) ++lg;
     a.log2TX = lg;
-}
-
-// Raise a kernel's dynamic LDS limit once per (kernel, device) and size, so that launches
-// recorded into a graph make no attribute calls after the first eager launch.
-hipError_t allow_lds(const void* kernel, int lds) {
-    static std::mutex mu;
-    static std::map<std::pair<const void*, int>, int> granted;
-    int dev = 0;
-    hipError_t e = hipGetDevice(&dev);
-    if (e != hipSuccess) return e;
-    std::lock_guard<std::mutex> lock(mu);
-    int& have = granted[{kernel, dev}];
-    if (have >= lds) return hipSuccess;
-    e = hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-    if (e == hipSuccess) have = lds;
-    return e;
-}
-
-template <typename A>
-hipError_t launch_one(void (*kernel)(A), const Launch& L, hipStream_t st, const A& a) {
-    if (L.lds > 65536) {
-        hipError_t e = allow_lds((const void*)kernel, (int)L.lds);
-        if (e != hipSuccess) return e;
-    }
-    hipLaunchKernelGGL(kernel, dim3((unsigned)L.blocks), dim3((unsigned)L.nthr), L.lds, st, a);
-    return hipGetLastError();
+    // unsplit addressing: element idx of a line at row idx, frequency index idx
+    a.nq = 1;
+    a.divQ = make_fastdiv(1);
+    a.in_step = a.out_step = a.c_step = 1;
+    a.Nc = ax.N;
 }
 
 // ---------------------------------------------------------------------------
-// Compile-time-length kernels (FP32).  Roles: 0 contiguous + source (4 inputs, 3
-// outputs), 1 contiguous plain transform, 2 strided 3 -> 2, 3 strided 2 -> intensity,
-// 4 strided plain transform.  G = rows or columns per block.
+// Compile-time-length kernels (FP32), looked up by (length, role).
 constexpr int kMaxAtomTab = 1024;  // atom-table entries the source pass keeps in LDS
-using S512 = Sizes<512, 8, 8, 8>;
-using S256 = Sizes<256, 4, 8, 8>;
-using S128 = Sizes<128, 4, 4, 8>;
-using S104 = Sizes<104, 8, 13>;
-using S64 = Sizes<64, 8, 8>;
 
-constexpr int fast_g(int role, int tpl) {
-    // source pass: 4 G TPL threads, about 256; plain contiguous: about 256 threads;
-    // strided: 16 columns (128-byte rows of FP32 complex) when the block stays within
-    // 512 threads, else 8
-    return role == 0 ? (tpl >= 64 ? 1 : 64 / tpl)
-                     : role == 1 ? (tpl >= 256 ? 1 : 256 / tpl) : (16 * tpl <= 512 ? 16 : 8);
+const FastEntry* fast_lookup(int64_t N, int role) {
+    for (const FastEntry& e : fast_registry())
+        if (e.N == N && e.role == role) return &e;
+    return nullptr;
 }
 
-template <class SZ>
-bool fast_shape_sz(int role, int atomtab_entries, Launch& L) {
-    const int G = fast_g(role, SZ::TPL);
-    const int F = role == 0 ? 4 : 1;  // inputs held in the tile at once
-    L.fast = 1;
-    L.contiguous = role <= 1;
-    L.nl = G;
-    L.nf = F;
-    L.nthr = (role == 0 ? 4 : 1) * G * SZ::TPL;
-    L.ept = 0;
-    // two stage buffers + the twiddle table (+ the atom table of the source pass)
-    const size_t tile = role <= 1 ? (size_t)F * G * SZ::PITCH : (size_t)SZ::N * (F * G + 1);
-    L.lds = (2 * tile + SZ::N) * 8 + (role == 0 ? 4 * atomtab_entries : 0);
-    return L.nthr <= 1024 && L.lds <= 160 * 1024;
-}
-
-bool fast_shape(int64_t N, int role, int atomtab_entries, Launch& L) {
-    switch (N) {
-        case 512: return fast_shape_sz<S512>(role, atomtab_entries, L);
-        case 256: return fast_shape_sz<S256>(role, atomtab_entries, L);
-        case 128: return fast_shape_sz<S128>(role, atomtab_entries, L);
-        case 104: return fast_shape_sz<S104>(role, atomtab_entries, L);
-        case 64: return fast_shape_sz<S64>(role, atomtab_entries, L);
-        default: return false;
-    }
-}
-
-template <class SZ>
-hipError_t launch_fast_sz(int role, const Launch& L, const PassArgs<float>& a, hipStream_t st) {
-    constexpr int T = SZ::TPL;
-    switch (role) {
-        case 0: return launch_one(fast_source<float, SZ, fast_g(0, T)>, L, st, a);
-        case 1: return launch_one(fast_contig<float, SZ, fast_g(1, T), 1, false>, L, st, a);
-        case 2: return launch_one(fast_strided<float, SZ, fast_g(2, T), 3, false>, L, st, a);
-        case 3: return launch_one(fast_strided<float, SZ, fast_g(3, T), 2, true>, L, st, a);
-        case 4: return launch_one(fast_strided<float, SZ, fast_g(4, T), 1, false>, L, st, a);
-        default: return hipErrorInvalidValue;
-    }
-}
-
-hipError_t launch_fast(int N, int role, const Launch& L, const PassArgs<float>& a, hipStream_t st) {
-    switch (N) {
-        case 512: return launch_fast_sz<S512>(role, L, a, st);
-        case 256: return launch_fast_sz<S256>(role, L, a, st);
-        case 128: return launch_fast_sz<S128>(role, L, a, st);
-        case 104: return launch_fast_sz<S104>(role, L, a, st);
-        case 64: return launch_fast_sz<S64>(role, L, a, st);
-        default: return hipErrorInvalidValue;
-    }
+void set_fast(Launch& L, const FastEntry* fe, size_t lds) {
+    L = Launch();
+    L.fe = fe;
+    L.contiguous = fe->role <= kRoleContig;
+    L.nl = fe->G;
+    L.nf = fe->role == kRoleSource ? 4 : 1;
+    L.nthr = fe->nthr;
+    L.lds = lds;
 }
 
 // Kernel instance for a pass: contiguous passes compute the source (3 outputs) or
@@ -522,11 +454,8 @@ hipError_t launch_fast(int N, int role, const Launch& L, const PassArgs<float>& 
 // write the intensity or run a plain transform.
 template <typename T>
 hipError_t launch_pass(const Launch& L, const PassArgs<T>& a, bool source, bool intensity, hipStream_t st) {
-    if (L.fast) {
-        if constexpr (std::is_same<T, float>::value) {
-            const int role = L.contiguous ? (source ? 0 : 1) : (intensity ? 3 : (a.nout == 2 ? 2 : 4));
-            return launch_fast(a.N, role, L, a, st);
-        }
+    if (L.fe) {
+        if constexpr (std::is_same<T, float>::value) return L.fe->launch(L, a, st);
         return hipErrorInvalidValue;
     }
     if (L.contiguous) {
@@ -547,47 +476,90 @@ hipError_t launch_pass(const Launch& L, const PassArgs<T>& a, bool source, bool 
     return launch_one(pass_strided<T, 8, 1, false>, L, st, a);
 }
 
-// Launch shapes of the three passes of a plan with the given inputs per chunk.
-int plan_launches(const AxisFFT fft[3], const int64_t dims[3], const int nin[3], size_t csize, Launch out[3],
-                  bool intensity_last = false, int atomtab_entries = 0) {
+// Schedules of pass d with nin inputs per chunk, the default first: the registered
+// compile-time kernels of the length and role (FP32), then, for strided lengths, every
+// split N1 x N2 into registered step lengths (most even first; only where allow_split),
+// then the generic kernel.  A split is the default only where the length has no kernel
+// of its own.
+int pass_candidates(const AxisFFT& ax, int d, const int64_t dims[3], int nin, size_t csize, bool intensity_last,
+                    int atomtab_entries, bool allow_split, std::vector<PassPlan>& out) {
+    out.clear();
     const int hard = max_lds_bytes();
     const int soft = std::min(hard, 65536);
-    for (int d = 0; d < 3; ++d) {
-        const bool contig = d == 0;
-        const int64_t lines = contig ? dims[1] * dims[2] : 0;
-        if (csize == 8 && fast_enabled()) {
-            int role = -1;
-            if (contig)
-                role = (nin[d] == 4 && atomtab_entries <= kMaxAtomTab) ? 0 : (nin[d] == 1 ? 1 : -1);
-            else if (nin[d] == 3)
-                role = 2;
-            else if (nin[d] == 2 && intensity_last && d == 2)
-                role = 3;
-            else if (nin[d] == 1)
-                role = 4;
-            if (role >= 0 && fast_shape(dims[d], role, atomtab_entries, out[d]) && (int)out[d].lds <= hard) {
-                if (contig) {
-                    out[d].blocks = (lines + out[d].nl - 1) / out[d].nl;
-                } else {
-                    const int64_t tiles0 = (dims[0] + out[d].nl - 1) / out[d].nl;
-                    out[d].blocks = tiles0 * (d == 1 ? dims[2] : dims[1]);
+    const bool contig = d == 0;
+    const int64_t N = dims[d];
+    const int64_t lines = contig ? dims[1] * dims[2] : 0;
+    const int64_t n_other = d == 1 ? dims[2] : dims[1];
+    if (csize == 8 && fast_enabled()) {
+        int role = -1;
+        if (contig)
+            role = (nin == 4 && atomtab_entries <= kMaxAtomTab) ? kRoleSource : (nin == 1 ? kRoleContig : -1);
+        else if (nin == 3)
+            role = kRoleStrided;
+        else if (nin == 2 && intensity_last && d == 2)
+            role = kRoleIntensity;
+        else if (nin == 1)
+            role = kRoleStrided1;
+        if (role >= 0) {
+            for (const FastEntry& e : fast_registry()) {
+                if (e.N != N || e.role != role) continue;
+                const size_t lds = e.lds + (role == kRoleSource ? (size_t)4 * atomtab_entries : 0);
+                if ((int)lds > hard) continue;
+                PassPlan pp;
+                set_fast(pp.L[0], &e, lds);
+                pp.L[0].blocks = contig ? (lines + e.G - 1) / e.G : (dims[0] + e.G - 1) / e.G * n_other;
+                if (pp.L[0].blocks <= 0x7fffffffLL) out.push_back(pp);
+            }
+            if (!contig && allow_split && (role == kRoleStrided || role == kRoleIntensity)) {
+                std::vector<std::pair<double, PassPlan>> splits;
+                for (int64_t n1 = 2; n1 <= N / 2; ++n1) {
+                    if (N % n1) continue;
+                    const int64_t n2 = N / n1;
+                    for (const FastEntry& ea : fast_registry()) {
+                        if (ea.N != n2 || ea.role != kRoleSplit || (int)ea.lds > hard) continue;
+                        const int role_b = role == kRoleStrided ? kRoleStridedB : kRoleIntensityB;
+                        for (const FastEntry& eb : fast_registry()) {
+                            if (eb.N != n1 || eb.role != role_b || (int)eb.lds > hard) continue;
+                            PassPlan pp;
+                            pp.nlaunch = 2;
+                            pp.N1 = (int)n1;
+                            pp.N2 = (int)n2;
+                            set_fast(pp.L[0], &ea, ea.lds);
+                            set_fast(pp.L[1], &eb, eb.lds);
+                            pp.L[0].blocks = (dims[0] + ea.G - 1) / ea.G * n1 * n_other;
+                            pp.L[1].blocks = (dims[0] + eb.G - 1) / eb.G * n2 * n_other;
+                            if (pp.L[0].blocks > 0x7fffffffLL || pp.L[1].blocks > 0x7fffffffLL) continue;
+                            const double score = std::fabs(std::log((double)n1 / (double)n2)) - 1e-6 * (double)n1;
+                            splits.push_back({score, pp});
+                        }
+                    }
                 }
-                continue;
+                std::stable_sort(splits.begin(), splits.end(),
+                                 [](const auto& x, const auto& y) { return x.first < y.first; });
+                for (auto& sp : splits) out.push_back(sp.second);
             }
         }
-        out[d].fast = 0;
-        const int ept_max = contig ? (nin[d] > 1 ? 4 : 8) : 8;
-        if (!choose_launch(fft[d], nin[d], contig, csize, lines, soft, hard, ept_max, out[d]))
-            return fail(QPD_ERR_UNSUPPORTED, "no launch shape fits dim " + std::to_string(d) + " of length " +
-                                                 std::to_string(dims[d]));
-        if (contig) {
-            out[d].blocks = (lines + out[d].nl - 1) / out[d].nl;
-        } else {
-            const int64_t tiles0 = (dims[0] + out[d].nl - 1) / out[d].nl;
-            const int64_t outer = d == 1 ? dims[2] : dims[1];
-            out[d].blocks = tiles0 * outer;
-        }
-        if (out[d].blocks > 0x7fffffffLL) return fail(QPD_ERR_UNSUPPORTED, "grid too large");
+    }
+    PassPlan gp;
+    Launch& L = gp.L[0];
+    const int ept_max = contig ? (nin > 1 ? 4 : 8) : 8;
+    if (choose_launch(ax, nin, contig, csize, lines, soft, hard, ept_max, L)) {
+        L.blocks = contig ? (lines + L.nl - 1) / L.nl : (dims[0] + L.nl - 1) / L.nl * n_other;
+        if (L.blocks <= 0x7fffffffLL) out.push_back(gp);
+    }
+    if (out.empty())
+        return fail(QPD_ERR_UNSUPPORTED, "no launch shape fits dim " + std::to_string(d) + " of length " + std::to_string(N));
+    return QPD_OK;
+}
+
+// Default schedules of the three passes (plain transforms never split: the second step
+// of a split writes to another buffer).
+int plan_launches(const AxisFFT fft[3], const int64_t dims[3], const int nin[3], size_t csize, PassPlan out[3]) {
+    for (int d = 0; d < 3; ++d) {
+        std::vector<PassPlan> c;
+        int rc = pass_candidates(fft[d], d, dims, nin[d], csize, false, 0, false, c);
+        if (rc) return rc;
+        out[d] = c[0];
     }
     return QPD_OK;
 }
@@ -595,7 +567,7 @@ int plan_launches(const AxisFFT fft[3], const int64_t dims[3], const int nin[3],
 template <typename T>
 void set_strided(PassArgs<T>& a, int d, const int64_t dims[3]) {
     a.dim = d;
-    a.ntiles0 = (uint32_t)((dims[0] + (1 << a.log2TX) - 1) >> a.log2TX);
+    a.ntiles0 = (uint32_t)((dims[0] + a.nl - 1) / a.nl);
     if (d == 1) {
         a.S = (uint32_t)dims[0];
         a.OS = (uint32_t)(dims[0] * dims[1]);
@@ -612,8 +584,10 @@ int run_intensity(qpd_plan* p, const qpd_fields* f, double* I, void* work, hipSt
     cx<T>* w2 = w1 + (size_t)3 * p->nY * n;
     for (int d = 0; d < 3; ++d) {
         if (only >= 0 && d != only) continue;
+        const PassPlan& pp = p->pass[d];
+        const Launch& L = pp.L[pp.nlaunch - 1];  // the launch that accumulates
         PassArgs<T> a;
-        fill_common(a, p->fft[d], p->stages[d], p->launch[d], p->dims);
+        fill_common(a, p->fft[d], p->stages[d], L, p->dims);
         a.W = static_cast<const cx<T>*>(p->W[d]);
         a.coef = static_cast<const cx<T>*>(p->coef[d]);
         a.k = static_cast<const T*>(p->k[d]);
@@ -647,7 +621,35 @@ int run_intensity(qpd_plan* p, const qpd_fields* f, double* I, void* work, hipSt
             else
                 a.I = I;
         }
-        QPD_HIP(launch_pass<T>(p->launch[d], a, d == 0, d == 2, st));
+        if (pp.nlaunch == 2) {
+            // step 1: N2-point sub-transforms of every input field, twiddled, in place
+            const uint32_t N1 = (uint32_t)pp.N1, N2 = (uint32_t)pp.N2;
+            PassArgs<T> s1 = a;
+            s1.N = (int)N2;
+            s1.nl = pp.L[0].nl;
+            set_strided(s1, d, p->dims);
+            s1.W = static_cast<const cx<T>*>(p->Wlen.at(pp.N2));
+            s1.Wsplit = static_cast<const cx<T>*>(p->W[d]);
+            s1.nq = N1;
+            s1.divQ = make_fastdiv(N1);
+            s1.in_qmul = s1.out_qmul = 1;
+            s1.in_step = s1.out_step = N1;
+            s1.out = const_cast<cx<T>*>(a.in);
+            s1.nfields = (d == 1 ? 3 * p->nY : 2 * p->nZ);
+            QPD_HIP(launch_pass<T>(pp.L[0], s1, false, false, st));
+            // step 2: N1-point transforms over rows N1 q + y1 into rows q + N2 k1, accumulated
+            a.N = (int)N1;
+            a.W = static_cast<const cx<T>*>(p->Wlen.at(pp.N1));
+            a.nq = N2;
+            a.divQ = make_fastdiv(N2);
+            a.in_qmul = N1;
+            a.in_step = 1;
+            a.out_qmul = 1;
+            a.out_step = N2;
+            a.c_qmul = 1;
+            a.c_step = N2;
+        }
+        QPD_HIP(launch_pass<T>(L, a, d == 0, d == 2, st));
     }
     return QPD_OK;
 }
@@ -658,7 +660,7 @@ constexpr int kMaxPlainFields = 64;
 
 struct FFTPlan {
     AxisFFT fft[3];
-    Launch launch[3];
+    PassPlan pass[3];
     void* dev = nullptr;
     const void* W[3];
     const Stage* stages[3];
@@ -684,7 +686,7 @@ int get_fft_plan(const int64_t dims[3], FFTPlan*& out) {
     for (int d = 0; d < 3; ++d)
         if (!make_axis(dims[d], fp.fft[d]))
             return fail(QPD_ERR_UNSUPPORTED, "unsupported transform length " + std::to_string(dims[d]));
-    int rc = plan_launches(fp.fft, dims, nin, sizeof(cx<T>), fp.launch);
+    int rc = plan_launches(fp.fft, dims, nin, sizeof(cx<T>), fp.pass);
     if (rc) return rc;
     TableBuilder tb;
     size_t offW[3], offSt[3];
@@ -723,7 +725,7 @@ int run_fft3d(const int64_t dims[3], void* data, int n_fields, hipStream_t st) {
     cx<T>* x = static_cast<cx<T>*>(data);
     for (int d = 0; d < 3; ++d) {
         PassArgs<T> a;
-        fill_common(a, fp->fft[d], fp->stages[d], fp->launch[d], dims);
+        fill_common(a, fp->fft[d], fp->stages[d], fp->pass[d].L[0], dims);
         a.W = static_cast<const cx<T>*>(fp->W[d]);
         a.k = static_cast<const T*>(fp->k);
         a.chunk_src = fp->index;
@@ -735,12 +737,63 @@ int run_fft3d(const int64_t dims[3], void* data, int n_fields, hipStream_t st) {
         a.in = x;
         a.out = x;
         if (d > 0) set_strided(a, d, dims);
-        QPD_HIP(launch_pass<T>(fp->launch[d], a, false, false, st));
+        QPD_HIP(launch_pass<T>(fp->pass[d].L[0], a, false, false, st));
     }
     return QPD_OK;
 }
 
+std::string describe_launch(const Launch& L) {
+    char buf[160];
+    if (L.fe)
+        std::snprintf(buf, sizeof(buf), "fast %d (%s) G%d", L.fe->N, L.fe->radices, L.fe->G);
+    else
+        std::snprintf(buf, sizeof(buf), "generic %s nl%d nf%d nthr%d ept%d", L.contiguous ? "contiguous" : "strided",
+                      L.nl, L.nf, L.nthr, L.ept);
+    return buf;
+}
+
+std::string describe(const PassPlan& pp) {
+    if (pp.nlaunch == 1) return describe_launch(pp.L[0]);
+    return "split " + std::to_string(pp.N1 * pp.N2) + " = " + std::to_string(pp.N1) + " x " + std::to_string(pp.N2) +
+           ": " + describe_launch(pp.L[0]) + " + " + describe_launch(pp.L[1]);
+}
+
 }  // namespace
+
+// ---------------------------------------------------------------------------
+namespace qpd {
+
+hipError_t allow_lds(const void* kernel, int lds) {
+    static std::mutex mu;
+    static std::map<std::pair<const void*, int>, int> granted;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> lock(mu);
+    int& have = granted[{kernel, dev}];
+    if (have >= lds) return hipSuccess;
+    e = hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    if (e == hipSuccess) have = lds;
+    return e;
+}
+
+const std::vector<FastEntry>& fast_registry() {
+    static const std::vector<FastEntry> reg = [] {
+        std::vector<FastEntry> v;
+        register_fast_part0(v);
+        register_fast_part1(v);
+        register_fast_part2(v);
+        register_fast_part3(v);
+        register_fast_part4(v);
+        register_fast_part5(v);
+        register_fast_part6(v);
+        register_fast_part7(v);
+        return v;
+    }();
+    return reg;
+}
+
+}  // namespace qpd
 
 // ---------------------------------------------------------------------------
 extern "C" {
@@ -793,7 +846,16 @@ int qpd_plan_create(qpd_plan** plan, const qpd_grid* grid, const qpd_crystal* c,
     p->csize = precision == 32 ? sizeof(cx<float>) : sizeof(cx<double>);
     for (int i = 0; i < 3; ++i) p->qC[i] = c->qC[i];
     const int nin[3] = {4, 3, 2};
-    int rc = plan_launches(p->fft, p->dims, nin, p->csize, p->launch, true, c->n_atom * c->n_phase * kAtomTab);
+    int rc = QPD_OK;
+    for (int d = 0; d < 3 && rc == QPD_OK; ++d) {
+        rc = pass_candidates(p->fft[d], d, p->dims, nin[d], p->csize, true, c->n_atom * c->n_phase * kAtomTab, true,
+                             p->cand[d]);
+        if (rc == QPD_OK) {
+            // default: a kernel of the length itself, else the most even split, else generic
+            p->sel[d] = 0;
+            p->pass[d] = p->cand[d][0];
+        }
+    }
     if (rc == QPD_OK) rc = precision == 32 ? build_tables<float>(p, grid, c) : build_tables<double>(p, grid, c);
     if (rc != QPD_OK) {
         if (p->dev) (void)hipFree(p->dev);
@@ -819,7 +881,49 @@ size_t qpd_workspace_bytes(const qpd_plan* p) {
     return (size_t)(3 * p->nY + 2 * p->nZ) * (size_t)p->ncell * p->csize;
 }
 
-int qpd_launches_per_volume(const qpd_plan* p) { return p ? 3 : 0; }
+int qpd_fast_lengths(int role, int64_t* lengths, int cap) {
+    if (role < 0 || role >= kNumRoles) return fail(QPD_ERR_ARG, "role must be 0..7");
+    std::vector<int64_t> ls;
+    for (const FastEntry& e : fast_registry())
+        if (e.role == role && std::find(ls.begin(), ls.end(), (int64_t)e.N) == ls.end()) ls.push_back(e.N);
+    std::sort(ls.begin(), ls.end());
+    for (int i = 0; i < cap && i < (int)ls.size() && lengths; ++i) lengths[i] = ls[i];
+    return (int)ls.size();
+}
+
+int qpd_plan_schedules(const qpd_plan* p, int pass) {
+    if (!p || pass < 0 || pass > 2) return fail(QPD_ERR_ARG, "null plan or pass not 0, 1, 2");
+    return (int)p->cand[pass].size();
+}
+
+int qpd_plan_schedule_name(const qpd_plan* p, int pass, int index, char* buf, int cap) {
+    if (!p || pass < 0 || pass > 2) return fail(QPD_ERR_ARG, "null plan or pass not 0, 1, 2");
+    if (index < 0 || index >= (int)p->cand[pass].size()) return fail(QPD_ERR_ARG, "schedule index out of range");
+    const std::string name = describe(p->cand[pass][index]);
+    if (buf && cap > 0) {
+        const size_t n = std::min((size_t)cap - 1, name.size());
+        std::memcpy(buf, name.data(), n);
+        buf[n] = 0;
+    }
+    return (int)name.size();
+}
+
+int qpd_plan_set_schedule(qpd_plan* p, int pass, int index) {
+    if (!p || pass < 0 || pass > 2) return fail(QPD_ERR_ARG, "null plan or pass not 0, 1, 2");
+    if (index < 0 || index >= (int)p->cand[pass].size()) return fail(QPD_ERR_ARG, "schedule index out of range");
+    p->sel[pass] = index;
+    p->pass[pass] = p->cand[pass][index];
+    return QPD_OK;
+}
+
+int qpd_plan_get_schedule(const qpd_plan* p, int pass) {
+    if (!p || pass < 0 || pass > 2) return fail(QPD_ERR_ARG, "null plan or pass not 0, 1, 2");
+    return p->sel[pass];
+}
+
+int qpd_launches_per_volume(const qpd_plan* p) {
+    return p ? p->pass[0].nlaunch + p->pass[1].nlaunch + p->pass[2].nlaunch : 0;
+}
 
 int qpd_intensity_pass(qpd_plan* p, const qpd_fields* f, double* I, void* work, int pass, void* stream) {
     if (!p || !f || !I || !work) return fail(QPD_ERR_ARG, "null argument");

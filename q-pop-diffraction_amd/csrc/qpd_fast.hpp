@@ -411,7 +411,8 @@ __global__ __launch_bounds__(4 * G * SZ::TPL) void fast_source(PassArgs<T> a) {
 // ---------------------------------------------------------------------------
 // Strided pass: block = G consecutive i0 (columns) x one outer index; thread
 // (column g = tid % G, j = tid / G).  INTENSITY: write fftshifted max(|A|^2, 1e-50).
-template <typename T, class SZ, int G, int NIN, bool INTENSITY>
+// Rows follow the split addressing of PassArgs (plain passes: nq = 1, q = 0, unit steps).
+template <typename T, class SZ, int G, int NIN, bool INTENSITY, bool SPLIT>
 __global__ __launch_bounds__(G* SZ::TPL) void fast_strided(PassArgs<T> a) {
     // the NIN inputs of a chunk are transformed one after another (one line per thread)
     constexpr int N = SZ::N, R0 = SZ::rad(0), NR0 = SZ::nr(0), LP = G + 1;
@@ -425,10 +426,16 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided(PassArgs<T> a) {
     const int tid = threadIdx.x;
     const int g = tid % G, j = tid / G;
     const uint32_t t0 = blockIdx.x % a.ntiles0;
-    const uint32_t outer = blockIdx.x / a.ntiles0;
+    const uint32_t outer_q = blockIdx.x / a.ntiles0;
+    // unsplit passes: q = 0 and unit steps, known at compile time
+    const uint32_t outer = SPLIT ? fdiv(outer_q, a.divQ) : outer_q;
+    const uint32_t q = SPLIT ? outer_q - outer * a.nq : 0u;
+    const uint32_t in_step = SPLIT ? a.in_step : 1u, out_step = SPLIT ? a.out_step : 1u;
+    const uint32_t c_step = SPLIT ? a.c_step : 1u;
     const uint32_t i0 = t0 * G + g;
     const bool col_ok = i0 < a.n0;
     const uint32_t colbase = i0 + outer * a.OS;
+    const uint32_t in0 = q * a.in_qmul, out0 = q * a.out_qmul, c0 = q * a.c_qmul;
     load_twiddles<T, N>(Wl, a.W, tid, G * SZ::TPL);
     __syncthreads();
     // inputs are consumed in the order (chunk, input); on long lines the next one is
@@ -440,7 +447,8 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided(PassArgs<T> a) {
         const cx<T>* in = a.in + ((size_t)a.chunk_src[c] * a.nin + f) * a.ncell + colbase;
         if (j < NR0) {
 #pragma unroll
-            for (int r = 0; r < R0; ++r) dst[0][r] = col_ok ? in[(uint32_t)(j + r * NR0) * a.S] : cx<T>{(T)0, (T)0};
+            for (int r = 0; r < R0; ++r)
+                dst[0][r] = col_ok ? in[(in0 + (uint32_t)(j + r * NR0) * in_step) * a.S] : cx<T>{(T)0, (T)0};
         }
     };
     cx<T> nxt[1][SZ::RMAX];
@@ -470,11 +478,11 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided(PassArgs<T> a) {
                 if (j < SZ::NRL) {
 #pragma unroll
                     for (int r = 0; r < SZ::RL; ++r) {
-                        const int idx = j + r * SZ::NRL;
+                        const int idx = (int)(c0 + (uint32_t)(j + r * SZ::NRL) * c_step);
                         cx<T> t = cmul(chunk_coef(a, ct, idx), v[0][r]);
                         if (f == 1) t = cscale(negi(t), a.k[idx]);
-                        const int q = f == 0 ? 0 : (f == 1 ? 0 : f - 1);
-                        acc[q][r] = acc[q][r] + t;
+                        const int oq = f <= 1 ? 0 : f - 1;
+                        acc[oq][r] = acc[oq][r] + t;
                     }
                 }
             }
@@ -487,7 +495,7 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided(PassArgs<T> a) {
             const uint32_t sB = (outer + nB / 2) >= nB ? outer + nB / 2 - nB : outer + nB / 2;
 #pragma unroll
             for (int r = 0; r < SZ::RL; ++r) {
-                const uint32_t idx = j + r * SZ::NRL;
+                const uint32_t idx = c0 + (uint32_t)(j + r * SZ::NRL) * c_step;
                 const uint32_t sA = (idx + nA / 2) >= nA ? idx + nA / 2 - nA : idx + nA / 2;
                 const uint32_t s1 = a.dim == 2 ? sB : sA;
                 const uint32_t s2 = a.dim == 2 ? sA : sB;
@@ -497,10 +505,70 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided(PassArgs<T> a) {
         } else {
             cx<T>* o = a.out + (size_t)grp * NOUT * a.ncell + colbase;
 #pragma unroll
-            for (int q = 0; q < NOUT; ++q)
+            for (int qq = 0; qq < NOUT; ++qq)
 #pragma unroll
                 for (int r = 0; r < SZ::RL; ++r)
-                    o[(size_t)q * a.ncell + (uint32_t)(j + r * SZ::NRL) * a.S] = acc[q][r];
+                    o[(size_t)qq * a.ncell + (out0 + (uint32_t)(j + r * SZ::NRL) * out_step) * a.S] = acc[qq][r];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// First step of a split strided transform of length Nc = N1 N, in place on every
+// field: for the line (column i0, q < N1, outer) the N points at rows q + N1 y2 are
+// transformed (length N = SZ::N) and output k2 is multiplied by exp(-2 pi i q k2 / Nc)
+// and written back to row q + N1 k2 (in_qmul = out_qmul = 1, in_step = out_step = N1).
+// Each block reads all of a line before it writes it, and blocks own disjoint lines.
+template <typename T, class SZ, int G>
+__global__ __launch_bounds__(G* SZ::TPL) void fast_split(PassArgs<T> a) {
+    constexpr int N = SZ::N, R0 = SZ::rad(0), NR0 = SZ::nr(0), LP = G + 1;
+    constexpr int TILE = N * LP;
+    constexpr bool ALT = (chunk_writes<SZ, false>() & 1) == 1;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T>* bufA = reinterpret_cast<cx<T>*>(smem_raw);
+    cx<T>* bufB = bufA + TILE;
+    cx<T>* Wl = bufB + TILE;
+    const int tid = threadIdx.x;
+    const int g = tid % G, j = tid / G;
+    const uint32_t t0 = blockIdx.x % a.ntiles0;
+    const uint32_t outer_q = blockIdx.x / a.ntiles0;
+    const uint32_t outer = fdiv(outer_q, a.divQ);
+    const uint32_t q = outer_q - outer * a.nq;
+    const uint32_t i0 = t0 * G + g;
+    const bool col_ok = i0 < a.n0;
+    const uint32_t colbase = i0 + outer * a.OS + q * a.in_qmul * a.S;
+    const uint32_t rstride = a.in_step * a.S;
+    load_twiddles<T, N>(Wl, a.W, tid, G * SZ::TPL);
+    __syncthreads();
+    constexpr bool PF = SZ::TPL >= 32;
+    auto load_field = [&](int f, cx<T>(&dst)[1][SZ::RMAX]) {
+        const cx<T>* in = a.out + (size_t)f * a.ncell + colbase;
+        if (j < NR0) {
+#pragma unroll
+            for (int r = 0; r < R0; ++r) dst[0][r] = col_ok ? in[(uint32_t)(j + r * NR0) * rstride] : cx<T>{(T)0, (T)0};
+        }
+    };
+    cx<T> nxt[1][SZ::RMAX];
+    if constexpr (PF) load_field(0, nxt);
+    int par = 0;
+    for (int f = 0; f < a.nfields; ++f) {
+        cx<T> v[1][SZ::RMAX];
+        if constexpr (PF) {
+#pragma unroll
+            for (int r = 0; r < SZ::RMAX; ++r) v[0][r] = nxt[0][r];
+            if (f + 1 < a.nfields) load_field(f + 1, nxt);
+        } else {
+            load_field(f, v);
+        }
+        transform<T, SZ, 1, true, LP, false>(par ? bufB : bufA, par ? bufA : bufB, v, j, g, 0, Wl);
+        if (ALT) par ^= 1;
+        if (col_ok && j < SZ::NRL) {
+            cx<T>* o = a.out + (size_t)f * a.ncell + colbase;
+#pragma unroll
+            for (int r = 0; r < SZ::RL; ++r) {
+                const uint32_t k2 = (uint32_t)(j + r * SZ::NRL);
+                o[k2 * rstride] = cmul(v[0][r], a.Wsplit[q * k2]);
+            }
         }
     }
 }

@@ -1,0 +1,87 @@
+// Instantiation helpers of the fast-kernel registry (included by qpd_fast_sizes.hip only).
+#pragma once
+
+#include "qpd_fast.hpp"
+#include "qpd_launch.hpp"
+
+namespace qpd {
+
+// Default tile width G of a role for a length with TPL threads per line: the source
+// pass runs 4 G TPL threads (about 256), the plain contiguous pass about 256; strided
+// passes take 16 columns (128-byte rows of FP32 complex) while the block stays within
+// 512 threads, else 8, else 4 (0: no strided kernel for this length).  Blocks of at
+// most 512 threads leave each wave 256 VGPRs.
+constexpr int fast_tile(int role, int tpl) {
+    return role == kRoleSource   ? (tpl >= 64 ? 1 : 64 / tpl)
+           : role == kRoleContig ? (tpl >= 256 ? 1 : 256 / tpl)
+                                 : (16 * tpl <= 512 ? 16 : (8 * tpl <= 512 ? 8 : (4 * tpl <= 512 ? 4 : 0)));
+}
+
+template <class SZ, int ROLE, int G>
+hipError_t fast_launch(const Launch& L, const PassArgs<float>& a, hipStream_t st) {
+    if constexpr (ROLE == kRoleSource)
+        return launch_one(fast_source<float, SZ, G>, L, st, a);
+    else if constexpr (ROLE == kRoleContig)
+        return launch_one(fast_contig<float, SZ, G, 1, false>, L, st, a);
+    else if constexpr (ROLE == kRoleStrided)
+        return launch_one(fast_strided<float, SZ, G, 3, false, false>, L, st, a);
+    else if constexpr (ROLE == kRoleIntensity)
+        return launch_one(fast_strided<float, SZ, G, 2, true, false>, L, st, a);
+    else if constexpr (ROLE == kRoleStrided1)
+        return launch_one(fast_strided<float, SZ, G, 1, false, false>, L, st, a);
+    else if constexpr (ROLE == kRoleSplit)
+        return launch_one(fast_split<float, SZ, G>, L, st, a);
+    else if constexpr (ROLE == kRoleStridedB)
+        return launch_one(fast_strided<float, SZ, G, 3, false, true>, L, st, a);
+    else
+        return launch_one(fast_strided<float, SZ, G, 2, true, true>, L, st, a);
+}
+
+template <class SZ, int ROLE, int G>
+void add_fast(std::vector<FastEntry>& v, const char* radices) {
+    if constexpr (G >= 1) {
+        constexpr int F = ROLE == kRoleSource ? 4 : 1;
+        constexpr int nthr = F * G * SZ::TPL;
+        // two stage buffers and the twiddle table
+        constexpr size_t tile = ROLE <= kRoleContig ? (size_t)F * G * SZ::PITCH : (size_t)SZ::N * (G + 1);
+        constexpr size_t lds = (2 * tile + SZ::N) * 8;
+        if constexpr (nthr <= 1024 && lds <= 160 * 1024)
+            v.push_back({SZ::N, ROLE, G, nthr, SZ::TPL, lds, radices, &fast_launch<SZ, ROLE, G>});
+    }
+}
+
+// The default tile width first, then the variants an autotuner may pick: the source
+// pass with twice the rows while the block stays within 512 threads (beyond that the
+// register budget halves and its accumulators spill), strided passes with half the
+// columns.
+template <class SZ, int ROLE>
+void add_fast_variants(std::vector<FastEntry>& v, const char* radices) {
+    constexpr int G = fast_tile(ROLE, SZ::TPL);
+    add_fast<SZ, ROLE, G>(v, radices);
+    if constexpr (ROLE == kRoleSource) {
+        if constexpr (8 * G * SZ::TPL <= 512) add_fast<SZ, ROLE, 2 * G>(v, radices);
+    } else if constexpr (ROLE != kRoleContig && G >= 8) {
+        add_fast<SZ, ROLE, G / 2>(v, radices);
+    }
+}
+
+constexpr unsigned kRolesContig = (1u << kRoleSource) | (1u << kRoleContig);
+constexpr unsigned kRolesStrided = (1u << kRoleStrided) | (1u << kRoleIntensity) | (1u << kRoleStrided1);
+constexpr unsigned kRolesSplitSteps = (1u << kRoleSplit) | (1u << kRoleStridedB) | (1u << kRoleIntensityB);
+
+template <class SZ, unsigned ROLES>
+void add_fast_roles(std::vector<FastEntry>& v, const char* radices) {
+    if constexpr ((ROLES >> kRoleSource) & 1u) add_fast_variants<SZ, kRoleSource>(v, radices);
+    if constexpr ((ROLES >> kRoleContig) & 1u) add_fast_variants<SZ, kRoleContig>(v, radices);
+    if constexpr ((ROLES >> kRoleStrided) & 1u) add_fast_variants<SZ, kRoleStrided>(v, radices);
+    if constexpr ((ROLES >> kRoleIntensity) & 1u) add_fast_variants<SZ, kRoleIntensity>(v, radices);
+    if constexpr ((ROLES >> kRoleStrided1) & 1u)
+        add_fast<SZ, kRoleStrided1, fast_tile(kRoleStrided1, SZ::TPL)>(v, radices);
+    if constexpr ((ROLES >> kRoleSplit) & 1u) add_fast<SZ, kRoleSplit, fast_tile(kRoleSplit, SZ::TPL)>(v, radices);
+    if constexpr ((ROLES >> kRoleStridedB) & 1u)
+        add_fast<SZ, kRoleStridedB, fast_tile(kRoleStridedB, SZ::TPL)>(v, radices);
+    if constexpr ((ROLES >> kRoleIntensityB) & 1u)
+        add_fast<SZ, kRoleIntensityB, fast_tile(kRoleIntensityB, SZ::TPL)>(v, radices);
+}
+
+}  // namespace qpd

@@ -56,6 +56,17 @@ struct PassArgs {
     uint32_t ntiles0;
     uint32_t S;               // element stride along the pass
     uint32_t OS;              // stride of the outer index
+    // split transforms (fast strided kernels): a length N = N1 N2 runs as N2-point
+    // sub-transforms over rows q + N1 y2 (q < N1), then N1-point ones over rows N1 q + y1
+    // (q < N2) written to rows q + N2 k1.  The outer index of a block is q + nq * outer;
+    // element idx of a line is read from row q in_qmul + idx in_step and written to row
+    // q out_qmul + idx out_step; its full-length frequency index is q c_qmul + idx c_step.
+    uint32_t nq;
+    FastDiv divQ;
+    uint32_t in_qmul, in_step, out_qmul, out_step, c_qmul, c_step;
+    int Nc;                   // length of the k and coefficient tables (the full length)
+    const cx<T>* Wsplit;      // exp(-2 pi i m / Nc), m < Nc: twiddles between the two steps
+    int nfields;              // fields transformed in place by the first step
     // data
     const cx<T>* in;
     cx<T>* out;
@@ -143,7 +154,7 @@ __device__ __forceinline__ void accumulate_term(int jj, cx<T> t, T kv, cx<T>* ac
 
 template <typename T>
 __device__ __forceinline__ cx<T> chunk_coef(const PassArgs<T>& a, int ct, int idx) {
-    return ct >= 0 ? a.coef[ct * a.N + idx] : cx<T>{(T)1, (T)0};
+    return ct >= 0 ? a.coef[ct * a.Nc + idx] : cx<T>{(T)1, (T)0};
 }
 
 // ---------------------------------------------------------------------------

@@ -19,8 +19,13 @@ LIB_PATH = Path(os.environ.get("QPD_LIB", Path(__file__).resolve().parent / "lib
 # Entry points declared in include/qpd.h
 EXPORTS = (
     "qpd_plan_create", "qpd_plan_destroy", "qpd_workspace_bytes", "qpd_launches_per_volume",
-    "qpd_intensity", "qpd_intensity_pass", "qpd_pass_bytes_per_cell", "qpd_fft3d", "qpd_fft_length_supported", "qpd_last_error", "qpd_version",
+    "qpd_plan_schedules", "qpd_plan_schedule_name", "qpd_plan_set_schedule", "qpd_plan_get_schedule",
+    "qpd_fast_lengths", "qpd_intensity", "qpd_intensity_pass", "qpd_pass_bytes_per_cell", "qpd_fft3d",
+    "qpd_fft_length_supported", "qpd_last_error", "qpd_version",
 )
+
+ROLES = {"source": 0, "contiguous": 1, "strided": 2, "intensity": 3, "strided1": 4, "split": 5,
+         "strided_split": 6, "intensity_split": 7}
 
 c_double3 = ctypes.c_double * 3
 
@@ -68,6 +73,16 @@ def load_library():
     lib.qpd_workspace_bytes.restype = ctypes.c_size_t
     lib.qpd_launches_per_volume.argtypes = [vp]
     lib.qpd_launches_per_volume.restype = i32
+    lib.qpd_plan_schedules.argtypes = [vp, i32]
+    lib.qpd_plan_schedules.restype = i32
+    lib.qpd_plan_schedule_name.argtypes = [vp, i32, i32, ctypes.c_char_p, i32]
+    lib.qpd_plan_schedule_name.restype = i32
+    lib.qpd_plan_set_schedule.argtypes = [vp, i32, i32]
+    lib.qpd_plan_set_schedule.restype = i32
+    lib.qpd_plan_get_schedule.argtypes = [vp, i32]
+    lib.qpd_plan_get_schedule.restype = i32
+    lib.qpd_fast_lengths.argtypes = [i32, ctypes.POINTER(i64), i32]
+    lib.qpd_fast_lengths.restype = i32
     lib.qpd_intensity.argtypes = [vp, ctypes.POINTER(Fields), vp, vp, vp]
     lib.qpd_intensity.restype = i32
     lib.qpd_intensity_pass.argtypes = [vp, ctypes.POINTER(Fields), vp, vp, i32, vp]
@@ -152,6 +167,29 @@ class Plan:
         fields = self._fields(P, U, phase, fraction, region)
         return float(load_library().qpd_pass_bytes_per_cell(self._h, ctypes.byref(fields), int(which)))
 
+    def schedules(self, which):
+        """Names of the launch schedules pass `which` can run (index 0 is the default)."""
+        lib = load_library()
+        n = lib.qpd_plan_schedules(self._h, int(which))
+        if n < 0:
+            check(n)
+        names = []
+        for i in range(n):
+            buf = ctypes.create_string_buffer(256)
+            lib.qpd_plan_schedule_name(self._h, int(which), i, buf, 256)
+            names.append(buf.value.decode())
+        return names
+
+    def schedule(self, which):
+        rc = load_library().qpd_plan_get_schedule(self._h, int(which))
+        if rc < 0:
+            check(rc)
+        return rc
+
+    def set_schedule(self, which, index):
+        check(load_library().qpd_plan_set_schedule(self._h, int(which), int(index)))
+        self.launches = int(load_library().qpd_launches_per_volume(self._h))
+
     def close(self):
         if getattr(self, "_h", None):
             load_library().qpd_plan_destroy(self._h)
@@ -183,3 +221,15 @@ def fft3d(x, precision=None, stream=None):
     check(load_library().qpd_fft3d(dims, prec, ctypes.c_void_p(x.data_ptr()), nf,
                                    stream if stream is not None else current_stream()))
     return x
+
+
+def fast_lengths(role):
+    """Transform lengths with a compile-time kernel of this build for a role (name or number)."""
+    lib = load_library()
+    r = ROLES[role] if isinstance(role, str) else int(role)
+    n = lib.qpd_fast_lengths(r, None, 0)
+    if n < 0:
+        check(n)
+    buf = (ctypes.c_int64 * max(1, n))()
+    lib.qpd_fast_lengths(r, buf, n)
+    return [int(buf[i]) for i in range(n)]

@@ -108,7 +108,8 @@ class Batch:
     (snapshot index, reflection index), default every pair.
     """
 
-    def __init__(self, snapshots, reflections, jobs=None, precision=32, n_streams=4, graph=True, plans=None):
+    def __init__(self, snapshots, reflections, jobs=None, precision=32, n_streams=4, graph=True, plans=None,
+                 tune=False, tune_cache=None):
         import torch
         self.snapshots = snapshots
         self.reflections = reflections
@@ -135,6 +136,22 @@ class Batch:
                                 sum(j.out.numel() * 8 for j in self.jobs), per_stream_cost=load)
         self.graph = None
         self.use_graph = bool(graph)
+        self.tuning = []
+        if tune:
+            self.tune(tune_cache)
+
+    def tune(self, cache=None):
+        """Autotune the schedules of every distinct plan on its first job (before capture)."""
+        from .autotune import tune
+        seen = set()
+        for j in self.jobs:
+            if id(j.plan) in seen:
+                continue
+            seen.add(id(j.plan))
+            snap = self.snapshots[j.snapshot]
+            self.tuning.append(tune(j.plan, snap.P, snap.U, snap.phase, j.out, self.work[j.stream],
+                                    getattr(snap, "fraction", None), getattr(snap, "region", None), cache=cache))
+        return self.tuning
 
     def _enqueue(self):
         import torch

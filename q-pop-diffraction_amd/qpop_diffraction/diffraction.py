@@ -139,6 +139,15 @@ class Engine:
             self._work[key] = torch.empty(max(1, key), dtype=torch.uint8, device="cuda")
         return self._work[key]
 
+    def tune(self, vol: Volume, cache=None, out=None):
+        """Autotune the launch schedules of the plan for this volume's layout and precision."""
+        import torch
+        from .autotune import tune
+        plan = self.plan(vol.dims, vol.axis, vol.precision)
+        if out is None:
+            out = torch.empty(vol.n, dtype=torch.float64, device="cuda")
+        return tune(plan, vol.P, vol.U, vol.phase, out, self.workspace(plan), vol.fraction, vol.region, cache=cache)
+
     def intensity(self, vol: Volume, out=None, stream=None):
         """fftshifted intensity (device float64 tensor shaped like the layout) of a volume."""
         import torch

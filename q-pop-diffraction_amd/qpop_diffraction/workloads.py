@@ -287,3 +287,67 @@ def serve_store(precision=32, seed0=1000):
         return hex_snapshot(key, *grid, seed=seed0 + int(sid), precision=precision)
 
     return SnapshotStore(spec, load)
+
+
+# -- the large single volume -----------------------------------------------------
+
+A_STO = 0.3905  # nm, SrTiO3-like substrate
+
+
+@dataclass
+class LargeVolume:
+    """Device fields of one volume in the reference's (x, y, z) C order (z fastest)."""
+    params: SizeContext
+    atoms: AtomList
+    dims: tuple   # memory dims, fastest first: (nz, ny, nx)
+    axis: tuple   # (2, 1, 0)
+    P: object
+    U: object
+    phase: object
+    name: str
+
+
+def vortex(nx=4000, ny=4000, nz=68, period_x=10, period_z=12, substrate=16, noise=0.05, strain=-0.01, seed=5,
+           precision=32, a=synth.A_PTO, slab=200, device="cuda"):
+    """Polar-vortex superlattice on a substrate, generated in x slabs straight into the z-fastest layout.
+
+    Film (z >= substrate): P = 0.5 (sin(2 pi z / period_z) cos(2 pi x / period_x), 0, cos(2 pi z / period_z))
+    C/m^2 plus N(0, noise * 0.5) on each component; substrate cells (z < substrate) have P = 0 and
+    phase -1 (a non-ferroelectric layer: no scattering, but its lattice mismatch enters the
+    u_z ramp as dc = a_STO / a - 1).  u = strain * (x dx, y dy) in-plane plus that ramp.
+    """
+    import torch
+    n = nx * ny * nz
+    real = torch.float32 if precision == 32 else torch.float64
+    P = torch.empty((3, n), dtype=real, device=device)
+    U = torch.empty((3, n), dtype=torch.float64, device=device)
+    phase = torch.empty(n, dtype=torch.int8, device=device)
+    gen = torch.Generator(device=device).manual_seed(seed)
+    z = torch.arange(nz, device=device, dtype=torch.float64)
+    y = torch.arange(ny, device=device, dtype=torch.float64)
+    dc = torch.where(z < substrate, A_STO / a - 1.0, 0.0)
+    uz = (torch.cumsum(dc, 0) - 0.5 * dc) * a  # (nz,)
+    film = (z >= substrate).view(1, 1, nz)
+    amp = 0.5
+    for x0 in range(0, nx, slab):
+        x1 = min(nx, x0 + slab)
+        x = torch.arange(x0, x1, device=device, dtype=torch.float64).view(-1, 1, 1)
+        shape = (x1 - x0, ny, nz)
+        sz = torch.sin(2 * math.pi * z / period_z).view(1, 1, nz)
+        cz = torch.cos(2 * math.pi * z / period_z).view(1, 1, nz)
+        cx = torch.cos(2 * math.pi * x / period_x)
+        lo, hi = x0 * ny * nz, x1 * ny * nz
+        comps = (amp * sz * cx, torch.zeros((1, 1, 1), device=device, dtype=torch.float64), amp * cz)
+        for d in range(3):
+            val = comps[d].expand(shape) + (noise * amp) * torch.randn(shape, generator=gen, device=device,
+                                                                          dtype=torch.float64)
+            P[d, lo:hi] = torch.where(film, val, 0.0).reshape(-1).to(real)
+        U[0, lo:hi] = (strain * a * x).expand(shape).reshape(-1)
+        U[1, lo:hi] = (strain * a * y.view(1, ny, 1)).expand(shape).reshape(-1)
+        U[2, lo:hi] = uz.view(1, 1, nz).expand(shape).reshape(-1)
+        phase[lo:hi] = torch.where(film, 0, -1).to(torch.int8).expand(shape).reshape(-1)
+    params = box(nx, ny, nz, a, 1)
+    params.strainAvg = np.array([strain, strain, 0.0, 0.0, 0.0, 0.0])
+    atoms = pto_atoms((0, 0, 2))
+    return LargeVolume(params, atoms, (nz, ny, nx), (2, 1, 0), P, U, phase,
+                       f"polar-vortex superlattice {nx}x{ny}x{nz} on a {substrate}-cell substrate")

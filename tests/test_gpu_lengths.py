@@ -1,0 +1,120 @@
+"""Every compile-time-length kernel and every launch schedule against numpy.fft / the oracle.
+
+* qpd_fft3d on each registered length of the plain contiguous and strided roles
+  (FP32, to 2e-6 of the largest output, like test_gpu_fft.py);
+* the volume path with each registered length on each memory dim it has a kernel for
+  (source pass on dim 0, 3 -> 2 pass on dim 1, intensity pass on dim 2), FP32 against
+  the FP64 oracle: max |dI| <= 1e-5 of the peak and the same peak voxel;
+* long strided lengths with no kernel of their own run split into two launches
+  (N = N1 x N2 up to 4000, the length of the 4000 x 4000 x 68 volume), same bound;
+* on one plan, every schedule of every pass gives the same bound.
+"""
+
+import numpy as np
+import pytest
+
+from helpers import rel_to_peak
+from qpop_diffraction import workloads
+from qpop_diffraction._native import fast_lengths
+from qpop_diffraction.batch import PlanCache, Reflection, resident_job
+
+pytestmark = pytest.mark.gpu
+
+FP32_PEAK_TOL = 1e-5
+
+
+@pytest.fixture(scope="module")
+def refl():
+    return Reflection((1, 0, 3), workloads.pto_atoms((1, 0, 3)))
+
+
+def fft_check(shape, nf=2):
+    import torch
+    from qpop_diffraction._native import fft3d
+    rng = np.random.default_rng(sum(shape))
+    x = rng.standard_normal((nf,) + shape) + 1j * rng.standard_normal((nf,) + shape)
+    t = torch.from_numpy(x).to(dtype=torch.complex64, device="cuda").contiguous()
+    fft3d(t)
+    got = t.cpu().numpy()
+    ref = np.fft.fftn(x, axes=(1, 2, 3))
+    return np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+
+
+def test_fft3d_every_contiguous_length():
+    worst = 0.0
+    for N in fast_lengths("contiguous"):
+        err = fft_check((2, 3, N))
+        worst = max(worst, err)
+        assert err < 2e-6, (N, err)
+    print(f"contiguous lengths: worst {worst:.2e}")
+
+
+def test_fft3d_every_strided_length():
+    worst = 0.0
+    for N in fast_lengths("strided1"):
+        for shape in ((N, 3, 5), (3, N, 5)):
+            err = fft_check(shape, nf=1)
+            worst = max(worst, err)
+            assert err < 2e-6, (shape, err)
+    print(f"strided lengths: worst {worst:.2e}")
+
+
+def volume_check(refl, nx, ny, nz, seed=0, expect=None, schedules=None):
+    """FP32 volume of a twin snapshot against the oracle; returns max |dI| / peak."""
+    import torch
+    from qpd_oracle import diffraction_fp64
+    snap = workloads.twin_snapshot(("len", nx, ny, nz), nx, ny, nz, seed=seed)
+    plans = PlanCache(32)
+    plan = plans.get(snap, 0, refl)
+    if expect is not None:
+        for d, prefix in expect.items():
+            names = plan.schedules(d)
+            assert names[plan.schedule(d)].startswith(prefix), (d, names)
+    if schedules is not None:
+        for d, idx in schedules.items():
+            plan.set_schedule(d, idx)
+    out = torch.empty(nx * ny * nz, dtype=torch.float64, device="cuda")
+    work = torch.empty(plan.workspace_bytes, dtype=torch.uint8, device="cuda")
+    resident_job(plans, snap, 0, refl, out, work)
+    I = out.cpu().numpy().reshape(nz, ny, nx).transpose(2, 1, 0)
+    ref = diffraction_fp64(snap.params, refl.atoms, *workloads.host_inputs(snap))[0]
+    assert np.argmax(I) == np.argmax(ref), (nx, ny, nz)
+    return rel_to_peak(I, ref)
+
+
+@pytest.mark.parametrize("role,dim", [("source", 0), ("strided", 1), ("intensity", 2)])
+def test_volume_every_length(refl, role, dim):
+    worst = 0.0
+    for N in fast_lengths(role):
+        shape = [6, 5, 4]
+        shape[dim] = N
+        print(f"{role} N={N} grid {shape}", flush=True)
+        err = volume_check(refl, *shape, expect={dim: "fast"})
+        worst = max(worst, err)
+        assert err <= FP32_PEAK_TOL, (N, err)
+    print(f"{role}: worst max|dI|/peak {worst:.2e}")
+
+
+@pytest.mark.parametrize("N", [640, 1000, 2000, 4000])
+@pytest.mark.parametrize("dim", [1, 2])
+def test_split_long_strided(refl, N, dim):
+    shape = [5, 4, 3]
+    shape[dim] = N
+    print(f"split N={N} on dim {dim}, grid {shape}", flush=True)
+    err = volume_check(refl, *shape, expect={dim: "split"})
+    print(f"  max|dI|/peak {err:.2e}")
+    assert err <= FP32_PEAK_TOL
+
+
+@pytest.mark.parametrize("shape", [(40, 1000, 6), (24, 20, 500)])
+def test_every_schedule(refl, shape):
+    import torch
+    snap = workloads.twin_snapshot(("sched",) + shape, *shape, seed=1)
+    plan = PlanCache(32).get(snap, 0, refl)
+    counts = [len(plan.schedules(d)) for d in range(3)]
+    print(f"{shape}: schedules per pass {counts}", flush=True)
+    for d in range(3):
+        for idx, name in enumerate(plan.schedules(d)):
+            print(f"  pass {d} schedule {idx}: {name}", flush=True)
+            err = volume_check(refl, *shape, seed=1, schedules={d: idx})
+            assert err <= FP32_PEAK_TOL, (d, name, err)
