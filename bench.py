@@ -43,6 +43,9 @@ def parse():
     ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--serve-rate", type=float, default=20.0, help="mean arrivals per second")
     ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch schedules")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="first copy BYTES device to device three times (a kernel of known traffic for counters)")
     return ap.parse_args()
 
 
@@ -285,8 +288,19 @@ def main():
     plan = engine.plan(vol.dims, vol.axis, 32)
     work = engine.workspace(plan)
     out = torch.empty(n, dtype=torch.float64, device="cuda")
+    if args.copy_calibration > 0:
+        src = torch.ones(args.copy_calibration // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy calibration: 3 x {args.copy_calibration} bytes")
+        del src, dst
     from qpop_diffraction.autotune import ScheduleCache
-    tuned = engine.tune(vol, cache=ScheduleCache(None), out=out)
+    if args.no_tune:
+        tuned = {"passes": [{"pass": d, "chosen": plan.schedules(d)[plan.schedule(d)]} for d in range(3)]}
+    else:
+        tuned = engine.tune(vol, cache=ScheduleCache(None), out=out)
     log("schedules: " + "; ".join(f"pass {p['pass']}: {p['chosen']}" for p in tuned["passes"]))
 
     log(f"timing FP32: {args.warmup} warmup + {args.steps} steps")

@@ -40,6 +40,17 @@ struct Sizes {
     static_assert(R0_ * (R1_ > 1 ? R1_ : 1) * (R2_ > 1 ? R2_ : 1) * (R3_ > 1 ? R3_ : 1) == N_, "radices must multiply to N");
 };
 
+// Workgroups are dispatched to the 8 XCDs round robin (block b to XCD b % 8), and each
+// XCD has its own L2.  Strided passes read rows of G columns; neighbouring column tiles
+// share 128-byte lines, so the logical block b / 8 + (b % 8) * (nblocks / 8) puts runs of
+// consecutive tiles on one XCD, where the second read of a line hits L2.  The last
+// nblocks % 8 blocks keep their index (a bijection on [0, nblocks)).
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nblocks) {
+    constexpr uint32_t kXcd = 8;
+    const uint32_t per = nblocks / kXcd;
+    return b < per * kXcd ? (b % kXcd) * per + b / kXcd : b;
+}
+
 // Per-cell inputs of the FP32 source with the phase split exp(-i qC.u_a) =
 // exp(-i qC.U) exp(-i g_a.P): e0 = exp(-i qC.U) is formed once per cell with the
 // FP64-reduced sincos; per atom only g_a.P (a few radians) remains.
@@ -425,8 +436,9 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided(PassArgs<T> a) {
     cx<T>* Wl = bufB + TILE;
     const int tid = threadIdx.x;
     const int g = tid % G, j = tid / G;
-    const uint32_t t0 = blockIdx.x % a.ntiles0;
-    const uint32_t outer_q = blockIdx.x / a.ntiles0;
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t t0 = blk % a.ntiles0;
+    const uint32_t outer_q = blk / a.ntiles0;
     // unsplit passes: q = 0 and unit steps, known at compile time
     const uint32_t outer = SPLIT ? fdiv(outer_q, a.divQ) : outer_q;
     const uint32_t q = SPLIT ? outer_q - outer * a.nq : 0u;
@@ -530,8 +542,9 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_split(PassArgs<T> a) {
     cx<T>* Wl = bufB + TILE;
     const int tid = threadIdx.x;
     const int g = tid % G, j = tid / G;
-    const uint32_t t0 = blockIdx.x % a.ntiles0;
-    const uint32_t outer_q = blockIdx.x / a.ntiles0;
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t t0 = blk % a.ntiles0;
+    const uint32_t outer_q = blk / a.ntiles0;
     const uint32_t outer = fdiv(outer_q, a.divQ);
     const uint32_t q = outer_q - outer * a.nq;
     const uint32_t i0 = t0 * G + g;
