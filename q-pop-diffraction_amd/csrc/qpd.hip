@@ -208,6 +208,16 @@ struct qpd_plan {
     int n_phase;
     int n_atom;
     int nY, nZ;  // classes after pass 0 and pass 1
+    // atom classes: y-class = equal (R[ax1], R[ax2]), z-class = equal R[ax2]
+    std::vector<int> ycls, zcls;
+    std::vector<std::pair<double, double>> ykeys;
+    std::vector<double> zkeys;
+    // source groups: atoms with equal su (all phases) and proportional form factors
+    int nS = 0;
+    std::vector<int> sgrp, srep;             // group of each atom, representative of each group
+    std::vector<std::complex<double>> phi;   // fE_a = phi_a fE_rep(group)
+    int grouped = 0;                         // 1: pass 0 transforms groups, pass 1 mixes them
+    int n1fields() const { return 3 * (grouped ? nS : nY); }
     size_t csize;
     AxisFFT fft[3];
     PassPlan pass[3];                 // the schedule each pass runs (one of cand)
@@ -225,6 +235,9 @@ struct qpd_plan {
     const int* chunk_coef[3];
     const int* group_first[3];
     const void* atomtab;
+    const void* mixc = nullptr;       // grouped plans: X_{Y,S}(k0) tables
+    const int* pair_first = nullptr;
+    const int* pair_y = nullptr;
     double qC[3];
 };
 
@@ -254,6 +267,65 @@ struct TableBuilder {
     }
 };
 
+// Atom classes and source groups of a plan.
+void classify(qpd_plan* p, const qpd_grid* g, const qpd_crystal* c) {
+    const int ax1 = g->axis[1], ax2 = g->axis[2];
+    const int na = c->n_atom, np = c->n_phase;
+    p->ycls.assign(na, 0);
+    p->ykeys.clear();
+    p->zkeys.clear();
+    for (int a = 0; a < na; ++a) {
+        std::pair<double, double> key(c->R[3 * a + ax1], c->R[3 * a + ax2]);
+        auto it = std::find(p->ykeys.begin(), p->ykeys.end(), key);
+        if (it == p->ykeys.end()) {
+            p->ykeys.push_back(key);
+            p->ycls[a] = (int)p->ykeys.size() - 1;
+        } else {
+            p->ycls[a] = (int)(it - p->ykeys.begin());
+        }
+    }
+    p->zcls.assign(p->ykeys.size(), 0);
+    for (size_t y = 0; y < p->ykeys.size(); ++y) {
+        auto it = std::find(p->zkeys.begin(), p->zkeys.end(), p->ykeys[y].second);
+        if (it == p->zkeys.end()) {
+            p->zkeys.push_back(p->ykeys[y].second);
+            p->zcls[y] = (int)p->zkeys.size() - 1;
+        } else {
+            p->zcls[y] = (int)(it - p->zkeys.begin());
+        }
+    }
+    p->nY = (int)p->ykeys.size();
+    p->nZ = (int)p->zkeys.size();
+    // source groups: same su in every phase and f_pa = phi_a f_p,rep for every phase
+    auto su = [&](int d, int k, int ph, int a) { return c->su[((d * c->n_struc + k) * np + ph) * na + a]; };
+    auto f = [&](int ph, int a) { return std::complex<double>(c->f_re[ph * na + a], c->f_im[ph * na + a]); };
+    p->sgrp.assign(na, -1);
+    p->srep.clear();
+    p->phi.assign(na, 0.0);
+    for (int a = 0; a < na; ++a) {
+        for (int s = 0; s < (int)p->srep.size() && p->sgrp[a] < 0; ++s) {
+            const int r = p->srep[s];
+            bool same = true;
+            for (int ph = 0; ph < np && same; ++ph)
+                for (int d = 0; d < 3 && same; ++d)
+                    for (int k = 0; k < 3 && same; ++k) same = su(d, k, ph, a) == su(d, k, ph, r);
+            if (!same || f(0, r) == 0.0) continue;
+            const std::complex<double> ratio = f(0, a) / f(0, r);
+            for (int ph = 1; ph < np && same; ++ph)
+                same = std::abs(f(ph, a) - ratio * f(ph, r)) <= 1e-15 * std::abs(f(ph, a));
+            if (!same) continue;
+            p->sgrp[a] = s;
+            p->phi[a] = ratio;
+        }
+        if (p->sgrp[a] < 0) {
+            p->sgrp[a] = (int)p->srep.size();
+            p->srep.push_back(a);
+            p->phi[a] = 1.0;
+        }
+    }
+    p->nS = (int)p->srep.size();
+}
+
 template <typename T>
 int build_tables(qpd_plan* p, const qpd_grid* g, const qpd_crystal* c) {
     TableBuilder tb;
@@ -262,33 +334,12 @@ int build_tables(qpd_plan* p, const qpd_grid* g, const qpd_crystal* c) {
     const int64_t n = p->ncell;
     const double h_of[3] = {g->h[0], g->h[1], g->h[2]};
 
-    // atom classes: pass-0 groups by equal (R[ax1], R[ax2]); pass-1 groups by equal R[ax2]
-    const int ax0 = g->axis[0], ax1 = g->axis[1], ax2 = g->axis[2];
-    std::vector<int> ycls(c->n_atom), zcls;
-    std::vector<std::pair<double, double>> ykeys;
-    std::vector<double> zkeys;
-    for (int a = 0; a < c->n_atom; ++a) {
-        std::pair<double, double> key(c->R[3 * a + ax1], c->R[3 * a + ax2]);
-        auto it = std::find(ykeys.begin(), ykeys.end(), key);
-        if (it == ykeys.end()) {
-            ykeys.push_back(key);
-            ycls[a] = (int)ykeys.size() - 1;
-        } else {
-            ycls[a] = (int)(it - ykeys.begin());
-        }
-    }
-    zcls.resize(ykeys.size());
-    for (size_t y = 0; y < ykeys.size(); ++y) {
-        auto it = std::find(zkeys.begin(), zkeys.end(), ykeys[y].second);
-        if (it == zkeys.end()) {
-            zkeys.push_back(ykeys[y].second);
-            zcls[y] = (int)zkeys.size() - 1;
-        } else {
-            zcls[y] = (int)(it - zkeys.begin());
-        }
-    }
-    p->nY = (int)ykeys.size();
-    p->nZ = (int)zkeys.size();
+    const int ax0 = g->axis[0];
+    const std::vector<int>& ycls = p->ycls;
+    const std::vector<int>& zcls = p->zcls;
+    const auto& ykeys = p->ykeys;
+    const auto& zkeys = p->zkeys;
+    size_t offMix = 0, offPairF = 0, offPairY = 0;
 
     for (int d = 0; d < 3; ++d) {
         const int64_t N = g->dims[d];
@@ -324,7 +375,48 @@ int build_tables(qpd_plan* p, const qpd_grid* g, const qpd_crystal* c) {
                 ci.push_back(v.imag());
             }
         };
-        if (d == 0) {
+        if (d == 0 && p->grouped) {
+            // pass 0 over source groups: group = source group, one chunk (its representative
+            // atom) with the identity coefficient; the atom coefficients move to pass 1:
+            // X_{Y,S}(k) = sum over a in Y and S of phi_a exp(-i G.R_a) / (V n) exp(-i k R_a,ax0) s(k)
+            for (int sg = 0; sg < p->nS; ++sg) {
+                first.push_back((int)src.size());
+                src.push_back(p->srep[sg]);
+                coef.push_back(-1);
+            }
+            first.push_back((int)src.size());
+            cr.assign(1, 0.0);
+            ci.assign(1, 0.0);
+            std::vector<double> xr((size_t)p->nY * p->nS * N, 0.0), xi((size_t)p->nY * p->nS * N, 0.0);
+            for (int a = 0; a < c->n_atom; ++a) {
+                const double gr = c->G[0] * c->R[3 * a] + c->G[1] * c->R[3 * a + 1] + c->G[2] * c->R[3 * a + 2];
+                const std::complex<double> scale = p->phi[a] * std::polar(1.0, -gr) / (c->V * (double)n);
+                const size_t base = ((size_t)ycls[a] * p->nS + p->sgrp[a]) * N;
+                for (int64_t m = 0; m < N; ++m) {
+                    const std::complex<double> v =
+                        scale * std::polar(1.0, -kk[m] * c->R[3 * a + ax0]) * shape_factor(kk[m], h);
+                    xr[base + m] += v.real();
+                    xi[base + m] += v.imag();
+                }
+            }
+            offMix = tb.add_cplx<T>(xr, xi);
+            // pairs (group S, z-class z) -> y-classes Y in z with atoms of S
+            std::vector<int> pf, py;
+            for (int sg = 0; sg < p->nS; ++sg)
+                for (int z = 0; z < p->nZ; ++z) {
+                    pf.push_back((int)py.size());
+                    for (int y = 0; y < p->nY; ++y) {
+                        if (zcls[y] != z) continue;
+                        bool any = false;
+                        for (int a = 0; a < c->n_atom; ++a) any = any || (ycls[a] == y && p->sgrp[a] == sg);
+                        if (any) py.push_back(y);
+                    }
+                }
+            pf.push_back((int)py.size());
+            if (py.empty()) py.push_back(0);
+            offPairF = tb.add(pf.data(), pf.size() * sizeof(int));
+            offPairY = tb.add(py.data(), py.size() * sizeof(int));
+        } else if (d == 0) {
             // pass 0: group = y-class, chunk = atom; c_a = exp(-i G.R_a) / (V n) exp(-i k R_a,ax0) s(k)
             for (int y = 0; y < p->nY; ++y) {
                 first.push_back((int)src.size());
@@ -399,6 +491,11 @@ int build_tables(qpd_plan* p, const qpd_grid* g, const qpd_crystal* c) {
     }
     p->atomtab = base + offAtom;
     for (auto& kv : offWs) p->Wlen[kv.first] = base + kv.second;
+    if (p->grouped) {
+        p->mixc = base + offMix;
+        p->pair_first = reinterpret_cast<const int*>(base + offPairF);
+        p->pair_y = reinterpret_cast<const int*>(base + offPairY);
+    }
     return QPD_OK;
 }
 
@@ -482,7 +579,7 @@ hipError_t launch_pass(const Launch& L, const PassArgs<T>& a, bool source, bool 
 // then the generic kernel.  A split is the default only where the length has no kernel
 // of its own.
 int pass_candidates(const AxisFFT& ax, int d, const int64_t dims[3], int nin, size_t csize, bool intensity_last,
-                    int atomtab_entries, bool allow_split, std::vector<PassPlan>& out) {
+                    int atomtab_entries, bool allow_split, std::vector<PassPlan>& out, int mix_role = -1) {
     out.clear();
     const int hard = max_lds_bytes();
     const int soft = std::min(hard, 65536);
@@ -495,7 +592,7 @@ int pass_candidates(const AxisFFT& ax, int d, const int64_t dims[3], int nin, si
         if (contig)
             role = (nin == 4 && atomtab_entries <= kMaxAtomTab) ? kRoleSource : (nin == 1 ? kRoleContig : -1);
         else if (nin == 3)
-            role = kRoleStrided;
+            role = (d == 1 && mix_role >= 0) ? mix_role : kRoleStrided;
         else if (nin == 2 && intensity_last && d == 2)
             role = kRoleIntensity;
         else if (nin == 1)
@@ -543,7 +640,8 @@ int pass_candidates(const AxisFFT& ax, int d, const int64_t dims[3], int nin, si
     PassPlan gp;
     Launch& L = gp.L[0];
     const int ept_max = contig ? (nin > 1 ? 4 : 8) : 8;
-    if (choose_launch(ax, nin, contig, csize, lines, soft, hard, ept_max, L)) {
+    // the generic kernels have no source-group form of pass 1
+    if (!(d == 1 && mix_role >= 0) && choose_launch(ax, nin, contig, csize, lines, soft, hard, ept_max, L)) {
         L.blocks = contig ? (lines + L.nl - 1) / L.nl : (dims[0] + L.nl - 1) / L.nl * n_other;
         if (L.blocks <= 0x7fffffffLL) out.push_back(gp);
     }
@@ -581,7 +679,7 @@ template <typename T>
 int run_intensity(qpd_plan* p, const qpd_fields* f, double* I, void* work, hipStream_t st, int only = -1) {
     const int64_t n = p->ncell;
     cx<T>* w1 = static_cast<cx<T>*>(work);
-    cx<T>* w2 = w1 + (size_t)3 * p->nY * n;
+    cx<T>* w2 = w1 + (size_t)p->n1fields() * n;
     for (int d = 0; d < 3; ++d) {
         if (only >= 0 && d != only) continue;
         const PassPlan& pp = p->pass[d];
@@ -620,6 +718,12 @@ int run_intensity(qpd_plan* p, const qpd_fields* f, double* I, void* work, hipSt
                 a.out = w2;
             else
                 a.I = I;
+            if (d == 1 && p->grouped) {
+                a.nsrc = p->nS;
+                a.pair_first = p->pair_first;
+                a.pair_y = p->pair_y;
+                a.mixc = static_cast<const cx<T>*>(p->mixc);
+            }
         }
         if (pp.nlaunch == 2) {
             // step 1: N2-point sub-transforms of every input field, twiddled, in place
@@ -635,7 +739,7 @@ int run_intensity(qpd_plan* p, const qpd_fields* f, double* I, void* work, hipSt
             s1.in_qmul = s1.out_qmul = 1;
             s1.in_step = s1.out_step = N1;
             s1.out = const_cast<cx<T>*>(a.in);
-            s1.nfields = (d == 1 ? 3 * p->nY : 2 * p->nZ);
+            s1.nfields = (d == 1 ? p->n1fields() : 2 * p->nZ);
             QPD_HIP(launch_pass<T>(pp.L[0], s1, false, false, st));
             // step 2: N1-point transforms over rows N1 q + y1 into rows q + N2 k1, accumulated
             a.N = (int)N1;
@@ -744,8 +848,11 @@ int run_fft3d(const int64_t dims[3], void* data, int n_fields, hipStream_t st) {
 
 std::string describe_launch(const Launch& L) {
     char buf[160];
+    static const char* role_name[kNumRoles] = {"source", "contiguous", "strided", "intensity", "strided1",
+                                               "split-step", "strided", "intensity", "groups", "groups"};
     if (L.fe)
-        std::snprintf(buf, sizeof(buf), "fast %d (%s) G%d", L.fe->N, L.fe->radices, L.fe->G);
+        std::snprintf(buf, sizeof(buf), "fast %d (%s) G%d %s", L.fe->N, L.fe->radices, L.fe->G,
+                      role_name[L.fe->role]);
     else
         std::snprintf(buf, sizeof(buf), "generic %s nl%d nf%d nthr%d ept%d", L.contiguous ? "contiguous" : "strided",
                       L.nl, L.nf, L.nthr, L.ept);
@@ -846,10 +953,20 @@ int qpd_plan_create(qpd_plan** plan, const qpd_grid* grid, const qpd_crystal* c,
     p->csize = precision == 32 ? sizeof(cx<float>) : sizeof(cx<double>);
     for (int i = 0; i < 3; ++i) p->qC[i] = c->qC[i];
     const int nin[3] = {4, 3, 2};
+    classify(p, grid, c);
+    // source groups: when they are fewer than the y-classes, pass 0 transforms one
+    // representative per group and pass 1 mixes them (FP32 compile-time kernels with one
+    // or two z-classes; QPD_CLASS_FORM=1 keeps the class form for comparison runs)
+    const int mix_role = p->nZ == 1 ? kRoleMix1 : (p->nZ == 2 ? kRoleMix2 : -1);
+    const char* cf = std::getenv("QPD_CLASS_FORM");
+    p->grouped = (precision == 32 && fast_enabled() && mix_role >= 0 && p->nS < p->nY &&
+                  fast_lookup(p->dims[1], mix_role) != nullptr && !(cf && cf[0] == '1'))
+                     ? 1
+                     : 0;
     int rc = QPD_OK;
     for (int d = 0; d < 3 && rc == QPD_OK; ++d) {
         rc = pass_candidates(p->fft[d], d, p->dims, nin[d], p->csize, true, c->n_atom * c->n_phase * kAtomTab, true,
-                             p->cand[d]);
+                             p->cand[d], p->grouped ? mix_role : -1);
         if (rc == QPD_OK) {
             // default: a kernel of the length itself, else the most even split, else generic
             p->sel[d] = 0;
@@ -878,11 +995,11 @@ int qpd_plan_destroy(qpd_plan* plan) {
 
 size_t qpd_workspace_bytes(const qpd_plan* p) {
     if (!p) return 0;
-    return (size_t)(3 * p->nY + 2 * p->nZ) * (size_t)p->ncell * p->csize;
+    return (size_t)(p->n1fields() + 2 * p->nZ) * (size_t)p->ncell * p->csize;
 }
 
 int qpd_fast_lengths(int role, int64_t* lengths, int cap) {
-    if (role < 0 || role >= kNumRoles) return fail(QPD_ERR_ARG, "role must be 0..7");
+    if (role < 0 || role >= kNumRoles) return fail(QPD_ERR_ARG, "role must be 0..9");
     std::vector<int64_t> ls;
     for (const FastEntry& e : fast_registry())
         if (e.role == role && std::find(ls.begin(), ls.end(), (int64_t)e.N) == ls.end()) ls.push_back(e.N);
@@ -941,9 +1058,9 @@ double qpd_pass_bytes_per_cell(const qpd_plan* p, const qpd_fields* f, int pass)
         double in = 3 * real + 3 * 8.0 + 1.0;  // P, U, phase
         if (f && f->region) in += real;
         if (f && f->fraction) in += p->n_phase * real;
-        return in + 3.0 * p->nY * c;
+        return in + (double)p->n1fields() * c;
     }
-    if (pass == 1) return 3.0 * p->nY * c + 2.0 * p->nZ * c;
+    if (pass == 1) return (double)p->n1fields() * c + 2.0 * p->nZ * c;
     return 2.0 * p->nZ * c + 8.0;
 }
 

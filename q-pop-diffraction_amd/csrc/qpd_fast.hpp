@@ -526,6 +526,104 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided(PassArgs<T> a) {
 }
 
 // ---------------------------------------------------------------------------
+// Pass 1 over source groups.  Atoms whose displacement coefficients su (and form-factor
+// ratios across phases) are equal have proportional sources, fE_a = phi_a fE_S, so pass 0
+// can transform one representative per group S and leave the atom coefficients for
+// later: its outputs are [A_S, B_S, C_S] = F0 [E - i k0 U_0, U_1, U_2] of the group.
+// With X_{Y,S}(k0) = sum over atoms a in class Y and group S of phi_a c_a(k0) (the
+// pass-0 coefficients of the class form, a function of the column only), this pass
+// forms for every z-class Z
+//   out_{Z,0} = sum_S sum_{Y in Z} c_Y(k1) X_{Y,S}(k0) [F1 A_S - i k1 F1 B_S],
+//   out_{Z,1} = sum_S sum_{Y in Z} c_Y(k1) X_{Y,S}(k0) F1 C_S,
+// the same outputs as the class form.  Each group's three inputs are transformed once.
+template <typename T, class SZ, int G, int NZ>
+__global__ __launch_bounds__(G* SZ::TPL) void fast_strided_mix(PassArgs<T> a) {
+    constexpr int N = SZ::N, R0 = SZ::rad(0), NR0 = SZ::nr(0), LP = G + 1, RL = SZ::RL, NRL = SZ::NRL;
+    constexpr int TILE = N * LP;
+    constexpr bool ALT = (chunk_writes<SZ, false>() & 1) == 1;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T>* bufA = reinterpret_cast<cx<T>*>(smem_raw);
+    cx<T>* bufB = bufA + TILE;
+    cx<T>* Wl = bufB + TILE;
+    const int tid = threadIdx.x;
+    const int g = tid % G, j = tid / G;
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t t0 = blk % a.ntiles0;
+    const uint32_t outer = blk / a.ntiles0;
+    const uint32_t i0 = t0 * G + g;
+    const bool col_ok = i0 < a.n0;
+    const uint32_t colbase = i0 + outer * a.OS;
+    load_twiddles<T, N>(Wl, a.W, tid, G * SZ::TPL);
+    __syncthreads();
+    constexpr bool PF = SZ::TPL >= 32;
+    const int nitems = 3 * a.nsrc;
+    auto load_item = [&](int it, cx<T>(&dst)[1][SZ::RMAX]) {
+        const cx<T>* in = a.in + (size_t)it * a.ncell + colbase;
+        if (j < NR0) {
+#pragma unroll
+            for (int r = 0; r < R0; ++r) dst[0][r] = col_ok ? in[(uint32_t)(j + r * NR0) * a.S] : cx<T>{(T)0, (T)0};
+        }
+    };
+    cx<T> acc[NZ][2][RL];
+#pragma unroll
+    for (int z = 0; z < NZ; ++z)
+#pragma unroll
+        for (int r = 0; r < RL; ++r) acc[z][0][r] = acc[z][1][r] = {(T)0, (T)0};
+    cx<T> nxt[1][SZ::RMAX];
+    if constexpr (PF) load_item(0, nxt);
+    int it = 0;
+    int par = 0;
+    for (int s = 0; s < a.nsrc; ++s) {
+        cx<T> tA[RL], tC[RL];
+#pragma unroll
+        for (int f = 0; f < 3; ++f, ++it) {
+            cx<T> v[1][SZ::RMAX];
+            if constexpr (PF) {
+#pragma unroll
+                for (int r = 0; r < SZ::RMAX; ++r) v[0][r] = nxt[0][r];
+                if (it + 1 < nitems) load_item(it + 1, nxt);
+            } else {
+                load_item(it, v);
+            }
+            transform<T, SZ, 1, true, LP, false>(par ? bufB : bufA, par ? bufA : bufB, v, j, g, 0, Wl);
+            if (ALT) par ^= 1;
+            if (j < NRL) {
+#pragma unroll
+                for (int r = 0; r < RL; ++r) {
+                    if (f == 0) tA[r] = v[0][r];
+                    if (f == 1) tA[r] = tA[r] + cscale(negi(v[0][r]), a.k[j + r * NRL]);
+                    if (f == 2) tC[r] = v[0][r];
+                }
+            }
+        }
+        if (j < NRL && col_ok) {
+#pragma unroll
+            for (int z = 0; z < NZ; ++z) {
+                for (int p = a.pair_first[s * NZ + z]; p < a.pair_first[s * NZ + z + 1]; ++p) {
+                    const int y = a.pair_y[p];
+                    const cx<T> xc = a.mixc[(size_t)(y * a.nsrc + s) * a.n0 + i0];
+#pragma unroll
+                    for (int r = 0; r < RL; ++r) {
+                        const cx<T> cf = cmul(a.coef[y * a.Nc + j + r * NRL], xc);
+                        acc[z][0][r] = acc[z][0][r] + cmul(cf, tA[r]);
+                        acc[z][1][r] = acc[z][1][r] + cmul(cf, tC[r]);
+                    }
+                }
+            }
+        }
+    }
+    if (!col_ok || j >= NRL) return;
+#pragma unroll
+    for (int z = 0; z < NZ; ++z)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            cx<T>* o = a.out + (size_t)(2 * z + q) * a.ncell + colbase;
+#pragma unroll
+            for (int r = 0; r < RL; ++r) o[(uint32_t)(j + r * NRL) * a.S] = acc[z][q][r];
+        }
+}
+
+// ---------------------------------------------------------------------------
 // First step of a split strided transform of length Nc = N1 N, in place on every
 // field: for the line (column i0, q < N1, outer) the N points at rows q + N1 y2 are
 // transformed (length N = SZ::N) and output k2 is multiplied by exp(-2 pi i q k2 / Nc)

@@ -33,8 +33,12 @@ hipError_t fast_launch(const Launch& L, const PassArgs<float>& a, hipStream_t st
         return launch_one(fast_split<float, SZ, G>, L, st, a);
     else if constexpr (ROLE == kRoleStridedB)
         return launch_one(fast_strided<float, SZ, G, 3, false, true>, L, st, a);
-    else
+    else if constexpr (ROLE == kRoleIntensityB)
         return launch_one(fast_strided<float, SZ, G, 2, true, true>, L, st, a);
+    else if constexpr (ROLE == kRoleMix1)
+        return launch_one(fast_strided_mix<float, SZ, G, 1>, L, st, a);
+    else
+        return launch_one(fast_strided_mix<float, SZ, G, 2>, L, st, a);
 }
 
 template <class SZ, int ROLE, int G>
@@ -66,7 +70,8 @@ void add_fast_variants(std::vector<FastEntry>& v, const char* radices) {
 }
 
 constexpr unsigned kRolesContig = (1u << kRoleSource) | (1u << kRoleContig);
-constexpr unsigned kRolesStrided = (1u << kRoleStrided) | (1u << kRoleIntensity) | (1u << kRoleStrided1);
+constexpr unsigned kRolesStrided = (1u << kRoleStrided) | (1u << kRoleIntensity) | (1u << kRoleStrided1) |
+                                   (1u << kRoleMix1) | (1u << kRoleMix2);
 constexpr unsigned kRolesSplitSteps = (1u << kRoleSplit) | (1u << kRoleStridedB) | (1u << kRoleIntensityB);
 
 template <class SZ, unsigned ROLES>
@@ -82,6 +87,8 @@ void add_fast_roles(std::vector<FastEntry>& v, const char* radices) {
         add_fast<SZ, kRoleStridedB, fast_tile(kRoleStridedB, SZ::TPL)>(v, radices);
     if constexpr ((ROLES >> kRoleIntensityB) & 1u)
         add_fast<SZ, kRoleIntensityB, fast_tile(kRoleIntensityB, SZ::TPL)>(v, radices);
+    if constexpr ((ROLES >> kRoleMix1) & 1u) add_fast_variants<SZ, kRoleMix1>(v, radices);
+    if constexpr ((ROLES >> kRoleMix2) & 1u) add_fast_variants<SZ, kRoleMix2>(v, radices);
 }
 
 }  // namespace qpd

@@ -67,6 +67,13 @@ struct PassArgs {
     int Nc;                   // length of the k and coefficient tables (the full length)
     const cx<T>* Wsplit;      // exp(-2 pi i m / Nc), m < Nc: twiddles between the two steps
     int nfields;              // fields transformed in place by the first step
+    // pass 1 over source groups (fast_strided_mix): nsrc groups of 3 inputs; for group s
+    // and z-class z the y-classes pair_y[pair_first[s NZ + z] .. pair_first[s NZ + z + 1])
+    // contribute with column factor mixc[(y nsrc + s) n0 + i0]
+    int nsrc;
+    const int* pair_first;
+    const int* pair_y;
+    const cx<T>* mixc;
     // data
     const cx<T>* in;
     cx<T>* out;

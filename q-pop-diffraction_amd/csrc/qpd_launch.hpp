@@ -26,7 +26,9 @@ enum FastRole {
     kRoleSplit = 5,      // strided length-N2 sub-transforms of a split length, with the twiddle, in place
     kRoleStridedB = 6,   // kRoleStrided as the second step of a split length
     kRoleIntensityB = 7, // kRoleIntensity as the second step of a split length
-    kNumRoles = 8,
+    kRoleMix1 = 8,       // strided pass 1 over source groups, one z-class
+    kRoleMix2 = 9,       // strided pass 1 over source groups, two z-classes
+    kNumRoles = 10,
 };
 
 struct FastEntry;

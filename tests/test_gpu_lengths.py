@@ -82,10 +82,17 @@ def volume_check(refl, nx, ny, nz, seed=0, expect=None, schedules=None):
     return rel_to_peak(I, ref)
 
 
-@pytest.mark.parametrize("role,dim", [("source", 0), ("strided", 1), ("intensity", 2)])
-def test_volume_every_length(refl, role, dim):
+@pytest.mark.parametrize("role,dim,form", [("source", 0, "groups"), ("source", 0, "class"), ("strided", 1, "class"),
+                                           ("groups", 1, "groups"), ("intensity", 2, "groups")])
+def test_volume_every_length(refl, role, dim, form, monkeypatch):
+    """Pass 1 runs the source-group kernel for PbTiO3 unless QPD_CLASS_FORM=1."""
+    if form == "class":
+        monkeypatch.setenv("QPD_CLASS_FORM", "1")
+    else:
+        monkeypatch.delenv("QPD_CLASS_FORM", raising=False)
     worst = 0.0
-    for N in fast_lengths(role):
+    lengths = fast_lengths("strided_groups2" if role == "groups" else role)
+    for N in lengths:
         shape = [6, 5, 4]
         shape[dim] = N
         print(f"{role} N={N} grid {shape}", flush=True)
@@ -118,3 +125,42 @@ def test_every_schedule(refl, shape):
             print(f"  pass {d} schedule {idx}: {name}", flush=True)
             err = volume_check(refl, *shape, seed=1, schedules={d: idx})
             assert err <= FP32_PEAK_TOL, (d, name, err)
+
+
+@pytest.mark.parametrize("shape,axis", [((48, 40, 24), (0, 1, 2)), ((24, 40, 48), (2, 1, 0)), ((104, 64, 30), (0, 1, 2))])
+def test_source_groups_match_class_form(refl, shape, axis, monkeypatch):
+    """PbTiO3 has two source groups (Pb; Ti and the three O share su) and four y-classes, so
+    FP32 plans transform the groups in pass 0 and mix them in pass 1; QPD_CLASS_FORM=1 keeps
+    the per-class form.  Both agree to FP32 rounding (2e-6 of the peak) and with the oracle."""
+    import torch
+    from qpd_oracle import diffraction_fp64
+    snap = workloads.twin_snapshot(("grp",) + shape, *shape, seed=9)
+    if axis != (0, 1, 2):
+        nx, ny, nz = shape
+        snap.P = snap.P.reshape(3, nz, ny, nx).permute(0, 3, 2, 1).contiguous().reshape(3, -1)
+        snap.U = snap.U.reshape(3, nz, ny, nx).permute(0, 3, 2, 1).contiguous().reshape(3, -1)
+        snap.phase = snap.phase.reshape(nz, ny, nx).permute(2, 1, 0).contiguous().reshape(-1)
+        snap.dims, snap.axis = (nz, ny, nx), (2, 1, 0)
+    outs = {}
+    for form in ("groups", "class"):
+        if form == "class":
+            monkeypatch.setenv("QPD_CLASS_FORM", "1")
+        else:
+            monkeypatch.delenv("QPD_CLASS_FORM", raising=False)
+        plans = PlanCache(32)
+        plan = plans.get(snap, 0, refl)
+        names = plan.schedules(1)
+        assert ("groups" in names[plan.schedule(1)]) == (form == "groups"), names
+        out = torch.empty(plan.n, dtype=torch.float64, device="cuda")
+        work = torch.empty(plan.workspace_bytes, dtype=torch.uint8, device="cuda")
+        resident_job(plans, snap, 0, refl, out, work)
+        outs[form] = out.cpu().numpy()
+    monkeypatch.delenv("QPD_CLASS_FORM", raising=False)
+    rel = np.max(np.abs(outs["groups"] - outs["class"])) / np.max(outs["class"])
+    print(f"{shape} axis {axis}: groups vs class max|dI|/peak {rel:.2e}")
+    assert rel <= 2e-6
+    nx, ny, nz = shape
+    I = outs["groups"].reshape(nz, ny, nx) if axis == (0, 1, 2) else outs["groups"].reshape(nx, ny, nz).transpose(2, 1, 0)
+    snap0 = workloads.twin_snapshot(("grp",) + shape, *shape, seed=9)
+    ref = diffraction_fp64(snap0.params, refl.atoms, *workloads.host_inputs(snap0))[0].transpose(2, 1, 0)
+    assert rel_to_peak(I, ref) <= FP32_PEAK_TOL and np.argmax(I) == np.argmax(ref)
