@@ -452,7 +452,7 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided(PassArgs<T> a) {
     __syncthreads();
     // inputs are consumed in the order (chunk, input); on long lines the next one is
     // prefetched into registers while the current one is transformed
-    constexpr bool PF = SZ::TPL >= 32;
+    constexpr bool PF = true;
     const int nitems = a.group_first[a.ngroups] * NIN;
     auto load_item = [&](int it, cx<T>(&dst)[1][SZ::RMAX]) {
         const int c = it / NIN, f = it - (it / NIN) * NIN;
@@ -555,7 +555,7 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided_mix(PassArgs<T> a) {
     const uint32_t colbase = i0 + outer * a.OS;
     load_twiddles<T, N>(Wl, a.W, tid, G * SZ::TPL);
     __syncthreads();
-    constexpr bool PF = SZ::TPL >= 32;
+    constexpr bool PF = true;
     const int nitems = 3 * a.nsrc;
     auto load_item = [&](int it, cx<T>(&dst)[1][SZ::RMAX]) {
         const cx<T>* in = a.in + (size_t)it * a.ncell + colbase;
@@ -651,7 +651,7 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_split(PassArgs<T> a) {
     const uint32_t rstride = a.in_step * a.S;
     load_twiddles<T, N>(Wl, a.W, tid, G * SZ::TPL);
     __syncthreads();
-    constexpr bool PF = SZ::TPL >= 32;
+    constexpr bool PF = true;
     auto load_field = [&](int f, cx<T>(&dst)[1][SZ::RMAX]) {
         const cx<T>* in = a.out + (size_t)f * a.ncell + colbase;
         if (j < NR0) {
