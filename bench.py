@@ -264,17 +264,52 @@ def serve_leg(args):
     return res
 
 
+TRAFFIC_FILE = ROOT / "profiles" / "volume_512x512x104_fp32_traffic.json"
+
+
+def measured_traffic(grid, pass_index, schedule):
+    """HBM bytes per launch of a kernel, from the committed rocprofv3 counter summary, or None.
+
+    The summary holds FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by what the same
+    counter reads for a device copy of known size; it is used only when it was taken for the
+    same grid, pass and schedule.
+    """
+    try:
+        t = json.loads(TRAFFIC_FILE.read_text())
+    except (OSError, ValueError):
+        return None
+    if list(t.get("grid", [])) != list(grid):
+        return None
+    for k in t.get("kernels", []):
+        if k.get("pass") == pass_index and k.get("schedule") == schedule:
+            return k.get("bytes")
+    return None
+
+
 def main():
+    # the library and the atom-file reader print progress on stdout, as the reference
+    # does; the bench keeps stdout for its one JSON line
+    real_stdout = sys.stdout
+    sys.stdout = sys.stderr
+    try:
+        result = run()
+    finally:
+        sys.stdout = real_stdout
+    if result is not None:
+        print(json.dumps(result), flush=True)
+
+
+def run():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(json.dumps({"error": "this build runs on one GPU: a volume, a batch or a served job runs whole on "
-                                   "one MI355X; --gpus must be 1", "n_gpus": args.gpus}))
+                                   "one MI355X; --gpus must be 1", "n_gpus": args.gpus}), file=sys.__stdout__)
         sys.exit(2)
     import numpy as np
     import torch
     if not torch.cuda.is_available():
-        print(json.dumps({"error": "no GPU"}))
+        print(json.dumps({"error": "no GPU"}), file=sys.__stdout__)
         sys.exit(1)
     from qpop_diffraction import Engine, Volume
     from qpop_diffraction import workloads
@@ -332,8 +367,9 @@ def main():
         "ms_per_volume_fp32": sec * 1e3,
         "kernels": [{"pass": p, "ms": passes[p], "bytes_per_cell": bpc[p], "GB_s": gbs[p],
                      "schedule": tuned["passes"][p]["chosen"]} for p in range(3)],
-        "roofline": {"bound": "hbm", "kernel": f"pass {dom}", "achieved": gbs[dom], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": gbs[dom] / HBM_PEAK_GBS, "traffic": None,
+        "roofline": {"bound": "hbm", "kernel": f"pass {dom}: {tuned['passes'][dom]['chosen']}",
+                     "achieved": gbs[dom], "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs[dom] / HBM_PEAK_GBS,
+                     "traffic": measured_traffic((nx, ny, nz), dom, tuned["passes"][dom]["chosen"]),
                      "bytes_per_launch": bpc[dom] * n},
     }
     if not args.no_fp64:
@@ -355,7 +391,7 @@ def main():
     if args.cpu_crop > 0:
         log("CPU baseline")
         result["cpu_baseline"] = cpu_baseline(w, args.cpu_crop)
-    print(json.dumps(result))
+    return result
 
 
 if __name__ == "__main__":
