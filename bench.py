@@ -44,6 +44,9 @@ def parse():
     ap.add_argument("--serve-rate", type=float, default=20.0, help="mean arrivals per second")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch schedules")
+    ap.add_argument("--large", action="store_true",
+                    help="also run the 4000 x 4000 x 68 polar-vortex volume (about 190 GB of HBM) with its FP64 crop")
+    ap.add_argument("--large-grid", type=int, nargs=3, default=(4000, 4000, 68), metavar=("NX", "NY", "NZ"))
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first copy BYTES device to device three times (a kernel of known traffic for counters)")
     return ap.parse_args()
@@ -286,6 +289,48 @@ def measured_traffic(grid, pass_index, schedule):
     return None
 
 
+def large_leg(args):
+    """One large volume on one GPU: FP32 time, and its error against FP64 on a 512 x 512 x nz crop around G."""
+    import torch
+    from qpop_diffraction import Engine, Volume, workloads
+    from qpop_diffraction.autotune import ScheduleCache
+    from qpop_diffraction.large import crop_of, fp64_crop
+    nx, ny, nz = args.large_grid
+    t0 = time.perf_counter()
+    v = workloads.vortex(nx, ny, nz, precision=32)
+    torch.cuda.synchronize()
+    t_gen = time.perf_counter() - t0
+    log(f"large: {v.name} generated in {t_gen:.1f} s")
+    eng = Engine(v.params, v.atoms)
+    vol = Volume(v.dims, v.axis, v.P, v.U, v.phase, precision=32)
+    out = torch.empty(vol.n, dtype=torch.float64, device="cuda")
+    tuned = eng.tune(vol, cache=ScheduleCache(None), out=out)
+    plan = eng.plan(vol.dims, vol.axis, 32)
+    sec = time_steps(lambda: eng.intensity(vol, out), 3, 1)
+    passes = pass_times(plan, vol, out, eng.workspace(plan), 2)
+    eng.intensity(vol, out)
+    I32 = crop_of(out, v.dims)
+    hbm = torch.cuda.max_memory_allocated()
+    log(f"large: {sec * 1e3:.1f} ms/volume, passes {passes}, peak HBM {hbm / 1e9:.1f} GB")
+    del out
+    eng._work.clear()
+    torch.cuda.empty_cache()
+    t0 = time.perf_counter()
+    ref = fp64_crop(v, v.params, eng.tables, log=log)
+    t_ref = time.perf_counter() - t0
+    dl, rel = max_dlog10(I32, ref)
+    bpc = [plan.pass_bytes_per_cell(p, vol.P, vol.U, vol.phase) for p in range(3)]
+    n = vol.n
+    return {"workload": f"{v.name}, layout z fastest, FP32, reflection 002",
+            "grid": [nx, ny, nz], "ms_per_volume": sec * 1e3, "voxels_per_s": n / sec,
+            "passes_ms": passes, "schedules": [p["chosen"] for p in tuned["passes"]],
+            "GB_s": [bpc[p] * n / (passes[p] * 1e-3) / 1e9 for p in range(3)],
+            "peak_hbm_bytes": hbm, "generate_s": t_gen,
+            "accuracy_vs_fp64_crop": {"crop": [512, 512, nz], "max_abs_dlog10_I": dl, "max_abs_dI_over_peak": rel,
+                                      "floor": "voxels >= 1e-10 of the crop's peak",
+                                      "fp64_crop_s": t_ref}}
+
+
 def main():
     # the library and the atom-file reader print progress on stdout, as the reference
     # does; the bench keeps stdout for its one JSON line
@@ -372,8 +417,10 @@ def run():
                      "traffic": measured_traffic((nx, ny, nz), dom, tuned["passes"][dom]["chosen"]),
                      "bytes_per_launch": bpc[dom] * n},
     }
+    del vol, out
+    engine._work.clear()
+    torch.cuda.empty_cache()
     if not args.no_fp64:
-        del vol
         w64 = workloads.two_phase(nx, ny, nz, precision=64)
         vol64 = Volume(w64.dims, w64.axis, w64.P, w64.U, w64.phase, precision=64)
         out64 = torch.empty(n, dtype=torch.float64, device="cuda")
@@ -388,6 +435,9 @@ def run():
         result["batched"] = batch_leg(args)
     if not args.no_serve:
         result["served"] = serve_leg(args)
+    if args.large:
+        torch.cuda.empty_cache()
+        result["large"] = large_leg(args)
     if args.cpu_crop > 0:
         log("CPU baseline")
         result["cpu_baseline"] = cpu_baseline(w, args.cpu_crop)

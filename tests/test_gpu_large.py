@@ -1,0 +1,54 @@
+"""The large-volume path at a size a full FP64 plan still handles.
+
+* ``large.fp64_crop`` (x slabs through passes 0 and 1, the crop's ky kept, pass 2 on the
+  reduced grid) equals the crop of a full FP64 plan of the same volume to 1e-12 of the peak;
+* the polar-vortex generator's FP32 volume agrees with that FP64 crop to the FP32 bound
+  (1e-5 of the peak, same peak voxel);
+* a long strided axis runs split (the schedule of pass 2 starts with "split").
+"""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def test_fp64_crop_matches_full_plan():
+    import torch
+    from qpop_diffraction import Engine, Volume, workloads
+    from qpop_diffraction.large import crop_of, fp64_crop
+    v = workloads.vortex(240, 200, 20, substrate=4, slab=64, precision=64)
+    print("vortex 240x200x20 fp64 crop vs full", flush=True)
+    eng = Engine(v.params, v.atoms)
+    vol = Volume(v.dims, v.axis, v.P, v.U, v.phase, precision=64)
+    full = eng.intensity(vol)
+    torch.cuda.synchronize()
+    ref = crop_of(full, v.dims, crop=(64, 80))
+    got = fp64_crop(v, v.params, eng.tables, crop=(64, 80), slab=50)
+    peak = ref.max().item()
+    err = (got - ref).abs().max().item() / peak
+    print(f"  max|dI|/peak {err:.2e}")
+    assert err < 1e-12
+    assert int(got.argmax()) == int(ref.argmax())
+
+
+def test_vortex_fp32_against_fp64_crop():
+    import torch
+    from qpop_diffraction import Engine, Volume, workloads
+    from qpop_diffraction.large import crop_of, fp64_crop
+    v = workloads.vortex(1000, 640, 68, precision=32)
+    print("vortex 1000x640x68 fp32 vs fp64 crop", flush=True)
+    eng = Engine(v.params, v.atoms)
+    vol = Volume(v.dims, v.axis, v.P, v.U, v.phase, precision=32)
+    plan = eng.plan(vol.dims, vol.axis, 32)
+    names = plan.schedules(2)
+    assert names[plan.schedule(2)].startswith("split"), names
+    I32 = crop_of(eng.intensity(vol), v.dims, crop=(128, 128))
+    ref = fp64_crop(v, v.params, eng.tables, crop=(128, 128), slab=250)
+    peak = ref.max()
+    rel = ((I32 - ref).abs().max() / peak).item()
+    mask = ref >= 1e-10 * peak
+    dl = (torch.log10(I32[mask]) - torch.log10(ref[mask])).abs().max().item()
+    print(f"  max|dI|/peak {rel:.2e}, max|dlog10 I| {dl:.2e}")
+    assert rel <= 1e-5
+    assert int(I32.argmax()) == int(ref.argmax())
