@@ -607,14 +607,19 @@ int pass_candidates(const AxisFFT& ax, int d, const int64_t dims[3], int nin, si
                 pp.L[0].blocks = contig ? (lines + e.G - 1) / e.G : (dims[0] + e.G - 1) / e.G * n_other;
                 if (pp.L[0].blocks <= 0x7fffffffLL) out.push_back(pp);
             }
-            if (!contig && allow_split && (role == kRoleStrided || role == kRoleIntensity)) {
+            const bool splittable =
+                role == kRoleStrided || role == kRoleIntensity || role == kRoleMix1 || role == kRoleMix2;
+            if (!contig && allow_split && splittable) {
                 std::vector<std::pair<double, PassPlan>> splits;
                 for (int64_t n1 = 2; n1 <= N / 2; ++n1) {
                     if (N % n1) continue;
                     const int64_t n2 = N / n1;
                     for (const FastEntry& ea : fast_registry()) {
                         if (ea.N != n2 || ea.role != kRoleSplit || (int)ea.lds > hard) continue;
-                        const int role_b = role == kRoleStrided ? kRoleStridedB : kRoleIntensityB;
+                        const int role_b = role == kRoleStrided     ? kRoleStridedB
+                                           : role == kRoleIntensity ? kRoleIntensityB
+                                           : role == kRoleMix1      ? kRoleMix1B
+                                                                    : kRoleMix2B;
                         for (const FastEntry& eb : fast_registry()) {
                             if (eb.N != n1 || eb.role != role_b || (int)eb.lds > hard) continue;
                             PassPlan pp;
@@ -849,7 +854,8 @@ int run_fft3d(const int64_t dims[3], void* data, int n_fields, hipStream_t st) {
 std::string describe_launch(const Launch& L) {
     char buf[160];
     static const char* role_name[kNumRoles] = {"source", "contiguous", "strided", "intensity", "strided1",
-                                               "split-step", "strided", "intensity", "groups", "groups"};
+                                               "split-step", "strided", "intensity", "groups", "groups",
+                                               "groups", "groups"};
     if (L.fe)
         std::snprintf(buf, sizeof(buf), "fast %d (%s) G%d %s", L.fe->N, L.fe->radices, L.fe->G,
                       role_name[L.fe->role]);
@@ -959,14 +965,20 @@ int qpd_plan_create(qpd_plan** plan, const qpd_grid* grid, const qpd_crystal* c,
     // or two z-classes; QPD_CLASS_FORM=1 keeps the class form for comparison runs)
     const int mix_role = p->nZ == 1 ? kRoleMix1 : (p->nZ == 2 ? kRoleMix2 : -1);
     const char* cf = std::getenv("QPD_CLASS_FORM");
-    p->grouped = (precision == 32 && fast_enabled() && mix_role >= 0 && p->nS < p->nY &&
-                  fast_lookup(p->dims[1], mix_role) != nullptr && !(cf && cf[0] == '1'))
+    p->grouped = (precision == 32 && fast_enabled() && mix_role >= 0 && p->nS < p->nY && !(cf && cf[0] == '1'))
                      ? 1
                      : 0;
     int rc = QPD_OK;
     for (int d = 0; d < 3 && rc == QPD_OK; ++d) {
         rc = pass_candidates(p->fft[d], d, p->dims, nin[d], p->csize, true, c->n_atom * c->n_phase * kAtomTab, true,
                              p->cand[d], p->grouped ? mix_role : -1);
+        if (d == 1 && p->grouped && rc != QPD_OK) {
+            // no source-group kernel (direct or split) for this length: keep the class form
+            g_error.clear();
+            p->grouped = 0;
+            rc = pass_candidates(p->fft[d], d, p->dims, nin[d], p->csize, true, c->n_atom * c->n_phase * kAtomTab,
+                                 true, p->cand[d], -1);
+        }
         if (rc == QPD_OK) {
             // default: a kernel of the length itself, else the most even split, else generic
             p->sel[d] = 0;
@@ -999,7 +1011,7 @@ size_t qpd_workspace_bytes(const qpd_plan* p) {
 }
 
 int qpd_fast_lengths(int role, int64_t* lengths, int cap) {
-    if (role < 0 || role >= kNumRoles) return fail(QPD_ERR_ARG, "role must be 0..9");
+    if (role < 0 || role >= kNumRoles) return fail(QPD_ERR_ARG, "role must be 0..11");
     std::vector<int64_t> ls;
     for (const FastEntry& e : fast_registry())
         if (e.role == role && std::find(ls.begin(), ls.end(), (int64_t)e.N) == ls.end()) ls.push_back(e.N);

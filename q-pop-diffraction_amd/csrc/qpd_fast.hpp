@@ -536,7 +536,7 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided(PassArgs<T> a) {
 //   out_{Z,0} = sum_S sum_{Y in Z} c_Y(k1) X_{Y,S}(k0) [F1 A_S - i k1 F1 B_S],
 //   out_{Z,1} = sum_S sum_{Y in Z} c_Y(k1) X_{Y,S}(k0) F1 C_S,
 // the same outputs as the class form.  Each group's three inputs are transformed once.
-template <typename T, class SZ, int G, int NZ>
+template <typename T, class SZ, int G, int NZ, bool SPLIT>
 __global__ __launch_bounds__(G* SZ::TPL) void fast_strided_mix(PassArgs<T> a) {
     constexpr int N = SZ::N, R0 = SZ::rad(0), NR0 = SZ::nr(0), LP = G + 1, RL = SZ::RL, NRL = SZ::NRL;
     constexpr int TILE = N * LP;
@@ -549,7 +549,13 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided_mix(PassArgs<T> a) {
     const int g = tid % G, j = tid / G;
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t t0 = blk % a.ntiles0;
-    const uint32_t outer = blk / a.ntiles0;
+    const uint32_t outer_q = blk / a.ntiles0;
+    // unsplit passes: q = 0 and unit steps, known at compile time
+    const uint32_t outer = SPLIT ? fdiv(outer_q, a.divQ) : outer_q;
+    const uint32_t q = SPLIT ? outer_q - outer * a.nq : 0u;
+    const uint32_t in_step = SPLIT ? a.in_step : 1u, out_step = SPLIT ? a.out_step : 1u;
+    const uint32_t c_step = SPLIT ? a.c_step : 1u;
+    const uint32_t in0 = q * a.in_qmul, out0 = q * a.out_qmul, c0 = q * a.c_qmul;
     const uint32_t i0 = t0 * G + g;
     const bool col_ok = i0 < a.n0;
     const uint32_t colbase = i0 + outer * a.OS;
@@ -561,7 +567,8 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided_mix(PassArgs<T> a) {
         const cx<T>* in = a.in + (size_t)it * a.ncell + colbase;
         if (j < NR0) {
 #pragma unroll
-            for (int r = 0; r < R0; ++r) dst[0][r] = col_ok ? in[(uint32_t)(j + r * NR0) * a.S] : cx<T>{(T)0, (T)0};
+            for (int r = 0; r < R0; ++r)
+                dst[0][r] = col_ok ? in[(in0 + (uint32_t)(j + r * NR0) * in_step) * a.S] : cx<T>{(T)0, (T)0};
         }
     };
     cx<T> acc[NZ][2][RL];
@@ -591,7 +598,7 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided_mix(PassArgs<T> a) {
 #pragma unroll
                 for (int r = 0; r < RL; ++r) {
                     if (f == 0) tA[r] = v[0][r];
-                    if (f == 1) tA[r] = tA[r] + cscale(negi(v[0][r]), a.k[j + r * NRL]);
+                    if (f == 1) tA[r] = tA[r] + cscale(negi(v[0][r]), a.k[c0 + (uint32_t)(j + r * NRL) * c_step]);
                     if (f == 2) tC[r] = v[0][r];
                 }
             }
@@ -604,7 +611,7 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided_mix(PassArgs<T> a) {
                     const cx<T> xc = a.mixc[(size_t)(y * a.nsrc + s) * a.n0 + i0];
 #pragma unroll
                     for (int r = 0; r < RL; ++r) {
-                        const cx<T> cf = cmul(a.coef[y * a.Nc + j + r * NRL], xc);
+                        const cx<T> cf = cmul(a.coef[y * a.Nc + c0 + (uint32_t)(j + r * NRL) * c_step], xc);
                         acc[z][0][r] = acc[z][0][r] + cmul(cf, tA[r]);
                         acc[z][1][r] = acc[z][1][r] + cmul(cf, tC[r]);
                     }
@@ -616,10 +623,10 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided_mix(PassArgs<T> a) {
 #pragma unroll
     for (int z = 0; z < NZ; ++z)
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            cx<T>* o = a.out + (size_t)(2 * z + q) * a.ncell + colbase;
+        for (int oq = 0; oq < 2; ++oq) {
+            cx<T>* o = a.out + (size_t)(2 * z + oq) * a.ncell + colbase;
 #pragma unroll
-            for (int r = 0; r < RL; ++r) o[(uint32_t)(j + r * NRL) * a.S] = acc[z][q][r];
+            for (int r = 0; r < RL; ++r) o[(out0 + (uint32_t)(j + r * NRL) * out_step) * a.S] = acc[z][oq][r];
         }
 }
 

@@ -36,9 +36,13 @@ hipError_t fast_launch(const Launch& L, const PassArgs<float>& a, hipStream_t st
     else if constexpr (ROLE == kRoleIntensityB)
         return launch_one(fast_strided<float, SZ, G, 2, true, true>, L, st, a);
     else if constexpr (ROLE == kRoleMix1)
-        return launch_one(fast_strided_mix<float, SZ, G, 1>, L, st, a);
+        return launch_one(fast_strided_mix<float, SZ, G, 1, false>, L, st, a);
+    else if constexpr (ROLE == kRoleMix2)
+        return launch_one(fast_strided_mix<float, SZ, G, 2, false>, L, st, a);
+    else if constexpr (ROLE == kRoleMix1B)
+        return launch_one(fast_strided_mix<float, SZ, G, 1, true>, L, st, a);
     else
-        return launch_one(fast_strided_mix<float, SZ, G, 2>, L, st, a);
+        return launch_one(fast_strided_mix<float, SZ, G, 2, true>, L, st, a);
 }
 
 template <class SZ, int ROLE, int G>
@@ -72,7 +76,8 @@ void add_fast_variants(std::vector<FastEntry>& v, const char* radices) {
 constexpr unsigned kRolesContig = (1u << kRoleSource) | (1u << kRoleContig);
 constexpr unsigned kRolesStrided = (1u << kRoleStrided) | (1u << kRoleIntensity) | (1u << kRoleStrided1) |
                                    (1u << kRoleMix1) | (1u << kRoleMix2);
-constexpr unsigned kRolesSplitSteps = (1u << kRoleSplit) | (1u << kRoleStridedB) | (1u << kRoleIntensityB);
+constexpr unsigned kRolesSplitSteps = (1u << kRoleSplit) | (1u << kRoleStridedB) | (1u << kRoleIntensityB) |
+                                      (1u << kRoleMix1B) | (1u << kRoleMix2B);
 
 template <class SZ, unsigned ROLES>
 void add_fast_roles(std::vector<FastEntry>& v, const char* radices) {
@@ -89,6 +94,8 @@ void add_fast_roles(std::vector<FastEntry>& v, const char* radices) {
         add_fast<SZ, kRoleIntensityB, fast_tile(kRoleIntensityB, SZ::TPL)>(v, radices);
     if constexpr ((ROLES >> kRoleMix1) & 1u) add_fast_variants<SZ, kRoleMix1>(v, radices);
     if constexpr ((ROLES >> kRoleMix2) & 1u) add_fast_variants<SZ, kRoleMix2>(v, radices);
+    if constexpr ((ROLES >> kRoleMix1B) & 1u) add_fast<SZ, kRoleMix1B, fast_tile(kRoleMix1B, SZ::TPL)>(v, radices);
+    if constexpr ((ROLES >> kRoleMix2B) & 1u) add_fast<SZ, kRoleMix2B, fast_tile(kRoleMix2B, SZ::TPL)>(v, radices);
 }
 
 }  // namespace qpd

@@ -129,6 +129,14 @@ void QPD_CAT(register_fast_part, QPD_FAST_PART)(std::vector<FastEntry>& v) {
     add_if_mine<98, Sizes<480, 8, 5, 4, 3>, kRolesStrided>(v, "8x5x4x3");
     add_if_mine<99, Sizes<500, 5, 5, 5, 4>, kRolesStrided>(v, "5x5x5x4");
     add_if_mine<100, Sizes<512, 8, 8, 8>, kRolesStrided>(v, "8x8x8");
+    // large radix first: fewer accumulators in the source pass, for the autotuner to weigh
+    add_if_mine<101, Sizes<68, 17, 4>, kRolesContig>(v, "17x4");
+    add_if_mine<102, Sizes<34, 17, 2>, kRolesContig>(v, "17x2");
+    add_if_mine<103, Sizes<136, 17, 8>, kRolesContig>(v, "17x8");
+    add_if_mine<104, Sizes<52, 13, 4>, kRolesContig>(v, "13x4");
+    add_if_mine<105, Sizes<104, 13, 8>, kRolesContig>(v, "13x8");
+    add_if_mine<106, Sizes<44, 11, 4>, kRolesContig>(v, "11x4");
+    add_if_mine<107, Sizes<88, 11, 8>, kRolesContig>(v, "11x8");
 }
 
 }  // namespace qpd

@@ -28,7 +28,9 @@ enum FastRole {
     kRoleIntensityB = 7, // kRoleIntensity as the second step of a split length
     kRoleMix1 = 8,       // strided pass 1 over source groups, one z-class
     kRoleMix2 = 9,       // strided pass 1 over source groups, two z-classes
-    kNumRoles = 10,
+    kRoleMix1B = 10,     // kRoleMix1 as the second step of a split length
+    kRoleMix2B = 11,     // kRoleMix2 as the second step of a split length
+    kNumRoles = 12,
 };
 
 struct FastEntry;

@@ -127,7 +127,8 @@ def test_every_schedule(refl, shape):
             assert err <= FP32_PEAK_TOL, (d, name, err)
 
 
-@pytest.mark.parametrize("shape,axis", [((48, 40, 24), (0, 1, 2)), ((24, 40, 48), (2, 1, 0)), ((104, 64, 30), (0, 1, 2))])
+@pytest.mark.parametrize("shape,axis", [((48, 40, 24), (0, 1, 2)), ((24, 40, 48), (2, 1, 0)), ((104, 64, 30), (0, 1, 2)),
+                                        ((24, 1000, 12), (0, 1, 2))])
 def test_source_groups_match_class_form(refl, shape, axis, monkeypatch):
     """PbTiO3 has two source groups (Pb; Ti and the three O share su) and four y-classes, so
     FP32 plans transform the groups in pass 0 and mix them in pass 1; QPD_CLASS_FORM=1 keeps
