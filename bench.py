@@ -289,6 +289,54 @@ def measured_traffic(grid, pass_index, schedule):
     return None
 
 
+def ingest_leg(nx, ny, nz, reps=10):
+    """Field ingest on the GPU (raw float32 P, float64 u in m, int32 materials over a deeper stack
+    with a 16-cell substrate below the crop) against the same preparation in NumPy on the host."""
+    import numpy as np
+    import torch
+    from qpop_diffraction.constants import L0
+    from qpop_diffraction.ingest import ingest
+    nz_full, z0 = nz + 16, 16
+    gen = torch.Generator(device="cuda").manual_seed(11)
+    P = [0.5 * torch.randn((nz, ny, nx), generator=gen, device="cuda") for _ in range(3)]
+    U = [1e-11 * torch.randn((nz, ny, nx), generator=gen, device="cuda", dtype=torch.float64) for _ in range(3)]
+    zz = torch.arange(nz_full, device="cuda").view(-1, 1, 1)
+    mat = torch.where(zz < z0, 0, 1 + (torch.rand((nz_full, ny, nx), generator=gen, device="cuda") < 0.5).int())
+    mat = mat.to(torch.int32).contiguous()
+    dc = np.array([0.0, 0.012, -0.004])
+    phase_of = np.array([-1, 0, 1], dtype=np.int8)
+    dz = 0.39
+    ingest(P, U, mat, dc, phase_of, z0, dz, 32)
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(reps):
+        Pd, Ud, phd = ingest(P, U, mat, dc, phase_of, z0, dz, 32)
+    e1.record(st)
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    n = nx * ny * nz
+    nbytes = n * (3 * 4 + 3 * 8 + 3 * 4 + 3 * 8 + 1) + nx * ny * nz_full * 4
+    # the same preparation in NumPy (cli.py:189-217), host arrays already in memory
+    Ph = [t.cpu().numpy() for t in P]
+    Uh = [t.cpu().numpy() for t in U]
+    mh = mat.cpu().numpy()
+    t0 = time.perf_counter()
+    Pn = np.stack(Ph).astype(np.float32)
+    Un = np.stack(Uh) / L0
+    d = dc[mh]
+    Un[2] += ((np.cumsum(d, axis=0) - 0.5 * d) * dz)[z0:]
+    phn = phase_of[mh[z0:]]
+    host_s = time.perf_counter() - t0
+    same = (np.array_equal(Pd.cpu().numpy().reshape(3, -1), Pn.reshape(3, -1)) and
+            np.array_equal(Ud.cpu().numpy().reshape(3, -1), Un.reshape(3, -1)) and
+            np.array_equal(phd.cpu().numpy(), phn.reshape(-1)))
+    return {"workload": f"{nx}x{ny}x{nz} crop of a {nz_full}-deep stack: float32 P, float64 u (m), int32 materials",
+            "ms": ms, "GB_s": nbytes / (ms * 1e-3) / 1e9, "bytes_per_launch": nbytes,
+            "frac": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "host_numpy_ms": host_s * 1e3, "bit_identical_to_host": bool(same)}
+
+
 def large_leg(args):
     """One large volume on one GPU: FP32 time, and its error against FP64 on a 512 x 512 x nz crop around G."""
     import torch
@@ -431,6 +479,10 @@ def run():
         result["accuracy_fp32_vs_fp64"] = {"max_abs_dlog10_I": dl, "max_abs_dI_over_peak": relpk,
                                            "floor": "voxels >= 1e-10 of the peak"}
         del vol64, out64, w64
+    log("field ingest")
+    result["ingest"] = ingest_leg(nx, ny, nz)
+    log(f"ingest: {result['ingest']['ms']:.3f} ms ({result['ingest']['GB_s']:.0f} GB/s) vs NumPy "
+        f"{result['ingest']['host_numpy_ms']:.0f} ms, identical {result['ingest']['bit_identical_to_host']}")
     if not args.no_batch:
         result["batched"] = batch_leg(args)
     if not args.no_serve:

@@ -82,6 +82,38 @@ typedef struct qpd_fields {
     const void* region;            /* (n) real scattering weight, or NULL = 1 */
 } qpd_fields;
 
+/* Field ingest: the raw, cropped FerroDyn fields of a run as read from the files (file
+ * order, x fastest, each component its own array, in the files' own precision) and the
+ * material ids of the crop's columns over the full stack.  qpd_ingest_fields turns them
+ * into the plan inputs of the x-fastest layout (dims {nx, ny, nz}, axes {0, 1, 2}):
+ *   P_out (3, n) = P / P_div in the plan precision,
+ *   U_out (3, n) = u / U_div in double, plus on z the reference ramp
+ *                  (cumsum_z dc[mat] - dc[mat] / 2) dz summed over the full stack from z = 0,
+ *   phase_out (n) = phase_of[mat],
+ * with the operations in the order of the reference CLI (cli.py:189-217; the native
+ * `columns` kernel, kernels.cuh:34-50), so the results equal the host preparation bit for
+ * bit.  Material ids outside [0, n_material) give dc = 0 and phase -1.  All pointers are
+ * device memory; dc and phase_of hold n_material entries. */
+typedef struct qpd_ingest {
+    int64_t dims[3];               /* crop extents nx, ny, nz */
+    int64_t nz_full;               /* z extent of the material field */
+    int64_t z0;                    /* first z of the crop in the full stack */
+    const void* P[3];              /* cropped P_x, P_y, P_z */
+    int P_bits;                    /* 32 or 64: precision of the P arrays */
+    double P_div;                  /* order-parameter unit (OS0) */
+    const void* U[3];              /* cropped u_x, u_y, u_z in m, or NULL: no displacement file */
+    int U_bits;                    /* 32 or 64 */
+    double U_div;                  /* metres per nm (L0) */
+    const int32_t* material;       /* (nz_full, ny, nx) material id of each cell of the crop's columns */
+    int n_material;
+    const double* dc;              /* c_m(T) / aC_z - 1 per material (size_context reference strain) */
+    const signed char* phase_of;   /* phase index per material, -1 = does not scatter */
+    double dz;                     /* nm */
+} qpd_ingest;
+
+int qpd_ingest_fields(const qpd_ingest* in, int precision, void* P_out, double* U_out, signed char* phase_out,
+                      void* stream);
+
 /* Create a plan for one grid, crystal and precision (32: complex FP32 bulk with FP64
  * phase reduction and FP64-generated twiddles; 64: complex FP64 throughout).
  * Replaces the per-call setup of diffraction.py:75-94 and the HeFFTe plan of

@@ -20,6 +20,7 @@
 #include "qpd.h"
 #include "qpd_kernels.hpp"
 #include "qpd_launch.hpp"
+#include "qpd_ingest.hpp"
 
 using namespace qpd;
 
@@ -909,6 +910,25 @@ const std::vector<FastEntry>& fast_registry() {
 }  // namespace qpd
 
 // ---------------------------------------------------------------------------
+namespace {
+template <typename TP, typename TU, typename TO>
+hipError_t launch_ingest(const IngestArgs& a, void* P_out, double* U_out, signed char* ph, hipStream_t st) {
+    const unsigned blocks = (a.nc + 255u) / 256u;
+    hipLaunchKernelGGL((ingest_columns<TP, TU, TO>), dim3(blocks), dim3(256), 0, st, a, static_cast<TO*>(P_out), U_out, ph);
+    return hipGetLastError();
+}
+
+template <typename TO>
+hipError_t ingest_dispatch(const IngestArgs& a, int pb, int ub, void* P_out, double* U_out, signed char* ph,
+                           hipStream_t st) {
+    if (pb == 32)
+        return ub == 64 ? launch_ingest<float, double, TO>(a, P_out, U_out, ph, st)
+                        : launch_ingest<float, float, TO>(a, P_out, U_out, ph, st);
+    return ub == 64 ? launch_ingest<double, double, TO>(a, P_out, U_out, ph, st)
+                    : launch_ingest<double, float, TO>(a, P_out, U_out, ph, st);
+}
+}  // namespace
+
 extern "C" {
 
 const char* qpd_last_error(void) { return g_error.c_str(); }
@@ -1008,6 +1028,51 @@ int qpd_plan_destroy(qpd_plan* plan) {
 size_t qpd_workspace_bytes(const qpd_plan* p) {
     if (!p) return 0;
     return (size_t)(p->n1fields() + 2 * p->nZ) * (size_t)p->ncell * p->csize;
+}
+
+
+int qpd_ingest_fields(const qpd_ingest* in, int precision, void* P_out, double* U_out, signed char* phase_out,
+                      void* stream) {
+    g_error.clear();
+    if (!in || !P_out || !U_out || !phase_out) return fail(QPD_ERR_ARG, "null argument");
+    if (precision != 32 && precision != 64) return fail(QPD_ERR_ARG, "precision must be 32 or 64");
+    if (in->P_bits != 32 && in->P_bits != 64) return fail(QPD_ERR_ARG, "P_bits must be 32 or 64");
+    const bool has_u = in->U[0] != nullptr;
+    if (has_u && (in->U_bits != 32 && in->U_bits != 64)) return fail(QPD_ERR_ARG, "U_bits must be 32 or 64");
+    if (has_u && (!in->U[1] || !in->U[2])) return fail(QPD_ERR_ARG, "U needs all three components or none");
+    if (!in->P[0] || !in->P[1] || !in->P[2] || !in->material || !in->dc || !in->phase_of)
+        return fail(QPD_ERR_ARG, "null field pointer");
+    for (int d = 0; d < 3; ++d)
+        if (in->dims[d] < 1) return fail(QPD_ERR_ARG, "dims must be >= 1");
+    if (in->z0 < 0 || in->z0 + in->dims[2] > in->nz_full) return fail(QPD_ERR_ARG, "the crop must lie inside the stack");
+    if (in->n_material < 1) return fail(QPD_ERR_ARG, "n_material must be >= 1");
+    if (!(in->P_div != 0.0) || (has_u && !(in->U_div != 0.0))) return fail(QPD_ERR_ARG, "zero unit divisor");
+    const int64_t nc = in->dims[0] * in->dims[1], n = nc * in->dims[2];
+    if (n >= (int64_t)1 << 31) return fail(QPD_ERR_UNSUPPORTED, "grids of 2^31 cells or more are not supported");
+    IngestArgs a;
+    a.nc = (uint32_t)nc;
+    a.n = (uint32_t)n;
+    a.nz_full = in->nz_full;
+    a.z0 = in->z0;
+    a.nz = in->dims[2];
+    a.P0 = in->P[0];
+    a.P1 = in->P[1];
+    a.P2 = in->P[2];
+    a.U0 = has_u ? in->U[0] : nullptr;
+    a.U1 = has_u ? in->U[1] : nullptr;
+    a.U2 = has_u ? in->U[2] : nullptr;
+    a.P_div = in->P_div;
+    a.U_div = has_u ? in->U_div : 1.0;
+    a.dz = in->dz;
+    a.mat = in->material;
+    a.n_material = in->n_material;
+    a.dc = in->dc;
+    a.phase_of = in->phase_of;
+    const int ub = has_u ? in->U_bits : 64;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    QPD_HIP(precision == 32 ? ingest_dispatch<float>(a, in->P_bits, ub, P_out, U_out, phase_out, st)
+                            : ingest_dispatch<double>(a, in->P_bits, ub, P_out, U_out, phase_out, st));
+    return QPD_OK;
 }
 
 int qpd_fast_lengths(int role, int64_t* lengths, int cap) {

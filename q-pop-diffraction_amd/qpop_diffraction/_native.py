@@ -21,7 +21,7 @@ EXPORTS = (
     "qpd_plan_create", "qpd_plan_destroy", "qpd_workspace_bytes", "qpd_launches_per_volume",
     "qpd_plan_schedules", "qpd_plan_schedule_name", "qpd_plan_set_schedule", "qpd_plan_get_schedule",
     "qpd_fast_lengths", "qpd_intensity", "qpd_intensity_pass", "qpd_pass_bytes_per_cell", "qpd_fft3d",
-    "qpd_fft_length_supported", "qpd_last_error", "qpd_version",
+    "qpd_fft_length_supported", "qpd_ingest_fields", "qpd_last_error", "qpd_version",
 )
 
 ROLES = {"source": 0, "contiguous": 1, "strided": 2, "intensity": 3, "strided1": 4, "split": 5,
@@ -47,6 +47,14 @@ class Crystal(ctypes.Structure):
 class Fields(ctypes.Structure):
     _fields_ = [("P", ctypes.c_void_p), ("U", ctypes.c_void_p), ("phase", ctypes.c_void_p),
                 ("fraction", ctypes.c_void_p), ("region", ctypes.c_void_p)]
+
+
+class Ingest(ctypes.Structure):
+    _fields_ = [("dims", ctypes.c_int64 * 3), ("nz_full", ctypes.c_int64), ("z0", ctypes.c_int64),
+                ("P", ctypes.c_void_p * 3), ("P_bits", ctypes.c_int), ("P_div", ctypes.c_double),
+                ("U", ctypes.c_void_p * 3), ("U_bits", ctypes.c_int), ("U_div", ctypes.c_double),
+                ("material", ctypes.c_void_p), ("n_material", ctypes.c_int), ("dc", ctypes.c_void_p),
+                ("phase_of", ctypes.c_void_p), ("dz", ctypes.c_double)]
 
 
 class QPDError(RuntimeError):
@@ -92,6 +100,8 @@ def load_library():
     lib.qpd_pass_bytes_per_cell.restype = ctypes.c_double
     lib.qpd_fft3d.argtypes = [ctypes.POINTER(i64), i32, vp, i32, vp]
     lib.qpd_fft3d.restype = i32
+    lib.qpd_ingest_fields.argtypes = [ctypes.POINTER(Ingest), i32, vp, vp, vp, vp]
+    lib.qpd_ingest_fields.restype = i32
     lib.qpd_fft_length_supported.argtypes = [i64]
     lib.qpd_fft_length_supported.restype = i32
     lib.qpd_last_error.argtypes = []

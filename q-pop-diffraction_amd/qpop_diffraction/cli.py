@@ -42,14 +42,21 @@ def _parse_args(argv=None) -> argparse.Namespace:
     parser.add_argument("--precision", type=int, choices=(32, 64), default=None,
                         help="Working precision: 32 (complex FP32, FP64 phase reduction) or 64. "
                              "Default: QPOP_DIFFRACTION_PRECISION or 32.")
+    parser.add_argument("--host-ingest", action="store_true",
+                        help="Prepare P, u, the phases and the u_z ramp with NumPy on the host instead of on the GPU "
+                             "(the results are bit-identical).")
     return parser.parse_args(argv)
 
 
-def prepare_run(args):
+def prepare_run(args, device=None):
     """Everything before the GPU call: config, crystal, fields in file order (cli.py:149-217).
 
-    Returns (params, atoms, fields dict, step, output base name) or raises.
+    device: prepare P, u, the phase index and the u_z ramp on the GPU (``ingest``) from the
+    raw cropped components; otherwise with NumPy on the host.  Default: the GPU unless
+    args.host_ingest.  Returns (params, atoms, fields dict, step, output base name) or None.
     """
+    if device is None:
+        device = not getattr(args, "host_ingest", False)
     cfg = load_input_json(args.input)
     lims = (cfg["x_limits"], cfg["y_limits"], cfg["z_limits"])
 
@@ -63,6 +70,8 @@ def prepare_run(args):
     if not atoms.read(args.atom, params.nPhase, params.nStruc) or not atoms.setup():
         return None
 
+    if device:
+        return _prepare_on_device(args, cfg, params, fe_ids, atoms, full, dz, sl, lims)
     real = np.float32 if args.precision == 32 else np.float64
     print(f"Loading P_Lattice file: {cfg['platt_file']}")
     P = read_vector(cfg["platt_file"], ("P_Lattice", "PLatt"), *full, *lims, dtype=real)
@@ -103,6 +112,45 @@ def prepare_run(args):
     uz_ref = ((np.cumsum(dc, axis=0) - 0.5 * dc) * dz)[zyx]
     U[2] += uz_ref
 
+    print("\nApplying domain slice:")
+    params.apply_slice(*lims)
+    params.setup()
+    fields = {"P": P, "U": U, "phase": phase, "fraction": fraction, "region": region}
+    return params, atoms, fields, step, cfg["output_file"]
+
+
+def _prepare_on_device(args, cfg, params, fe_ids, atoms, full, dz, sl, lims):
+    """prepare_run with the fields built on the GPU from the raw cropped components."""
+    from .ingest import ingest, read_components, read_material_columns
+    real = np.float32 if args.precision == 32 else np.float64
+    print(f"Loading P_Lattice file: {cfg['platt_file']}")
+    P_raw = read_components(cfg["platt_file"], ("P_Lattice", "PLatt"), *full, sl)
+    with open_file(cfg["platt_file"], "r") as fh:
+        step = int(fh.attrs["Timestep"])
+    U_raw = None
+    if cfg["displacement_file"] is not None:
+        print(f"Loading Displacement file: {cfg['displacement_file']}")
+        U_raw = read_components(cfg["displacement_file"], ("Displacement",), *full, sl)
+    else:
+        print("No displacement_file provided — using zero displacement field.")
+    mat_file = Path(args.system).parent / "data" / "materialField_0.h5"
+    print(f"Loading material field: {mat_file}")
+    mat = read_material_columns(mat_file, *full, sl)
+    phase_of = np.full(len(params.names), -1, dtype=np.int8)
+    phase_of[fe_ids] = np.arange(len(fe_ids), dtype=np.int8)
+    fraction = None
+    if cfg["phase_file"] is not None:
+        print(f"Loading phase fractions: {cfg['phase_file']}")
+        with open_file(cfg["phase_file"], "r") as fh:
+            fraction = np.stack([read_field(fh, f"phase_{p + 1}", *full, sl, real) for p in range(params.nPhase)])
+    region = None
+    if cfg["region_file"] is not None:
+        print(f"Loading scattering weight: {cfg['region_file']}")
+        with open_file(cfg["region_file"], "r") as fh:
+            region = read_field(fh, "region", *full, sl, real)
+    print(f"\nTimestep {step}:")
+    dc = params.reference_strain(np.diag(atoms.aC), params.temperature(step))
+    P, U, phase = ingest(P_raw, U_raw, mat, dc, phase_of, sl[2].start, dz, args.precision)
     print("\nApplying domain slice:")
     params.apply_slice(*lims)
     params.setup()

@@ -56,3 +56,24 @@ def test_plan_rejects_bad_arguments():
     cr.n_struc = 2
     rc = lib.qpd_plan_create(ctypes.byref(handle), ctypes.byref(grid), ctypes.byref(cr), 32)
     assert rc == -2
+
+
+@pytest.mark.skipif(not _native.LIB_PATH.exists(), reason="libqpd.so not built")
+def test_ingest_rejects_bad_arguments():
+    """qpd_ingest_fields checks its description on the host before any device call."""
+    lib = _native.load_library()
+    fake = ctypes.c_void_p(0x1000)
+    arg = _native.Ingest((ctypes.c_int64 * 3)(4, 4, 4), 6, 1, (ctypes.c_void_p * 3)(0x1000, 0x1000, 0x1000), 32, 1.0,
+                         (ctypes.c_void_p * 3)(None, None, None), 64, 1e-9, 0x1000, 2, 0x1000, 0x1000, 0.4)
+    assert lib.qpd_ingest_fields(None, 32, fake, fake, fake, None) == -1
+    assert lib.qpd_ingest_fields(ctypes.byref(arg), 16, fake, fake, fake, None) == -1
+    arg.z0 = 3  # crop z 3..6 outside a stack of 6
+    assert lib.qpd_ingest_fields(ctypes.byref(arg), 32, fake, fake, fake, None) == -1
+    assert b"stack" in lib.qpd_last_error()
+    arg.z0 = 1
+    arg.P_bits = 16
+    assert lib.qpd_ingest_fields(ctypes.byref(arg), 32, fake, fake, fake, None) == -1
+    arg.P_bits = 32
+    arg.U = (ctypes.c_void_p * 3)(0x1000, None, None)
+    assert lib.qpd_ingest_fields(ctypes.byref(arg), 32, fake, fake, fake, None) == -1
+    assert b"three components" in lib.qpd_last_error()
