@@ -337,6 +337,57 @@ def ingest_leg(nx, ny, nz, reps=10):
             "host_numpy_ms": host_s * 1e3, "bit_identical_to_host": bool(same)}
 
 
+def output_leg(I, dims):
+    """Writing the float64 intensity of the volume to diffraction.h5: copy-then-write against the
+    streamed writer (slabs copied to pinned memory on their own stream while the previous one
+    is written)."""
+    import tempfile
+    import numpy as np
+    import torch
+    from qpop_diffraction.io_utils import write_diffraction_output, write_diffraction_streamed
+    nx, ny, nz = dims
+    q = [np.linspace(-1.0, 1.0, m) for m in (nx, ny, nz)]
+    with tempfile.TemporaryDirectory(prefix="qpd_out_") as d:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        host = I.cpu().numpy().reshape(nz, ny, nx)
+        write_diffraction_output(os.path.join(d, "plain"), host, *q, 0, zyx=True)
+        plain = time.perf_counter() - t0
+        del host
+        write_diffraction_streamed(os.path.join(d, "warm"), I, *q, 0)  # staging buffers, file system
+        streamed = write_diffraction_streamed(os.path.join(d, "streamed"), I, *q, 0)
+    nbytes = I.numel() * 8
+    return {"bytes": nbytes, "copy_then_write_s": plain, "streamed_s": streamed,
+            "streamed_GB_s": nbytes / streamed / 1e9, "speedup": plain / streamed,
+            "target": "HDF5 file in a temporary directory of the host"}
+
+
+def rsm_leg(I, dims, spacing):
+    """Reciprocal-space maps (q_x-q_y, q_x-q_z, q_y-q_z) of the volume's intensity from device axis
+    sums, against the reference's way (numpy.histogram2d of every voxel, on the host copy) for one
+    of the three maps."""
+    import numpy as np
+    import torch
+    sys.path.insert(0, str(ROOT / "oracle"))
+    from qpd_oracle import rsm_map
+    from qpop_diffraction.rsm import reciprocal_space_maps
+    nx, ny, nz = dims
+    q = [np.fft.fftshift(np.fft.fftfreq(m, spacing)) * 2 * np.pi for m in (nx, ny, nz)]
+    reciprocal_space_maps(I, dims, (0, 1, 2), q)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    maps = reciprocal_space_maps(I, dims, (0, 1, 2), q)
+    dev_s = time.perf_counter() - t0
+    Ixyz = I.cpu().numpy().reshape(nz, ny, nx).transpose(2, 1, 0)
+    t0 = time.perf_counter()
+    ref = rsm_map(Ixyz, q, 0, 2)[2]
+    host_s = time.perf_counter() - t0
+    err = float(np.max(np.abs(maps["qx_qz"][2] - ref)) / np.max(ref))
+    return {"maps": 3, "device_s": dev_s, "host_histogram_s_one_map": host_s,
+            "max_rel_diff_qx_qz": err, "note": "device: three axis sums over the volume in HBM plus three "
+            "2-D histograms of the sums; host: numpy.histogram2d of every voxel for the q_x-q_z map"}
+
+
 def large_leg(args):
     """One large volume on one GPU: FP32 time, and its error against FP64 on a 512 x 512 x nz crop around G."""
     import torch
@@ -465,6 +516,14 @@ def run():
                      "traffic": measured_traffic((nx, ny, nz), dom, tuned["passes"][dom]["chosen"]),
                      "bytes_per_launch": bpc[dom] * n},
     }
+    log("output path")
+    result["output"] = output_leg(out, (nx, ny, nz))
+    log(f"output: copy-then-write {result['output']['copy_then_write_s']:.3f} s, streamed "
+        f"{result['output']['streamed_s']:.3f} s")
+    log("reciprocal-space maps")
+    result["rsm"] = rsm_leg(out, (nx, ny, nz), w.params.dx)
+    log(f"rsm: device {result['rsm']['device_s'] * 1e3:.1f} ms for 3 maps, host histogram "
+        f"{result['rsm']['host_histogram_s_one_map']:.2f} s for 1")
     del vol, out
     engine._work.clear()
     torch.cuda.empty_cache()

@@ -114,6 +114,13 @@ typedef struct qpd_ingest {
 int qpd_ingest_fields(const qpd_ingest* in, int precision, void* P_out, double* U_out, signed char* phase_out,
                       void* stream);
 
+/* Sums of a float64 array A (dims[0] fastest) over each memory dim, for reciprocal-space
+ * maps (rsm.py:88-113: with the default bins on the rectilinear q grid, the `sum` map of
+ * two axes is the intensity summed over the third).  S0 (dims[1] x dims[2], dims[1]
+ * fastest) sums over dims[0], S1 (dims[0] x dims[2]) over dims[1], S2 (dims[0] x dims[1])
+ * over dims[2]; a NULL output is skipped.  Each sum runs in a fixed order. */
+int qpd_project(const int64_t dims[3], const double* A, double* S0, double* S1, double* S2, void* stream);
+
 /* Create a plan for one grid, crystal and precision (32: complex FP32 bulk with FP64
  * phase reduction and FP64-generated twiddles; 64: complex FP64 throughout).
  * Replaces the per-call setup of diffraction.py:75-94 and the HeFFTe plan of

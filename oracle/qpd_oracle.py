@@ -22,7 +22,12 @@ How each function is pinned (tests/test_oracle.py):
     tests/test_a1a2_domains.py of the reference, linearity in the weights, and
     golden vectors written by the reference's own NumPy path
     (tests/golden/make_golden.py);
-  * dft_direct: the DFT definition, checked against numpy.fft.
+  * dft_direct: the DFT definition, checked against numpy.fft;
+  * rsm_map: the reciprocal-space map of src/python/rsm.py:66-113 (bin edges from the
+    unique q values or a linspace, numpy.histogram2d of the voxels), pinned by its
+    closed form on the rectilinear q grid (a `sum` map with default bins is the
+    intensity summed over the third axis, a `mean` map that sum over its length) and
+    by conservation of the total intensity for any bins.
 """
 
 from __future__ import annotations
@@ -114,3 +119,30 @@ def max_dlog10(I, ref, floor_rel=1e-10):
     ref = np.asarray(ref, dtype=np.float64)
     mask = ref >= floor_rel * ref.max()
     return float(np.max(np.abs(np.log10(np.maximum(I[mask], 1e-300)) - np.log10(ref[mask]))))
+
+
+def rsm_edges(values, n_bins=None):
+    """Bin edges and centres of one q component (rsm.py:66-85)."""
+    if n_bins is not None:
+        edges = np.linspace(values.min(), values.max(), n_bins + 1)
+        return edges, 0.5 * (edges[:-1] + edges[1:])
+    unique = np.unique(np.round(values, decimals=10))
+    if unique.size == 1:
+        return np.array([unique[0] - 0.5, unique[0] + 0.5]), unique.copy()
+    mid = 0.5 * (unique[:-1] + unique[1:])
+    first = unique[0] - 0.5 * (unique[1] - unique[0])
+    last = unique[-1] + 0.5 * (unique[-1] - unique[-2])
+    return np.concatenate(([first], mid, [last])), unique
+
+
+def rsm_map(I, q, axis_a, axis_b, n_bins=None, aggregation="sum"):
+    """Map of the (nx, ny, nz) intensity on the q_a - q_b plane (rsm.py:88-113), every voxel histogrammed."""
+    grids = np.meshgrid(*q, indexing="ij")
+    qa, qb = grids[axis_a].ravel(), grids[axis_b].ravel()
+    ea, ca = rsm_edges(qa, n_bins)
+    eb, cb = rsm_edges(qb, n_bins)
+    total, _, _ = np.histogram2d(qa, qb, bins=(ea, eb), weights=np.asarray(I).ravel())
+    if aggregation == "sum":
+        return ca, cb, total
+    counts, _, _ = np.histogram2d(qa, qb, bins=(ea, eb))
+    return ca, cb, np.divide(total, counts, out=np.zeros_like(total), where=counts > 0)

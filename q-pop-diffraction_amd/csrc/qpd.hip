@@ -21,6 +21,7 @@
 #include "qpd_kernels.hpp"
 #include "qpd_launch.hpp"
 #include "qpd_ingest.hpp"
+#include "qpd_project.hpp"
 
 using namespace qpd;
 
@@ -1072,6 +1073,31 @@ int qpd_ingest_fields(const qpd_ingest* in, int precision, void* P_out, double* 
     hipStream_t st = static_cast<hipStream_t>(stream);
     QPD_HIP(precision == 32 ? ingest_dispatch<float>(a, in->P_bits, ub, P_out, U_out, phase_out, st)
                             : ingest_dispatch<double>(a, in->P_bits, ub, P_out, U_out, phase_out, st));
+    return QPD_OK;
+}
+
+int qpd_project(const int64_t dims[3], const double* A, double* S0, double* S1, double* S2, void* stream) {
+    g_error.clear();
+    if (!dims || !A) return fail(QPD_ERR_ARG, "null argument");
+    for (int d = 0; d < 3; ++d)
+        if (dims[d] < 1) return fail(QPD_ERR_ARG, "dims must be >= 1");
+    if (dims[0] * dims[1] * dims[2] >= (int64_t)1 << 31)
+        return fail(QPD_ERR_UNSUPPORTED, "arrays of 2^31 values or more are not supported");
+    const uint32_t d0 = (uint32_t)dims[0], d1 = (uint32_t)dims[1], d2 = (uint32_t)dims[2];
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (S0) {
+        const uint32_t rows = d1 * d2;
+        hipLaunchKernelGGL(project_inner, dim3((rows + 3u) / 4u), dim3(256), 0, st, A, S0, d0, rows);
+        QPD_HIP(hipGetLastError());
+    }
+    if (S1) {
+        hipLaunchKernelGGL(project_middle, dim3((d0 * d2 + 255u) / 256u), dim3(256), 0, st, A, S1, d0, d1, d2);
+        QPD_HIP(hipGetLastError());
+    }
+    if (S2) {
+        hipLaunchKernelGGL(project_outer, dim3((d0 * d1 + 255u) / 256u), dim3(256), 0, st, A, S2, d0, d1, d2);
+        QPD_HIP(hipGetLastError());
+    }
     return QPD_OK;
 }
 

@@ -21,7 +21,7 @@ EXPORTS = (
     "qpd_plan_create", "qpd_plan_destroy", "qpd_workspace_bytes", "qpd_launches_per_volume",
     "qpd_plan_schedules", "qpd_plan_schedule_name", "qpd_plan_set_schedule", "qpd_plan_get_schedule",
     "qpd_fast_lengths", "qpd_intensity", "qpd_intensity_pass", "qpd_pass_bytes_per_cell", "qpd_fft3d",
-    "qpd_fft_length_supported", "qpd_ingest_fields", "qpd_last_error", "qpd_version",
+    "qpd_fft_length_supported", "qpd_ingest_fields", "qpd_project", "qpd_last_error", "qpd_version",
 )
 
 ROLES = {"source": 0, "contiguous": 1, "strided": 2, "intensity": 3, "strided1": 4, "split": 5,
@@ -102,6 +102,8 @@ def load_library():
     lib.qpd_fft3d.restype = i32
     lib.qpd_ingest_fields.argtypes = [ctypes.POINTER(Ingest), i32, vp, vp, vp, vp]
     lib.qpd_ingest_fields.restype = i32
+    lib.qpd_project.argtypes = [ctypes.POINTER(i64), vp, vp, vp, vp, vp]
+    lib.qpd_project.restype = i32
     lib.qpd_fft_length_supported.argtypes = [i64]
     lib.qpd_fft_length_supported.restype = i32
     lib.qpd_last_error.argtypes = []
@@ -244,3 +246,20 @@ def fast_lengths(role):
     buf = (ctypes.c_int64 * max(1, n))()
     lib.qpd_fast_lengths(r, buf, n)
     return [int(buf[i]) for i in range(n)]
+
+
+def project(A, dims, which=(0, 1, 2), stream=None):
+    """Sums of a device float64 array (dims[0] fastest) over the memory dims in ``which``.
+
+    Returns {d: tensor}: the sum over dim d, shaped by the two other dims (slower one first).
+    """
+    import torch
+    if A.dtype != torch.float64 or not A.is_cuda or not A.is_contiguous():
+        raise TypeError("project needs a contiguous float64 CUDA tensor")
+    d0, d1, d2 = (int(v) for v in dims)
+    shapes = {0: (d2, d1), 1: (d2, d0), 2: (d1, d0)}
+    out = {d: torch.empty(shapes[d], dtype=torch.float64, device=A.device) for d in which}
+    ptr = lambda d: ctypes.c_void_p(out[d].data_ptr()) if d in out else None
+    check(load_library().qpd_project((ctypes.c_int64 * 3)(d0, d1, d2), ctypes.c_void_p(A.data_ptr()), ptr(0), ptr(1),
+                                     ptr(2), stream if stream is not None else current_stream()))
+    return out

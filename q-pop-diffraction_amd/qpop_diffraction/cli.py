@@ -23,7 +23,7 @@ from .atom_list import AtomList
 from .constants import L0, OS0
 from .diffraction import Engine, Volume, default_precision, q_axes
 from .h5lite import open_file
-from .io_utils import crop_slices, read_field, read_material_field, read_vector, write_diffraction_output
+from .io_utils import crop_slices, read_field, read_material_field, read_vector, write_diffraction_streamed
 from .json_input import load_input_json
 from .size_context import SizeContext
 
@@ -173,10 +173,12 @@ def main(argv=None) -> int:
     engine = Engine(params, atoms)
     dims, axis = (params.nx, params.ny, params.nz), (0, 1, 2)
     vol = Volume(dims, axis, f["P"], f["U"], f["phase"], f["fraction"], f["region"], args.precision)
-    I = engine.intensity(vol).cpu().numpy().reshape(params.nz, params.ny, params.nx)
+    I = engine.intensity(vol)
     print(f"  GPU path {time.perf_counter() - t0:.3f} s for {params.n} cells")
     qC = engine.tables[4]
-    write_diffraction_output(out_base, I, *q_axes(params, qC), step, zyx=True)
+    # the intensity streams from HBM into the file slab by slab
+    dt = write_diffraction_streamed(out_base, I, *q_axes(params, qC), step)
+    print(f"  wrote {out_base}.h5 in {dt:.3f} s")
     print("\nDiffraction simulation completed.")
     return 0
 

@@ -272,6 +272,29 @@ class Dataset:
     def __getitem__(self, key):
         return self.read(key)
 
+    def __setitem__(self, key, data):
+        self.write(key, data)
+
+    def write(self, key, data):
+        """Write ``data`` into the selection ``key`` (slices of unit step)."""
+        lib = _load()
+        start, count, _ = self._selection(key)
+        arr = np.ascontiguousarray(np.asarray(data, dtype=self.dtype)).reshape(count)
+        rank = len(self.shape)
+        c_start = (hsize_t * rank)(*start)
+        c_count = (hsize_t * rank)(*count)
+        fspace = lib.H5Dget_space(self._id)
+        try:
+            _check(lib.H5Sselect_hyperslab(fspace, _H5S_SELECT_SET, c_start, None, c_count, None), "select")
+            mspace = lib.H5Screate_simple(rank, c_count, None)
+            try:
+                _check(lib.H5Dwrite(self._id, _native(arr.dtype), mspace, fspace, _H5P_DEFAULT, arr.ctypes.data),
+                       f"write {self.name}")
+            finally:
+                lib.H5Sclose(mspace)
+        finally:
+            lib.H5Sclose(fspace)
+
     def __array__(self, dtype=None, copy=None):
         return self.read(dtype=dtype)
 
@@ -413,8 +436,21 @@ class File:
     def __setitem__(self, name, data):
         self.create_dataset(name, data=data)
 
-    def create_dataset(self, name, data, compression=None, shuffle=False, chunks=None, compression_opts=None):
+    def create_dataset(self, name, data=None, compression=None, shuffle=False, chunks=None, compression_opts=None,
+                       shape=None, dtype=None):
+        """Create (and write) a dataset; with data None, an unwritten one of the given shape and dtype."""
         lib = _load()
+        if data is None:
+            rank = len(shape)
+            native = _native(np.dtype(dtype))
+            space = lib.H5Screate_simple(rank, (hsize_t * rank)(*shape), None)
+            try:
+                did = _check(lib.H5Dcreate2(self._id, name.encode(), native, space, _H5P_DEFAULT, _H5P_DEFAULT,
+                                            _H5P_DEFAULT), f"create dataset {name}")
+                lib.H5Dclose(did)
+            finally:
+                lib.H5Sclose(space)
+            return self[name]
         arr = np.ascontiguousarray(np.asarray(data))
         if arr.dtype.kind == "b":
             arr = arr.astype(np.int8)

@@ -160,6 +160,70 @@ def write_diffraction_output(base_name, I, qx, qy, qz, step, zyx=False):
         fh["intensity"] = np.asarray(data, dtype=np.float64)
 
 
+_STAGING = {}
+
+
+def _staging(n, pinned):
+    """Two reusable host buffers of at least n float64 values (page-locked for device copies)."""
+    import torch
+    key = bool(pinned)
+    have = _STAGING.get(key)
+    if have is None or have[0].numel() < n:
+        have = [torch.empty(n, dtype=torch.float64, pin_memory=key) for _ in range(2)]
+        _STAGING[key] = have
+    return have
+
+
+def write_diffraction_streamed(base_name, I, qx, qy, qz, step, slab_bytes=64 << 20):
+    """``write_diffraction_output`` for an intensity still on the GPU, written as it arrives.
+
+    I: device tensor of nz ny nx float64 values in (z, y, x) order (the command line's
+    layout).  The file is the one ``write_diffraction_output`` writes; the intensity goes in
+    slabs of whole z planes: the copy of slab k into a pinned host buffer runs on its own
+    stream while slab k - 1 is written from the other buffer.  Returns the seconds spent.
+    """
+    import time
+    import torch
+    t0 = time.perf_counter()
+    nz, ny, nx = (len(qz), len(qy), len(qx))
+    flat = I.reshape(nz, ny * nx)
+    write_diffraction_axes(base_name, qx, qy, qz, step)
+    planes = max(1, min(nz, slab_bytes // max(1, ny * nx * 8)))
+    on_gpu = flat.is_cuda
+    bufs = _staging(planes * ny * nx, on_gpu)
+    bufs = [b[:planes * ny * nx].view(planes, ny * nx) for b in bufs]
+    copy = torch.cuda.Stream() if on_gpu else None
+    events = [torch.cuda.Event() if on_gpu else None for _ in range(2)]
+    if on_gpu:
+        copy.wait_stream(torch.cuda.current_stream())
+    with open_file(f"{base_name}.h5", "a") as fh:
+        ds = fh.create_dataset("intensity", shape=(nz, ny, nx), dtype=np.float64)
+        slabs = [(z0, min(nz, z0 + planes)) for z0 in range(0, nz, planes)]
+
+        def fetch(k):
+            z0, z1 = slabs[k]
+            b = bufs[k % 2]
+            if on_gpu:
+                with torch.cuda.stream(copy):
+                    b[:z1 - z0].copy_(flat[z0:z1], non_blocking=True)
+                    events[k % 2].record(copy)
+            else:
+                b[:z1 - z0].copy_(flat[z0:z1])
+
+        def store(k):
+            z0, z1 = slabs[k]
+            if on_gpu:
+                events[k % 2].synchronize()
+            ds[z0:z1] = bufs[k % 2][:z1 - z0].numpy().reshape(z1 - z0, ny, nx)
+
+        fetch(0)
+        for k in range(1, len(slabs)):
+            fetch(k)
+            store(k - 1)
+        store(len(slabs) - 1)
+    return time.perf_counter() - t0
+
+
 def load_diffraction_hdf5(filepath) -> dict:
     """Read a diffraction.h5 back: intensity as (1, nx, ny, nz) and the q grid (io_utils.py:186-207)."""
     with open_file(filepath, "r") as fh:

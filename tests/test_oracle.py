@@ -132,3 +132,24 @@ def test_oracle_weight_scaling(tmp_path):
     I1, _ = oracle_run(tmp_path / "w")
     mask = I0 > 1e-20 * I0.max()  # voxels above the FP64 round-off floor
     np.testing.assert_allclose(I1[mask], 0.09 * I0[mask], rtol=1e-6)
+
+
+def test_rsm_map_closed_forms():
+    """The oracle's reciprocal-space map on a rectilinear q grid: a default-bin `sum` map is the
+    intensity summed over the third axis, `mean` that sum over the axis length, and any binning
+    keeps the total intensity."""
+    from qpd_oracle import rsm_map
+    rng = np.random.default_rng(4)
+    I = rng.random((6, 5, 4)) ** 3
+    q = [np.linspace(-1, 1, 6) + 3.0, np.linspace(-2, 2, 5), np.linspace(30.0, 34.0, 4)]
+    for a, b in ((0, 1), (0, 2), (1, 2)):
+        c = 3 - a - b
+        ca, cb, m = rsm_map(I, q, a, b)
+        np.testing.assert_allclose(ca, q[a])
+        np.testing.assert_allclose(cb, q[b])
+        np.testing.assert_allclose(m, I.sum(axis=c), rtol=1e-13)
+        _, _, mean = rsm_map(I, q, a, b, aggregation="mean")
+        np.testing.assert_allclose(mean, I.sum(axis=c) / I.shape[c], rtol=1e-13)
+        _, _, coarse = rsm_map(I, q, a, b, n_bins=3)
+        assert coarse.shape == (3, 3)
+        np.testing.assert_allclose(coarse.sum(), I.sum(), rtol=1e-13)
