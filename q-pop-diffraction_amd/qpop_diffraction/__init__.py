@@ -1,8 +1,9 @@
 """Q-POP Diffraction for MI355X: kinematic X-ray diffraction from Q-POP-FerroDyn
 polarization and displacement fields, computed by hand-written HIP kernels.
 
-The public names mirror the reference package (src/python/__init__.py:6-40);
-``write_mgpu_params`` is absent because the multi-GPU two-step is not rebuilt.
+The public names mirror the reference package (src/python/__init__.py:6-40).
+``write_mgpu_params`` writes the reference's prepared-run file; ``params_run`` runs
+such a file on one GPU instead of the reference's MPI program.
 """
 
 from .constants import L0, OS0, PI
@@ -18,6 +19,7 @@ from .io_utils import (
     crop_slices,
     write_diffraction_output,
     write_diffraction_axes,
+    write_mgpu_params,
     load_diffraction_hdf5,
 )
 
@@ -39,5 +41,6 @@ __all__ = [
     "crop_slices",
     "write_diffraction_output",
     "write_diffraction_axes",
+    "write_mgpu_params",
     "load_diffraction_hdf5",
 ]

@@ -21,9 +21,10 @@ import numpy as np
 
 from .atom_list import AtomList
 from .constants import L0, OS0
-from .diffraction import Engine, Volume, default_precision, q_axes
+from .diffraction import Engine, Volume, default_precision, q_axes, reciprocal_centre
 from .h5lite import open_file
-from .io_utils import crop_slices, read_field, read_material_field, read_vector, write_diffraction_streamed
+from .io_utils import (crop_slices, read_field, read_material_field, read_vector, write_diffraction_axes,
+                       write_diffraction_streamed, write_mgpu_params)
 from .json_input import load_input_json
 from .size_context import SizeContext
 
@@ -42,6 +43,10 @@ def _parse_args(argv=None) -> argparse.Namespace:
     parser.add_argument("--precision", type=int, choices=(32, 64), default=None,
                         help="Working precision: 32 (complex FP32, FP64 phase reduction) or 64. "
                              "Default: QPOP_DIFFRACTION_PRECISION or 32.")
+    parser.add_argument("--mgpu", action="store_true",
+                        help="Only prepare the run: write {output_file}_params.h5 and the {output_file}.h5 skeleton "
+                             "(as the reference does for its multi-GPU program); run it on one GPU with "
+                             "`python -m qpop_diffraction.params_run {output_file}_params.h5`.")
     parser.add_argument("--host-ingest", action="store_true",
                         help="Prepare P, u, the phases and the u_z ramp with NumPy on the host instead of on the GPU "
                              "(the results are bit-identical).")
@@ -70,6 +75,8 @@ def prepare_run(args, device=None):
     if not atoms.read(args.atom, params.nPhase, params.nStruc) or not atoms.setup():
         return None
 
+    if getattr(args, "mgpu", False):
+        return _prepare_params(args, cfg, params, fe_ids, atoms, lims)
     if device:
         return _prepare_on_device(args, cfg, params, fe_ids, atoms, full, dz, sl, lims)
     real = np.float32 if args.precision == 32 else np.float64
@@ -119,6 +126,33 @@ def prepare_run(args, device=None):
     return params, atoms, fields, step, cfg["output_file"]
 
 
+def _prepare_params(args, cfg, params, fe_ids, atoms, lims):
+    """Write {output_file}_params.h5 and the output skeleton (cli.py:105-147); returns "prepared"."""
+    full = (params.nx, params.ny, params.nz)
+    with open_file(cfg["platt_file"], "r") as fh:
+        step = int(fh.attrs["Timestep"])
+    print(f"\nTimestep {step}:")
+    dc = params.reference_strain(np.diag(atoms.aC), params.temperature(step))
+    print("\nApplying domain slice:")
+    limits = params.apply_slice(*lims)
+    params.setup()
+    G, qC = reciprocal_centre(params, atoms)
+    phase_of = np.full(len(params.names), -1)
+    phase_of[fe_ids] = np.arange(len(fe_ids))
+    out = Path(f"{cfg['output_file']}.h5").resolve()
+    params_file = Path(f"{cfg['output_file']}_params.h5").resolve()
+    write_diffraction_axes(cfg["output_file"], *q_axes(params, qC), step)
+    resolve = lambda v: Path(v).resolve() if v is not None else None
+    write_mgpu_params(params_file, full, limits, (params.dx, params.dy, params.dz), G, qC, np.linalg.det(atoms.aC),
+                      atoms.xAtom @ atoms.aC, atoms.fAtom, np.stack([atoms.su1, atoms.su2, atoms.su3]) / OS0,
+                      phase_of, dc, resolve(cfg["platt_file"]), resolve(cfg["displacement_file"]),
+                      (Path(args.system).parent / "data" / "materialField_0.h5").resolve(),
+                      resolve(cfg["phase_file"]), resolve(cfg["region_file"]), out)
+    print("\nRun it on one GPU with:")
+    print(f"  python -m qpop_diffraction.params_run {params_file}")
+    return "prepared"
+
+
 def _prepare_on_device(args, cfg, params, fe_ids, atoms, full, dz, sl, lims):
     """prepare_run with the fields built on the GPU from the raw cropped components."""
     from .ingest import ingest, read_components, read_material_columns
@@ -166,6 +200,8 @@ def main(argv=None) -> int:
     prepared = prepare_run(args)
     if prepared is None:
         return 1
+    if prepared == "prepared":
+        return 0
     params, atoms, f, step, out_base = prepared
 
     print(f"\nComputing diffraction (MI355X, complex FP{args.precision})")

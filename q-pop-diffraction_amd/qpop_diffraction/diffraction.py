@@ -124,6 +124,13 @@ class Engine:
         self._plans = {}
         self._work = {}
 
+    @classmethod
+    def from_tables(cls, params, tables):
+        """An engine for crystal tables (R, f, su, G, qC, V) that were prepared elsewhere."""
+        eng = cls.__new__(cls)
+        eng.params, eng.atoms, eng.tables, eng._plans, eng._work = params, None, tuple(tables), {}, {}
+        return eng
+
     def plan(self, dims, axis, precision):
         key = (tuple(dims), tuple(axis), precision)
         if key not in self._plans:

@@ -344,8 +344,21 @@ class _Attrs:
             lib.H5Adelete(fid, name.encode())
         space = lib.H5Screate(_H5S_SCALAR)
         try:
-            if isinstance(value, (str, bytes, np.bytes_, np.str_)):
-                text = value.decode() if isinstance(value, (bytes, np.bytes_)) else str(value)
+            if isinstance(value, (bytes, np.bytes_)):
+                # fixed-length ASCII, as h5py stores numpy bytes
+                raw = bytes(value)
+                t = lib.H5Tcopy(_global_id("H5T_C_S1_g"))
+                lib.H5Tset_size(t, max(1, len(raw)))
+                try:
+                    aid = _check(lib.H5Acreate2(fid, name.encode(), t, space, _H5P_DEFAULT, _H5P_DEFAULT), name)
+                    buf = ctypes.create_string_buffer(raw, max(1, len(raw)))
+                    _check(lib.H5Awrite(aid, t, buf), name)
+                    lib.H5Aclose(aid)
+                finally:
+                    lib.H5Tclose(t)
+                return
+            if isinstance(value, (str, np.str_)):
+                text = str(value)
                 t = lib.H5Tcopy(_global_id("H5T_C_S1_g"))
                 lib.H5Tset_size(t, _H5T_VARIABLE)
                 lib.H5Tset_cset(t, _H5T_CSET_UTF8)

@@ -160,6 +160,60 @@ def write_diffraction_output(base_name, I, qx, qy, qz, step, zyx=False):
         fh["intensity"] = np.asarray(data, dtype=np.float64)
 
 
+def write_mgpu_params(path, full, limits, spacing, G, qC, V, R, fAtom, su, phase_of, dc,
+                      platt_file, displacement_file, material_file, phase_file, region_file, output_file):
+    """The inputs of a prepared run, ``{output_file}_params.h5`` (io_utils.py:144-183).
+
+    Datasets grid, crop (x0, x1, y0, y1, z0, z1 inclusive), spacing (nm), G, qC (nm^-1), V
+    (nm^3), R (nAtom, 3), fAtom_re / fAtom_im (nPhase, nAtom), su (3, nStruc, nPhase, nAtom),
+    phase_of and dc per material, u_scale (nm per m); file paths as string attributes.  The
+    reference's multi-GPU program reads this file (params.hpp:13-108); ``params_run`` runs it
+    on one GPU.
+    """
+    with open_file(str(path), "w") as fh:
+        fh["grid"] = np.asarray(full, dtype=np.int64)
+        fh["crop"] = np.asarray(limits, dtype=np.int64)
+        fh["spacing"] = np.asarray(spacing, dtype=np.float64)
+        fh["G"] = np.asarray(G, dtype=np.float64)
+        fh["qC"] = np.asarray(qC, dtype=np.float64)
+        fh["V"] = np.array([V], dtype=np.float64)
+        fh["R"] = np.asarray(R, dtype=np.float64)
+        fh["fAtom_re"] = np.ascontiguousarray(np.real(fAtom), dtype=np.float64)
+        fh["fAtom_im"] = np.ascontiguousarray(np.imag(fAtom), dtype=np.float64)
+        fh["su"] = np.asarray(su, dtype=np.float64)
+        fh["phase_of"] = np.asarray(phase_of, dtype=np.int32)
+        fh["dc"] = np.asarray(dc, dtype=np.float64)
+        fh["u_scale"] = np.array([1.0 / L0])
+        with open_file(str(platt_file), "r") as fp:
+            platt_prefix = next(p for p in ("P_Lattice", "PLatt") if f"{p}_x" in fp)
+        for key, value in (("platt_file", platt_file), ("platt_prefix", platt_prefix),
+                           ("displacement_file", displacement_file), ("material_file", material_file),
+                           ("phase_file", phase_file), ("region_file", region_file), ("output_file", output_file)):
+            if value is not None:
+                fh.attrs[key] = np.bytes_(str(value))
+    print(f"Writing prepared-run parameters → {path}")
+
+
+def read_mgpu_params(path) -> dict:
+    """``{output}_params.h5`` as written by ``write_mgpu_params`` (params.hpp:98-140); missing file
+    attributes are None."""
+    out = {}
+    with open_file(str(path), "r") as fh:
+        for key in ("grid", "crop", "spacing", "G", "qC", "V", "R", "fAtom_re", "fAtom_im", "su", "phase_of", "dc",
+                    "u_scale"):
+            if key not in fh:
+                raise KeyError(f"Missing dataset {key} in the prepared-run parameters {path}")
+            dt = np.int64 if key in ("grid", "crop", "phase_of") else np.float64
+            out[key] = read_dataset(fh, key, dtype=dt)
+        for key in ("platt_file", "platt_prefix", "displacement_file", "material_file", "phase_file", "region_file",
+                    "output_file"):
+            v = fh.attrs[key] if key in fh.attrs else None
+            out[key] = (v.decode() if isinstance(v, (bytes, np.bytes_)) else (None if v is None else str(v)))
+    if out["su"].ndim != 4 or out["su"].shape[1] != 3:
+        raise ValueError("the prepared run expects nStruc = 3 (P_x, P_y, P_z)")
+    return out
+
+
 _STAGING = {}
 
 
@@ -174,7 +228,7 @@ def _staging(n, pinned):
     return have
 
 
-def write_diffraction_streamed(base_name, I, qx, qy, qz, step, slab_bytes=64 << 20):
+def write_diffraction_streamed(base_name, I, qx, qy, qz, step, slab_bytes=64 << 20, axes=True):
     """``write_diffraction_output`` for an intensity still on the GPU, written as it arrives.
 
     I: device tensor of nz ny nx float64 values in (z, y, x) order (the command line's
@@ -187,7 +241,8 @@ def write_diffraction_streamed(base_name, I, qx, qy, qz, step, slab_bytes=64 << 
     t0 = time.perf_counter()
     nz, ny, nx = (len(qz), len(qy), len(qx))
     flat = I.reshape(nz, ny * nx)
-    write_diffraction_axes(base_name, qx, qy, qz, step)
+    if axes:
+        write_diffraction_axes(base_name, qx, qy, qz, step)
     planes = max(1, min(nz, slab_bytes // max(1, ny * nx * 8)))
     on_gpu = flat.is_cuda
     bufs = _staging(planes * ny * nx, on_gpu)
@@ -197,6 +252,8 @@ def write_diffraction_streamed(base_name, I, qx, qy, qz, step, slab_bytes=64 << 
     if on_gpu:
         copy.wait_stream(torch.cuda.current_stream())
     with open_file(f"{base_name}.h5", "a") as fh:
+        if "intensity" in fh:
+            del fh["intensity"]
         ds = fh.create_dataset("intensity", shape=(nz, ny, nx), dtype=np.float64)
         slabs = [(z0, min(nz, z0 + planes)) for z0 in range(0, nz, planes)]
 
