@@ -8,8 +8,9 @@ beside it) and parameter.atom.in, and writes {output_file}.h5 (/intensity shaped
 The fields stay in FerroDyn's file order (x fastest) from the HDF5 reader to the
 GPU and back to the writer: the plan is built for memory dims (nx, ny, nz) with
 physical axes (0, 1, 2), so the intensity comes out already shaped (nz, ny, nx).
-``--mgpu`` (the reference's multi-GPU two-step) is not offered: one MI355X holds
-the grids that program was written for.
+``--mgpu`` writes the reference's prepared-run file ({output_file}_params.h5) and the
+output skeleton; ``python -m qpop_diffraction.params_run`` then computes it on one
+MI355X, which holds the grids the reference's multi-GPU program was written for.
 """
 
 import argparse
