@@ -155,7 +155,8 @@ int qpd_plan_get_schedule(const qpd_plan* plan, int pass);   /* the selected ind
  * 2 strided 3 -> 2 pass, 3 strided pass to the intensity, 4 strided transform, 5 first
  * step of a split length, 6 and 7 roles 2 and 3 as the second step of a split length,
  * 8 and 9 the strided pass over source groups with one or two z-classes, 10 and 11 those
- * as the second step of a split length), ascending:
+ * as the second step of a split length, 12 the source pass over source groups with one
+ * wave per row), ascending:
  * writes up to cap of them and returns how many there are.  No counterpart in the
  * reference. */
 int qpd_fast_lengths(int role, int64_t* lengths, int cap);

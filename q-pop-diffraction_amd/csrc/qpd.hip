@@ -541,9 +541,9 @@ const FastEntry* fast_lookup(int64_t N, int role) {
 void set_fast(Launch& L, const FastEntry* fe, size_t lds) {
     L = Launch();
     L.fe = fe;
-    L.contiguous = fe->role <= kRoleContig;
+    L.contiguous = fe->role <= kRoleContig || fe->role == kRoleSourceW;
     L.nl = fe->G;
-    L.nf = fe->role == kRoleSource ? 4 : 1;
+    L.nf = (fe->role == kRoleSource || fe->role == kRoleSourceW) ? 4 : 1;
     L.nthr = fe->nthr;
     L.lds = lds;
 }
@@ -600,14 +600,20 @@ int pass_candidates(const AxisFFT& ax, int d, const int64_t dims[3], int nin, si
         else if (nin == 1)
             role = kRoleStrided1;
         if (role >= 0) {
-            for (const FastEntry& e : fast_registry()) {
-                if (e.N != N || e.role != role) continue;
-                const size_t lds = e.lds + (role == kRoleSource ? (size_t)4 * atomtab_entries : 0);
-                if ((int)lds > hard) continue;
-                PassPlan pp;
-                set_fast(pp.L[0], &e, lds);
-                pp.L[0].blocks = contig ? (lines + e.G - 1) / e.G : (dims[0] + e.G - 1) / e.G * n_other;
-                if (pp.L[0].blocks <= 0x7fffffffLL) out.push_back(pp);
+            // source-group plans (mix_role >= 0) run the one-wave-per-row source pass first
+            const int roles[2] = {role == kRoleSource && mix_role >= 0 ? kRoleSourceW : -1, role};
+            for (int rl : roles) {
+                if (rl < 0) continue;
+                for (const FastEntry& e : fast_registry()) {
+                    if (e.N != N || e.role != rl) continue;
+                    const bool src = rl == kRoleSource || rl == kRoleSourceW;
+                    const size_t lds = e.lds + (src ? (size_t)4 * atomtab_entries : 0);
+                    if ((int)lds > hard) continue;
+                    PassPlan pp;
+                    set_fast(pp.L[0], &e, lds);
+                    pp.L[0].blocks = contig ? (lines + e.G - 1) / e.G : (dims[0] + e.G - 1) / e.G * n_other;
+                    if (pp.L[0].blocks <= 0x7fffffffLL) out.push_back(pp);
+                }
             }
             const bool splittable =
                 role == kRoleStrided || role == kRoleIntensity || role == kRoleMix1 || role == kRoleMix2;
@@ -857,7 +863,7 @@ std::string describe_launch(const Launch& L) {
     char buf[160];
     static const char* role_name[kNumRoles] = {"source", "contiguous", "strided", "intensity", "strided1",
                                                "split-step", "strided", "intensity", "groups", "groups",
-                                               "groups", "groups"};
+                                               "groups", "groups", "source-wave"};
     if (L.fe)
         std::snprintf(buf, sizeof(buf), "fast %d (%s) G%d %s", L.fe->N, L.fe->radices, L.fe->G,
                       role_name[L.fe->role]);
@@ -999,6 +1005,13 @@ int qpd_plan_create(qpd_plan** plan, const qpd_grid* grid, const qpd_crystal* c,
             p->grouped = 0;
             rc = pass_candidates(p->fft[d], d, p->dims, nin[d], p->csize, true, c->n_atom * c->n_phase * kAtomTab,
                                  true, p->cand[d], -1);
+            // pass 0 of the class form has no one-wave-per-row schedule
+            std::vector<PassPlan>& c0 = p->cand[0];
+            c0.erase(std::remove_if(c0.begin(), c0.end(),
+                                    [](const PassPlan& pp) { return pp.L[0].fe && pp.L[0].fe->role == kRoleSourceW; }),
+                     c0.end());
+            p->sel[0] = 0;
+            p->pass[0] = c0[0];
         }
         if (rc == QPD_OK) {
             // default: a kernel of the length itself, else the most even split, else generic
@@ -1102,7 +1115,7 @@ int qpd_project(const int64_t dims[3], const double* A, double* S0, double* S1, 
 }
 
 int qpd_fast_lengths(int role, int64_t* lengths, int cap) {
-    if (role < 0 || role >= kNumRoles) return fail(QPD_ERR_ARG, "role must be 0..11");
+    if (role < 0 || role >= kNumRoles) return fail(QPD_ERR_ARG, "role must be 0..12");
     std::vector<int64_t> ls;
     for (const FastEntry& e : fast_registry())
         if (e.role == role && std::find(ls.begin(), ls.end(), (int64_t)e.N) == ls.end()) ls.push_back(e.N);

@@ -83,6 +83,105 @@ __device__ __forceinline__ void cell_load_f(const PassArgs<float>& a, uint32_t o
     c.e0 = {cs, -sn};
 }
 
+// The same inputs in two steps: the loads of a cell (raw_load_f), then the FP64-reduced
+// phase and the conversions (cell_prepare_f), so that a kernel can issue the loads of
+// several cells before any arithmetic.
+struct RawCellF {
+    float P0, P1, P2;
+    double U0, U1, U2;
+    float rg;
+    int ph;
+};
+
+// Per-lane base pointers of a lane's first cell; cell x of the lane is element x of each
+// (64-bit pointer arithmetic, so constant cell offsets fold into the load instructions).
+struct CellPtrsF {
+    const float *P0, *P1, *P2;
+    const double *U0, *U1, *U2;
+    const signed char* ph;
+    const float* rg;
+};
+
+__device__ __forceinline__ CellPtrsF cell_ptrs_f(const PassArgs<float>& a, size_t c0) {
+    CellPtrsF q;
+    q.P0 = a.P + c0;
+    q.P1 = q.P0 + a.ncell;
+    q.P2 = q.P1 + a.ncell;
+    q.U0 = a.U + c0;
+    q.U1 = q.U0 + a.ncell;
+    q.U2 = q.U1 + a.ncell;
+    q.ph = a.phase + c0;
+    q.rg = a.region ? a.region + c0 : nullptr;
+    return q;
+}
+
+__device__ __forceinline__ void raw_load_f(const CellPtrsF& q, int x, bool valid, RawCellF& c) {
+    c.ph = -1;
+    c.rg = 1.0f;
+    c.P0 = c.P1 = c.P2 = 0.0f;
+    c.U0 = c.U1 = c.U2 = 0.0;
+    if (!valid) return;
+    c.ph = q.ph[x];
+    if (q.rg) c.rg = q.rg[x];
+    c.P0 = q.P0[x];
+    c.P1 = q.P1[x];
+    c.P2 = q.P2[x];
+    c.U0 = q.U0[x];
+    c.U1 = q.U1[x];
+    c.U2 = q.U2[x];
+}
+
+// u in memory-dim order (u0 = U[ax0], ...), for cell_source_m.
+__device__ __forceinline__ void cell_prepare_f(const PassArgs<float>& a, const RawCellF& r, CellInputsF& c) {
+    c.ph = r.ph;
+    c.rg = r.rg;
+    c.P0 = r.P0;
+    c.P1 = r.P1;
+    c.P2 = r.P2;
+    const float ux = (float)r.U0, uy = (float)r.U1, uz = (float)r.U2;
+    c.u0 = a.ax0 == 0 ? ux : (a.ax0 == 1 ? uy : uz);
+    c.u1 = a.ax1 == 0 ? ux : (a.ax1 == 1 ? uy : uz);
+    c.u2 = a.ax2 == 0 ? ux : (a.ax2 == 1 ? uy : uz);
+    c.e0 = {1.0f, 0.0f};
+    if (r.ph < 0) return;
+    float sn, cs;
+    sincos_fp64_reduced(a.qC0 * r.U0 + a.qC1 * r.U1 + a.qC2 * r.U2, sn, cs);
+    c.e0 = {cs, -sn};
+}
+
+// Source of one cell for one atom, with the atom table's su rows and u already in
+// memory-dim order (tab[2 + 3 m + k] = su[ax_m][k], u_m = U[ax_m]): the same values as
+// cell_source_f without the per-atom axis selection.  One-hot phases take the
+// straight-line path; fractional phases loop over the phases.
+__device__ __forceinline__ void cell_source_m(const PassArgs<float>& a, const float* atomtab, const CellInputsF& ci,
+                                              uint32_t off, int atom, cx<float>& E, cx<float>& V0, cx<float>& V1,
+                                              cx<float>& V2) {
+    E = V0 = V1 = V2 = {0.0f, 0.0f};
+    if (ci.ph < 0) return;
+    auto add = [&](const float* tab, float w) {
+        const float u0 = ci.u0 + (tab[2] * ci.P0 + tab[3] * ci.P1 + tab[4] * ci.P2);
+        const float u1 = ci.u1 + (tab[5] * ci.P0 + tab[6] * ci.P1 + tab[7] * ci.P2);
+        const float u2 = ci.u2 + (tab[8] * ci.P0 + tab[9] * ci.P1 + tab[10] * ci.P2);
+        float sn, cs;
+        sincos_small(tab[11] * ci.P0 + tab[12] * ci.P1 + tab[13] * ci.P2, sn, cs);
+        const cx<float> fw = {tab[0] * w, tab[1] * w};
+        const cx<float> e = cmul(cmul(fw, ci.e0), cx<float>{cs, -sn});
+        E = E + e;
+        V0 = V0 + cscale(e, u0);
+        V1 = V1 + cscale(e, u1);
+        V2 = V2 + cscale(e, u2);
+    };
+    if (!a.fraction) {
+        add(atomtab + (atom * a.n_phase + ci.ph) * kAtomTab, ci.rg);
+        return;
+    }
+    for (int p = 0; p < a.n_phase; ++p) {
+        const float w = ci.rg * a.fraction[(size_t)p * a.ncell + off];
+        if (w == 0.0f) continue;
+        add(atomtab + (atom * a.n_phase + p) * kAtomTab, w);
+    }
+}
+
 // Source of one cell for one atom from CellInputsF (same values as cell_source).
 __device__ __forceinline__ void cell_source_f(const PassArgs<float>& a, const float* atomtab, const CellInputsF& ci,
                                               uint32_t off, int atom, cx<float>& E, cx<float>& V0, cx<float>& V1,
@@ -416,6 +515,208 @@ __global__ __launch_bounds__(4 * G * SZ::TPL) void fast_source(PassArgs<T> a) {
         }
         __syncthreads();
         par = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Source pass over source groups, one wave per row (64 / TPL rows per wave).
+//
+// Plans in the source-group form transform one representative atom per group in pass 0
+// with the identity coefficient: [A, B, C] = F0[E - i k0 U_d0, U_d1, U_d2].  Lane (g, j)
+// of a wave computes the source of cells j + r N / R0 of its row g, which are exactly the
+// inputs of its stage-0 butterfly, so the source never passes through LDS.  The four
+// fields then run one after another through the row's own LDS line.  Every hand-off is
+// between lanes of one wave, and the LDS serves a wave's instructions in program order,
+// so a compiler-level fence (wave_sync) orders them and no workgroup barrier is needed.
+// F[E] stays in registers until F[U_d0] is formed; A, B and C are stored from registers.
+// The twiddles of stage s sit in LDS as entry (r - 1) NS + k, so the lanes of a wave read
+// consecutive entries; odd radices keep their DFT constants after them.
+constexpr int kSrcWaves = 4;  // waves per block of the one-wave-per-row source pass
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Layout of the per-stage twiddle table: stage s starts at off(s) with ntw(s) twiddles
+// W_N^{r k N / (NS R)} at (r - 1) NS + k, then ncs(s) constants W_N^{m N / R}, m = 1..R/2.
+template <class SZ>
+struct StageTw {
+    static constexpr int ntw(int s) { return SZ::ns(s) > 1 ? (SZ::rad(s) - 1) * SZ::ns(s) : 0; }
+    static constexpr int ncs(int s) { return (SZ::rad(s) & 1) ? SZ::rad(s) / 2 : 0; }
+    static constexpr int off(int s) { return s == 0 ? 0 : off(s - 1) + ntw(s - 1) + ncs(s - 1); }
+    static constexpr int total() { return off(SZ::NST); }
+};
+
+template <typename T, class SZ, int S>
+__device__ __forceinline__ void fill_stage_tw(cx<T>* tw, const cx<T>* __restrict__ W, int tid, int nthr) {
+    if constexpr (S < SZ::NST) {
+        constexpr int R = SZ::rad(S), NS = SZ::ns(S), NR = SZ::nr(S);
+        constexpr int OFF = StageTw<SZ>::off(S), NTW = StageTw<SZ>::ntw(S), NCS = StageTw<SZ>::ncs(S);
+        constexpr int TWS = SZ::N / (NS * R);
+        for (int e = tid; e < NTW + NCS; e += nthr)
+            tw[OFF + e] = e < NTW ? W[(e / NS + 1) * (e % NS) * TWS] : W[(e - NTW + 1) * NR];
+    }
+}
+
+// Stage S of one line held in v (the same arithmetic as stage_compute).
+template <typename T, class SZ, int S>
+__device__ __forceinline__ void stage_compute_tw(cx<T> (&v)[SZ::RMAX], int j, const cx<T>* tw) {
+    constexpr int R = SZ::rad(S), NS = SZ::ns(S);
+    constexpr int OFF = StageTw<SZ>::off(S), NTW = StageTw<SZ>::ntw(S);
+    T cr[R / 2 + 1], sr[R / 2 + 1];
+    if constexpr ((R & 1) == 1) {
+#pragma unroll
+        for (int m = 1; m <= R / 2; ++m) {
+            const cx<T> w = tw[OFF + NTW + m - 1];
+            cr[m] = w.x;
+            sr[m] = -w.y;
+        }
+    }
+    if constexpr (NS > 1) {
+        const int k = j % NS;
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[r] = cmul(v[r], tw[OFF + (r - 1) * NS + k]);
+    }
+    dft<R>(v, cr, sr);
+}
+
+// All stages of one line: the stage-0 inputs x[j + r N/R0] are in v on entry, the outputs
+// X[j + r N/RL] on return.  act: the lane owns a line (inactive lanes touch no memory).
+template <typename T, class SZ>
+__device__ __forceinline__ void wave_line_transform(cx<T>* line, cx<T> (&v)[SZ::RMAX], int j, bool act,
+                                                    const cx<T>* tw) {
+    constexpr int NST = SZ::NST;
+    wave_sync();  // the previous line's reads come before this line's writes
+    const bool a0 = act && j < SZ::nr(0);
+    if (a0) stage_compute_tw<T, SZ, 0>(v, j, tw);
+    if constexpr (NST > 1) {
+        if (a0) {
+            constexpr int R = SZ::rad(0);
+#pragma unroll
+            for (int r = 0; r < R; ++r) line[lds_addr<false, SZ::PITCH, 0>(0, j * R + r)] = v[r];
+        }
+        wave_sync();
+        const bool a1 = act && j < SZ::nr(1);
+        if (a1) {
+            constexpr int R = SZ::rad(1), NR = SZ::nr(1), NS = SZ::ns(1);
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[r] = line[lds_addr<false, SZ::PITCH, 0>(0, j + r * NR)];
+            stage_compute_tw<T, SZ, 1>(v, j, tw);
+            if constexpr (NST > 2) {
+                const int o = (j / NS) * NS * R + (j % NS);
+#pragma unroll
+                for (int r = 0; r < R; ++r) line[lds_addr<false, SZ::PITCH, 0>(0, o + r * NS)] = v[r];
+            }
+        }
+        if constexpr (NST > 2) {
+            wave_sync();
+            const bool a2 = act && j < SZ::nr(2);
+            if (a2) {
+                constexpr int R = SZ::rad(2), NR = SZ::nr(2), NS = SZ::ns(2);
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[r] = line[lds_addr<false, SZ::PITCH, 0>(0, j + r * NR)];
+                stage_compute_tw<T, SZ, 2>(v, j, tw);
+                if constexpr (NST > 3) {
+                    const int o = (j / NS) * NS * R + (j % NS);
+#pragma unroll
+                    for (int r = 0; r < R; ++r) line[lds_addr<false, SZ::PITCH, 0>(0, o + r * NS)] = v[r];
+                }
+            }
+            if constexpr (NST > 3) {
+                wave_sync();
+                const bool a3 = act && j < SZ::nr(3);
+                if (a3) {
+                    constexpr int R = SZ::rad(3), NR = SZ::nr(3);
+#pragma unroll
+                    for (int r = 0; r < R; ++r) v[r] = line[lds_addr<false, SZ::PITCH, 0>(0, j + r * NR)];
+                    stage_compute_tw<T, SZ, 3>(v, j, tw);
+                }
+            }
+        }
+    }
+}
+
+#ifndef QPD_SRCW_WAVES
+#define QPD_SRCW_WAVES 3
+#endif
+template <typename T, class SZ, int ROWS>
+__global__ __launch_bounds__(64 * kSrcWaves) __attribute__((amdgpu_waves_per_eu(QPD_SRCW_WAVES)))
+void fast_source_wave(PassArgs<T> a) {
+    static_assert(std::is_same<T, float>::value, "the fast source pass is FP32");
+    constexpr int N = SZ::N, TPL = SZ::TPL, R0 = SZ::rad(0), NR0 = SZ::nr(0), RL = SZ::RL, NRL = SZ::NRL;
+    constexpr int NTHR = 64 * kSrcWaves;
+    constexpr int NTW = StageTw<SZ>::total();
+    static_assert(ROWS >= 1 && ROWS * TPL <= 64, "a wave holds whole rows");
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T>* lines = reinterpret_cast<cx<T>*>(smem_raw);
+    cx<T>* tw = lines + kSrcWaves * ROWS * SZ::PITCH;
+    T* tabl = reinterpret_cast<T*>(tw + NTW);
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int g = lane / TPL, j = lane - g * TPL;
+    const bool inrow = g < ROWS;
+    const uint32_t nrows = a.n1 * a.n2;
+    const uint32_t row = (blockIdx.x * (uint32_t)kSrcWaves + (uint32_t)wave) * ROWS + (uint32_t)g;
+    const bool live = inrow && row < nrows;
+    fill_stage_tw<T, SZ, 0>(tw, a.W, tid, NTHR);
+    fill_stage_tw<T, SZ, 1>(tw, a.W, tid, NTHR);
+    fill_stage_tw<T, SZ, 2>(tw, a.W, tid, NTHR);
+    fill_stage_tw<T, SZ, 3>(tw, a.W, tid, NTHR);
+    // atom table with the su rows in memory-dim order: entry 2 + 3 m + k <- 2 + 3 ax_m + k
+    for (int m = tid; m < a.n_atom_tab; m += NTHR) {
+        const int e = m % kAtomTab;
+        int src = m;
+        if (e >= 2 && e < 11) {
+            const int dm = (e - 2) / 3;
+            const int ax = dm == 0 ? a.ax0 : (dm == 1 ? a.ax1 : a.ax2);
+            src = m - e + 2 + 3 * ax + (e - 2) % 3;
+        }
+        tabl[m] = a.atomtab[src];
+    }
+    __syncthreads();
+    cx<T>* line = lines + (wave * ROWS + (inrow ? g : 0)) * SZ::PITCH;
+    const uint32_t cell0 = row * (uint32_t)N;
+
+    const bool has0 = live && j < NR0;  // the lane owns stage-0 inputs
+    const CellPtrsF cp = cell_ptrs_f(a, (size_t)cell0 + (size_t)j);
+    for (int grp = 0; grp < a.ngroups; ++grp) {
+        const int src = a.chunk_src[a.group_first[grp]];
+        cx<T> v[4][SZ::RMAX];
+        // every global load of the row's cells first, then one cell at a time (the
+        // scheduling barriers keep each cell's atom-table reads next to its arithmetic)
+        RawCellF raw[R0];
+#pragma unroll
+        for (int r = 0; r < R0; ++r) raw_load_f(cp, r * NR0, has0, raw[r]);
+#pragma unroll
+        for (int r = 0; r < R0; ++r) {
+            __builtin_amdgcn_sched_barrier(0);
+            CellInputsF ci;
+            cell_prepare_f(a, raw[r], ci);
+            cell_source_m(a, tabl, ci, cell0 + (uint32_t)(j + r * NR0), src, v[0][r], v[1][r], v[2][r], v[3][r]);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        cx<T>* oA = a.out + (size_t)grp * 3 * a.ncell + (size_t)cell0 + (size_t)j;
+        cx<T>* oB = oA + a.ncell;
+        cx<T>* oC = oB + a.ncell;
+        const T* kj = a.k + j;
+        wave_line_transform<T, SZ>(line, v[0], j, inrow, tw);
+        wave_line_transform<T, SZ>(line, v[1], j, inrow, tw);
+        if (live && j < NRL) {
+#pragma unroll
+            for (int r = 0; r < RL; ++r) oA[r * NRL] = v[0][r] + cscale(negi(v[1][r]), kj[r * NRL]);
+        }
+        wave_line_transform<T, SZ>(line, v[2], j, inrow, tw);
+        if (live && j < NRL) {
+#pragma unroll
+            for (int r = 0; r < RL; ++r) oB[r * NRL] = v[2][r];
+        }
+        wave_line_transform<T, SZ>(line, v[3], j, inrow, tw);
+        if (live && j < NRL) {
+#pragma unroll
+            for (int r = 0; r < RL; ++r) oC[r * NRL] = v[3][r];
+        }
     }
 }
 

@@ -21,6 +21,8 @@ template <class SZ, int ROLE, int G>
 hipError_t fast_launch(const Launch& L, const PassArgs<float>& a, hipStream_t st) {
     if constexpr (ROLE == kRoleSource)
         return launch_one(fast_source<float, SZ, G>, L, st, a);
+    else if constexpr (ROLE == kRoleSourceW)
+        return launch_one(fast_source_wave<float, SZ, G / kSrcWaves>, L, st, a);
     else if constexpr (ROLE == kRoleContig)
         return launch_one(fast_contig<float, SZ, G, 1, false>, L, st, a);
     else if constexpr (ROLE == kRoleStrided)
@@ -58,6 +60,19 @@ void add_fast(std::vector<FastEntry>& v, const char* radices) {
     }
 }
 
+// The one-wave-per-row source pass (lengths whose lines fit one wave and whose first
+// radix is at most 8, so that the four fields' stage-0 inputs fit in registers): G is the
+// rows of a block; the LDS holds one line per row and the per-stage twiddle table.
+template <class SZ>
+void add_source_wave(std::vector<FastEntry>& v, const char* radices) {
+    if constexpr (SZ::TPL <= 64 && SZ::rad(0) <= 8) {
+        constexpr int rows = 64 / SZ::TPL;
+        constexpr size_t lds = ((size_t)kSrcWaves * rows * SZ::PITCH + StageTw<SZ>::total()) * 8;
+        v.push_back({SZ::N, kRoleSourceW, kSrcWaves * rows, 64 * kSrcWaves, SZ::TPL, lds, radices,
+                     &fast_launch<SZ, kRoleSourceW, kSrcWaves * rows>});
+    }
+}
+
 // The default tile width first, then the variants an autotuner may pick: the source
 // pass with twice the rows while the block stays within 512 threads (beyond that the
 // register budget halves and its accumulators spill), strided passes with half the
@@ -81,7 +96,10 @@ constexpr unsigned kRolesSplitSteps = (1u << kRoleSplit) | (1u << kRoleStridedB)
 
 template <class SZ, unsigned ROLES>
 void add_fast_roles(std::vector<FastEntry>& v, const char* radices) {
-    if constexpr ((ROLES >> kRoleSource) & 1u) add_fast_variants<SZ, kRoleSource>(v, radices);
+    if constexpr ((ROLES >> kRoleSource) & 1u) {
+        add_source_wave<SZ>(v, radices);
+        add_fast_variants<SZ, kRoleSource>(v, radices);
+    }
     if constexpr ((ROLES >> kRoleContig) & 1u) add_fast_variants<SZ, kRoleContig>(v, radices);
     if constexpr ((ROLES >> kRoleStrided) & 1u) add_fast_variants<SZ, kRoleStrided>(v, radices);
     if constexpr ((ROLES >> kRoleIntensity) & 1u) add_fast_variants<SZ, kRoleIntensity>(v, radices);

@@ -30,7 +30,8 @@ enum FastRole {
     kRoleMix2 = 9,       // strided pass 1 over source groups, two z-classes
     kRoleMix1B = 10,     // kRoleMix1 as the second step of a split length
     kRoleMix2B = 11,     // kRoleMix2 as the second step of a split length
-    kNumRoles = 12,
+    kRoleSourceW = 12,   // contiguous pass with the source over source groups, one wave per row
+    kNumRoles = 13,
 };
 
 struct FastEntry;

@@ -26,7 +26,7 @@ EXPORTS = (
 
 ROLES = {"source": 0, "contiguous": 1, "strided": 2, "intensity": 3, "strided1": 4, "split": 5,
          "strided_split": 6, "intensity_split": 7, "strided_groups1": 8, "strided_groups2": 9,
-         "groups1_split": 10, "groups2_split": 11}
+         "groups1_split": 10, "groups2_split": 11, "source_wave": 12}
 
 c_double3 = ctypes.c_double * 3
 

@@ -59,8 +59,11 @@ def test_fft3d_every_strided_length():
     print(f"strided lengths: worst {worst:.2e}")
 
 
-def volume_check(refl, nx, ny, nz, seed=0, expect=None, schedules=None):
-    """FP32 volume of a twin snapshot against the oracle; returns max |dI| / peak."""
+def volume_check(refl, nx, ny, nz, seed=0, expect=None, schedules=None, pick=None):
+    """FP32 volume of a twin snapshot against the oracle; returns max |dI| / peak.
+
+    expect: {pass: prefix of the default schedule's name}; pick: {pass: suffix} selects the
+    first schedule whose name ends with it (which must exist)."""
     import torch
     from qpd_oracle import diffraction_fp64
     snap = workloads.twin_snapshot(("len", nx, ny, nz), nx, ny, nz, seed=seed)
@@ -70,6 +73,11 @@ def volume_check(refl, nx, ny, nz, seed=0, expect=None, schedules=None):
         for d, prefix in expect.items():
             names = plan.schedules(d)
             assert names[plan.schedule(d)].startswith(prefix), (d, names)
+    if pick is not None:
+        for d, suffix in pick.items():
+            names = plan.schedules(d)
+            idx = next(i for i, nm in enumerate(names) if nm.endswith(suffix))
+            plan.set_schedule(d, idx)
     if schedules is not None:
         for d, idx in schedules.items():
             plan.set_schedule(d, idx)
@@ -82,10 +90,13 @@ def volume_check(refl, nx, ny, nz, seed=0, expect=None, schedules=None):
     return rel_to_peak(I, ref)
 
 
-@pytest.mark.parametrize("role,dim,form", [("source", 0, "groups"), ("source", 0, "class"), ("strided", 1, "class"),
+@pytest.mark.parametrize("role,dim,form", [("source_wave", 0, "groups"), ("source", 0, "groups"),
+                                           ("source", 0, "class"), ("strided", 1, "class"),
                                            ("groups", 1, "groups"), ("intensity", 2, "groups")])
 def test_volume_every_length(refl, role, dim, form, monkeypatch):
-    """Pass 1 runs the source-group kernel for PbTiO3 unless QPD_CLASS_FORM=1."""
+    """Pass 1 runs the source-group kernel for PbTiO3 unless QPD_CLASS_FORM=1; pass 0 of
+    the source-group form defaults to the one-wave-per-row kernel where the length has one,
+    and the block-per-row source kernel is picked explicitly for the "source" case."""
     if form == "class":
         monkeypatch.setenv("QPD_CLASS_FORM", "1")
     else:
@@ -93,10 +104,17 @@ def test_volume_every_length(refl, role, dim, form, monkeypatch):
     worst = 0.0
     lengths = fast_lengths("strided_groups2" if role == "groups" else role)
     for N in lengths:
-        shape = [6, 5, 4]
+        # pass 1 of the source-group form needs a compile-time length on dim 1 (20); 500 rows
+        # leave a partial last block for most rows-per-block of the source kernels
+        shape = [6, 20, 25] if role.startswith("source") and form == "groups" else [6, 5, 4]
         shape[dim] = N
         print(f"{role} N={N} grid {shape}", flush=True)
-        err = volume_check(refl, *shape, expect={dim: "fast"})
+        if role == "source_wave":
+            err = volume_check(refl, *shape, expect={dim: "fast"}, pick={dim: " source-wave"})
+        elif role == "source" and form == "groups":
+            err = volume_check(refl, *shape, expect={dim: "fast"}, pick={dim: " source"})
+        else:
+            err = volume_check(refl, *shape, expect={dim: "fast"})
         worst = max(worst, err)
         assert err <= FP32_PEAK_TOL, (N, err)
     print(f"{role}: worst max|dI|/peak {worst:.2e}")
@@ -165,3 +183,48 @@ def test_source_groups_match_class_form(refl, shape, axis, monkeypatch):
     snap0 = workloads.twin_snapshot(("grp",) + shape, *shape, seed=9)
     ref = diffraction_fp64(snap0.params, refl.atoms, *workloads.host_inputs(snap0))[0].transpose(2, 1, 0)
     assert rel_to_peak(I, ref) <= FP32_PEAK_TOL and np.argmax(I) == np.argmax(ref)
+
+
+@pytest.mark.parametrize("case", ["two_phase", "fraction_region"])
+def test_source_schedules_with_weights(case):
+    """Every pass-0 schedule of a source-group plan (the one-wave-per-row kernel, the
+    block-per-row kernel, the generic one) on a two-phase volume: with non-scattering cells
+    (one-hot phase index, -1 in places), and with fractional phase weights and a region
+    weight (the fraction path).  Each against the oracle: 1e-5 of the peak, same peak voxel."""
+    import torch
+    from qpd_oracle import diffraction_fp64
+    from qpop_diffraction import Engine, Volume
+    nx, ny, nz = 64, 24, 20
+    w = workloads.two_phase(nx, ny, nz, n_grains=6, seed=4)
+    n = nx * ny * nz
+    gen = torch.Generator(device="cuda").manual_seed(11)
+    phase, fraction, region = w.phase.clone(), None, None
+    if case == "two_phase":
+        phase[torch.rand(n, generator=gen, device="cuda") < 0.2] = -1
+    else:
+        fraction = torch.rand((2, n), generator=gen, device="cuda", dtype=torch.float64)
+        fraction[fraction < 0.15] = 0.0
+        region = torch.rand(n, generator=gen, device="cuda", dtype=torch.float64)
+        phase = torch.zeros(n, dtype=torch.int8, device="cuda")
+    eng = Engine(w.params, w.atoms)
+    vol = Volume(w.dims, w.axis, w.P, w.U, phase, fraction, region, 32)
+    plan = eng.plan(vol.dims, vol.axis, 32)
+    names = plan.schedules(0)
+    assert "source-wave" in names[plan.schedule(0)], names
+    to_xyz = lambda t, k: t.double().reshape(k, nz, ny, nx).permute(0, 3, 2, 1).contiguous().cpu().numpy()
+    P = to_xyz(w.P, 3).ravel()
+    U = to_xyz(w.U, 3).ravel()
+    if fraction is None:
+        ph = to_xyz(phase, 1)[0]
+        oPhase = np.stack([(ph == p).astype(np.float64) for p in range(2)]).ravel()
+        reg = (ph >= 0).astype(np.float64).ravel()
+    else:
+        oPhase = to_xyz(fraction, 2).ravel()
+        reg = to_xyz(region, 1).ravel()
+    ref = diffraction_fp64(w.params, w.atoms, oPhase, P, U, reg)[0].transpose(2, 1, 0)
+    for idx, name in enumerate(names):
+        plan.set_schedule(0, idx)
+        I = eng.intensity(vol).cpu().numpy().reshape(nz, ny, nx)
+        err = rel_to_peak(I, ref)
+        print(f"{case} pass 0 schedule {name}: max|dI|/peak {err:.2e}", flush=True)
+        assert err <= FP32_PEAK_TOL and np.argmax(I) == np.argmax(ref), (name, err)
