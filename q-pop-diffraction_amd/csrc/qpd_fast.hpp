@@ -182,6 +182,24 @@ __device__ __forceinline__ void cell_source_m(const PassArgs<float>& a, const fl
     }
 }
 
+// One-hot phases: the source of one cell for one atom as e = w f e0 e^{-i g.P} and the
+// displacement u_m = u[ax_m] + su[ax_m] . P in memory-dim order (the fields are e and
+// e u_m; the same values as cell_source_m).
+__device__ __forceinline__ void cell_source_1h(const PassArgs<float>& a, const float* atomtab, const CellInputsF& ci,
+                                               int atom, cx<float>& e, float& u0, float& u1, float& u2) {
+    e = {0.0f, 0.0f};
+    u0 = u1 = u2 = 0.0f;
+    if (ci.ph < 0) return;
+    const float* tab = atomtab + (atom * a.n_phase + ci.ph) * kAtomTab;
+    u0 = ci.u0 + (tab[2] * ci.P0 + tab[3] * ci.P1 + tab[4] * ci.P2);
+    u1 = ci.u1 + (tab[5] * ci.P0 + tab[6] * ci.P1 + tab[7] * ci.P2);
+    u2 = ci.u2 + (tab[8] * ci.P0 + tab[9] * ci.P1 + tab[10] * ci.P2);
+    float sn, cs;
+    sincos_small(tab[11] * ci.P0 + tab[12] * ci.P1 + tab[13] * ci.P2, sn, cs);
+    const cx<float> fw = {tab[0] * ci.rg, tab[1] * ci.rg};
+    e = cmul(cmul(fw, ci.e0), cx<float>{cs, -sn});
+}
+
 // Source of one cell for one atom from CellInputsF (same values as cell_source).
 __device__ __forceinline__ void cell_source_f(const PassArgs<float>& a, const float* atomtab, const CellInputsF& ci,
                                               uint32_t off, int atom, cx<float>& E, cx<float>& V0, cx<float>& V1,
@@ -638,11 +656,11 @@ __device__ __forceinline__ void wave_line_transform(cx<T>* line, cx<T> (&v)[SZ::
     }
 }
 
-#ifndef QPD_SRCW_WAVES
-#define QPD_SRCW_WAVES 3
-#endif
+// Three waves per SIMD: the register budget (168) that holds a row's eight cells of raw
+// inputs and the four fields' stage-0 values with at most a couple of spilled values
+// (measured faster than two waves without spills or four with more spills).
 template <typename T, class SZ, int ROWS>
-__global__ __launch_bounds__(64 * kSrcWaves) __attribute__((amdgpu_waves_per_eu(QPD_SRCW_WAVES)))
+__global__ __launch_bounds__(64 * kSrcWaves) __attribute__((amdgpu_waves_per_eu(3)))
 void fast_source_wave(PassArgs<T> a) {
     static_assert(std::is_same<T, float>::value, "the fast source pass is FP32");
     constexpr int N = SZ::N, TPL = SZ::TPL, R0 = SZ::rad(0), NR0 = SZ::nr(0), RL = SZ::RL, NRL = SZ::NRL;
@@ -680,27 +698,12 @@ void fast_source_wave(PassArgs<T> a) {
     const uint32_t cell0 = row * (uint32_t)N;
 
     const bool has0 = live && j < NR0;  // the lane owns stage-0 inputs
-    const CellPtrsF cp = cell_ptrs_f(a, (size_t)cell0 + (size_t)j);
-    for (int grp = 0; grp < a.ngroups; ++grp) {
-        const int src = a.chunk_src[a.group_first[grp]];
-        cx<T> v[4][SZ::RMAX];
-        // every global load of the row's cells first, then one cell at a time (the
-        // scheduling barriers keep each cell's atom-table reads next to its arithmetic)
-        RawCellF raw[R0];
-#pragma unroll
-        for (int r = 0; r < R0; ++r) raw_load_f(cp, r * NR0, has0, raw[r]);
-#pragma unroll
-        for (int r = 0; r < R0; ++r) {
-            __builtin_amdgcn_sched_barrier(0);
-            CellInputsF ci;
-            cell_prepare_f(a, raw[r], ci);
-            cell_source_m(a, tabl, ci, cell0 + (uint32_t)(j + r * NR0), src, v[0][r], v[1][r], v[2][r], v[3][r]);
-        }
-        __builtin_amdgcn_sched_barrier(0);
+    const T* kj = a.k + j;
+    // transform group grp's four fields held in v and store A, B, C
+    auto emit = [&](cx<T>(&v)[4][SZ::RMAX], int grp) {
         cx<T>* oA = a.out + (size_t)grp * 3 * a.ncell + (size_t)cell0 + (size_t)j;
         cx<T>* oB = oA + a.ncell;
         cx<T>* oC = oB + a.ncell;
-        const T* kj = a.k + j;
         wave_line_transform<T, SZ>(line, v[0], j, inrow, tw);
         wave_line_transform<T, SZ>(line, v[1], j, inrow, tw);
         if (live && j < NRL) {
@@ -716,6 +719,94 @@ void fast_source_wave(PassArgs<T> a) {
         if (live && j < NRL) {
 #pragma unroll
             for (int r = 0; r < RL; ++r) oC[r * NRL] = v[3][r];
+        }
+    };
+    // Every global load of the row's cells is issued first, then the cells are formed one at
+    // a time (the scheduling barriers keep each cell's atom-table reads next to its use).
+    if (!a.fraction) {
+        // One-hot phases: a group's fields are e and e u_d, so a pair of groups is formed
+        // from one read of the cells and only e and u_d (5 values a cell) are kept.  Each
+        // field e u_d is formed just before its transform; F[e u_0] waits for F[e], which is
+        // transformed last, in place.
+        auto emit_eu = [&](cx<T>(&e)[SZ::RMAX], T(&u)[3][R0], int grp, uint32_t c0) {
+            cx<T>* oA = a.out + (size_t)grp * 3 * a.ncell + (size_t)c0 + (size_t)j;
+            cx<T>* oB = oA + a.ncell;
+            cx<T>* oC = oB + a.ncell;
+            cx<T> w[SZ::RMAX], f0[RL];
+#pragma unroll
+            for (int r = 0; r < R0; ++r) w[r] = cscale(e[r], u[0][r]);
+            wave_line_transform<T, SZ>(line, w, j, inrow, tw);
+#pragma unroll
+            for (int r = 0; r < RL; ++r) f0[r] = w[r];
+#pragma unroll
+            for (int r = 0; r < R0; ++r) w[r] = cscale(e[r], u[1][r]);
+            wave_line_transform<T, SZ>(line, w, j, inrow, tw);
+            if (live && j < NRL) {
+#pragma unroll
+                for (int r = 0; r < RL; ++r) oB[r * NRL] = w[r];
+            }
+#pragma unroll
+            for (int r = 0; r < R0; ++r) w[r] = cscale(e[r], u[2][r]);
+            wave_line_transform<T, SZ>(line, w, j, inrow, tw);
+            if (live && j < NRL) {
+#pragma unroll
+                for (int r = 0; r < RL; ++r) oC[r * NRL] = w[r];
+            }
+            wave_line_transform<T, SZ>(line, e, j, inrow, tw);
+            if (live && j < NRL) {
+#pragma unroll
+                for (int r = 0; r < RL; ++r) oA[r * NRL] = e[r] + cscale(negi(f0[r]), kj[r * NRL]);
+            }
+        };
+        for (int grp = 0; grp < a.ngroups; grp += 2) {
+            const bool pair = grp + 1 < a.ngroups;
+            const int srcA = a.chunk_src[a.group_first[grp]];
+            const int srcB = pair ? a.chunk_src[a.group_first[grp + 1]] : srcA;
+            // the row offset is re-formed in every iteration (the empty asm hides its value),
+            // so the 64-bit cell and output pointers are not held across the transforms
+            uint32_t c0 = cell0;
+            asm volatile("" : "+v"(c0));
+            const CellPtrsF cp = cell_ptrs_f(a, (size_t)c0 + (size_t)j);
+            cx<T> eA[SZ::RMAX], eB[SZ::RMAX];
+            T uA[3][R0], uB[3][R0];
+            // at most kAhead cells' loads in flight: cell r + kAhead is requested once cell r
+            // has been formed
+            constexpr int kAhead = R0 < 4 ? R0 : 4;
+            RawCellF raw[R0];
+#pragma unroll
+            for (int r = 0; r < kAhead; ++r) raw_load_f(cp, r * NR0, has0, raw[r]);
+#pragma unroll
+            for (int r = 0; r < R0; ++r) {
+                __builtin_amdgcn_sched_barrier(0);
+                CellInputsF ci;
+                cell_prepare_f(a, raw[r], ci);
+                if (r + kAhead < R0) raw_load_f(cp, (r + kAhead) * NR0, has0, raw[r + kAhead]);
+                cell_source_1h(a, tabl, ci, srcA, eA[r], uA[0][r], uA[1][r], uA[2][r]);
+                if (pair) cell_source_1h(a, tabl, ci, srcB, eB[r], uB[0][r], uB[1][r], uB[2][r]);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+            emit_eu(eA, uA, grp, c0);
+            if (pair) emit_eu(eB, uB, grp + 1, c0);
+        }
+    } else {
+        // fractional phases: one group per read of the cells
+        const CellPtrsF cp = cell_ptrs_f(a, (size_t)cell0 + (size_t)j);
+        for (int grp = 0; grp < a.ngroups; ++grp) {
+            const int src = a.chunk_src[a.group_first[grp]];
+            cx<T> v[4][SZ::RMAX];
+            RawCellF raw[R0];
+#pragma unroll
+            for (int r = 0; r < R0; ++r) raw_load_f(cp, r * NR0, has0, raw[r]);
+#pragma unroll
+            for (int r = 0; r < R0; ++r) {
+                __builtin_amdgcn_sched_barrier(0);
+                CellInputsF ci;
+                cell_prepare_f(a, raw[r], ci);
+                cell_source_m(a, tabl, ci, cell0 + (uint32_t)(j + r * NR0), src, v[0][r], v[1][r], v[2][r],
+                              v[3][r]);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+            emit(v, grp);
         }
     }
 }
@@ -737,7 +828,8 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided(PassArgs<T> a) {
     cx<T>* Wl = bufB + TILE;
     const int tid = threadIdx.x;
     const int g = tid % G, j = tid / G;
-    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    // rows of 256 bytes or more gain nothing from the XCD order (measured: they lose a little)
+    const uint32_t blk = G * sizeof(cx<T>) < 256 ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
     const uint32_t t0 = blk % a.ntiles0;
     const uint32_t outer_q = blk / a.ntiles0;
     // unsplit passes: q = 0 and unit steps, known at compile time

@@ -85,6 +85,9 @@ void add_fast_variants(std::vector<FastEntry>& v, const char* radices) {
         if constexpr (8 * G * SZ::TPL <= 512) add_fast<SZ, ROLE, 2 * G>(v, radices);
     } else if constexpr (ROLE != kRoleContig && G >= 8) {
         add_fast<SZ, ROLE, G / 2>(v, radices);
+        // 256-byte rows: fewer, longer HBM bursts per line of the tile
+        if constexpr (G == 16 && 32 * SZ::TPL <= 512 && (ROLE == kRoleIntensity || ROLE == kRoleStrided))
+            add_fast<SZ, ROLE, 32>(v, radices);
     }
 }
 
