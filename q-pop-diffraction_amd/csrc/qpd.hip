@@ -532,6 +532,20 @@ void fill_common(PassArgs<T>& a, const AxisFFT& ax, const Stage* stages_dev, con
 // Compile-time-length kernels (FP32), looked up by (length, role).
 constexpr int kMaxAtomTab = 1024;  // atom-table entries the source pass keeps in LDS
 
+// Private (scratch) bytes per lane of a kernel, 0 when unknown.
+size_t kernel_scratch_bytes(const void* kernel) {
+    if (!kernel) return 0;
+    static std::mutex mu;
+    static std::map<const void*, size_t> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find(kernel);
+    if (it != cache.end()) return it->second;
+    hipFuncAttributes attr{};
+    const size_t bytes = hipFuncGetAttributes(&attr, kernel) == hipSuccess ? attr.localSizeBytes : 0;
+    cache[kernel] = bytes;
+    return bytes;
+}
+
 const FastEntry* fast_lookup(int64_t N, int role) {
     for (const FastEntry& e : fast_registry())
         if (e.N == N && e.role == role) return &e;
@@ -600,12 +614,17 @@ int pass_candidates(const AxisFFT& ax, int d, const int64_t dims[3], int nin, si
         else if (nin == 1)
             role = kRoleStrided1;
         if (role >= 0) {
-            // source-group plans (mix_role >= 0) run the one-wave-per-row source pass first
-            const int roles[2] = {role == kRoleSource && mix_role >= 0 ? kRoleSourceW : -1, role};
-            for (int rl : roles) {
+            // source-group plans (mix_role >= 0) run the one-wave-per-row source pass first,
+            // unless its registers spill (more than 32 bytes a lane: lengths with a radix-11,
+            // 13 or 17 stage), in which case it follows the block-per-row kernels
+            const bool wave = role == kRoleSource && mix_role >= 0;
+            const int roles[3] = {wave ? kRoleSourceW : -1, role, wave ? kRoleSourceW : -1};
+            for (int pos = 0; pos < 3; ++pos) {
+                const int rl = roles[pos];
                 if (rl < 0) continue;
                 for (const FastEntry& e : fast_registry()) {
                     if (e.N != N || e.role != rl) continue;
+                    if (rl == kRoleSourceW && (pos == 0) != (kernel_scratch_bytes(e.kernel) <= 32)) continue;
                     const bool src = rl == kRoleSource || rl == kRoleSourceW;
                     const size_t lds = e.lds + (src ? (size_t)4 * atomtab_entries : 0);
                     if ((int)lds > hard) continue;

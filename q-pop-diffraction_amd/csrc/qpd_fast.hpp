@@ -728,8 +728,8 @@ void fast_source_wave(PassArgs<T> a) {
         // from one read of the cells and only e and u_d (5 values a cell) are kept.  Each
         // field e u_d is formed just before its transform; F[e u_0] waits for F[e], which is
         // transformed last, in place.
-        auto emit_eu = [&](cx<T>(&e)[SZ::RMAX], T(&u)[3][R0], int grp, uint32_t c0) {
-            cx<T>* oA = a.out + (size_t)grp * 3 * a.ncell + (size_t)c0 + (size_t)j;
+        auto emit_eu = [&](cx<T>(&e)[SZ::RMAX], T(&u)[3][R0], int grp, uint32_t lane_cell) {
+            cx<T>* oA = a.out + (size_t)grp * 3 * a.ncell + (size_t)lane_cell;
             cx<T>* oB = oA + a.ncell;
             cx<T>* oC = oB + a.ncell;
             cx<T> w[SZ::RMAX], f0[RL];
@@ -762,11 +762,12 @@ void fast_source_wave(PassArgs<T> a) {
             const bool pair = grp + 1 < a.ngroups;
             const int srcA = a.chunk_src[a.group_first[grp]];
             const int srcB = pair ? a.chunk_src[a.group_first[grp + 1]] : srcA;
-            // the row offset is re-formed in every iteration (the empty asm hides its value),
-            // so the 64-bit cell and output pointers are not held across the transforms
-            uint32_t c0 = cell0;
-            asm volatile("" : "+v"(c0));
-            const CellPtrsF cp = cell_ptrs_f(a, (size_t)c0 + (size_t)j);
+            // the lane's first cell is re-formed in every iteration (the empty asm hides its
+            // value), so only this 32-bit offset, not the 64-bit cell and output pointers
+            // made from it, is held across the transforms
+            uint32_t lane_cell = cell0 + (uint32_t)j;
+            asm volatile("" : "+v"(lane_cell));
+            const CellPtrsF cp = cell_ptrs_f(a, (size_t)lane_cell);
             cx<T> eA[SZ::RMAX], eB[SZ::RMAX];
             T uA[3][R0], uB[3][R0];
             // at most kAhead cells' loads in flight: cell r + kAhead is requested once cell r
@@ -785,8 +786,8 @@ void fast_source_wave(PassArgs<T> a) {
                 if (pair) cell_source_1h(a, tabl, ci, srcB, eB[r], uB[0][r], uB[1][r], uB[2][r]);
             }
             __builtin_amdgcn_sched_barrier(0);
-            emit_eu(eA, uA, grp, c0);
-            if (pair) emit_eu(eB, uB, grp + 1, c0);
+            emit_eu(eA, uA, grp, lane_cell);
+            if (pair) emit_eu(eB, uB, grp + 1, lane_cell);
         }
     } else {
         // fractional phases: one group per read of the cells

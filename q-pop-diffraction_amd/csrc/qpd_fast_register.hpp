@@ -56,7 +56,7 @@ void add_fast(std::vector<FastEntry>& v, const char* radices) {
         constexpr size_t tile = ROLE <= kRoleContig ? (size_t)F * G * SZ::PITCH : (size_t)SZ::N * (G + 1);
         constexpr size_t lds = (2 * tile + SZ::N) * 8;
         if constexpr (nthr <= 1024 && lds <= 160 * 1024)
-            v.push_back({SZ::N, ROLE, G, nthr, SZ::TPL, lds, radices, &fast_launch<SZ, ROLE, G>});
+            v.push_back({SZ::N, ROLE, G, nthr, SZ::TPL, lds, radices, &fast_launch<SZ, ROLE, G>, nullptr});
     }
 }
 
@@ -69,7 +69,8 @@ void add_source_wave(std::vector<FastEntry>& v, const char* radices) {
         constexpr int rows = 64 / SZ::TPL;
         constexpr size_t lds = ((size_t)kSrcWaves * rows * SZ::PITCH + StageTw<SZ>::total()) * 8;
         v.push_back({SZ::N, kRoleSourceW, kSrcWaves * rows, 64 * kSrcWaves, SZ::TPL, lds, radices,
-                     &fast_launch<SZ, kRoleSourceW, kSrcWaves * rows>});
+                     &fast_launch<SZ, kRoleSourceW, kSrcWaves * rows>,
+                     (const void*)fast_source_wave<float, SZ, rows>});
     }
 }
 

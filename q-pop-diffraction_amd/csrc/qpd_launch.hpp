@@ -68,6 +68,7 @@ struct FastEntry {
     size_t lds;   // dynamic LDS bytes, without the atom table of the source role
     const char* radices;
     FastLaunchFn launch;
+    const void* kernel;  // the kernel function (for its attributes)
 };
 
 // Raise a kernel's dynamic LDS limit once per (kernel, device) and size.
