@@ -86,27 +86,41 @@ def pass_times(plan, vol, out, work, steps):
 
 
 def cpu_baseline(w, crop):
-    """The FP64 NumPy oracle (one core) on a crop of the same workload: voxels/s."""
+    """The FP64 NumPy oracle (one core) on crops of the same workload: voxels/s.
+
+    `value` is the lateral crop x crop x nz corner; `grids` adds the 64x64x32 grid of the
+    CPU-runnable case and the two smaller grids of the served mix (128x128x32, 256x256x64),
+    each the corner crop of the same volume."""
     import numpy as np
     sys.path.insert(0, str(ROOT / "oracle"))
     from qpd_oracle import diffraction_fp64
     from qpop_diffraction.workloads import box, host_inputs, Workload
     nx, ny, nz = w.dims
+
+    def timed(cx, cy, cz):
+        P = w.P.reshape(3, nz, ny, nx)[:, :cz, :cy, :cx]
+        U = w.U.reshape(3, nz, ny, nx)[:, :cz, :cy, :cx]
+        ph = w.phase.reshape(nz, ny, nx)[:cz, :cy, :cx]
+        sub = Workload(box(cx, cy, cz, w.params.dx, w.params.nPhase), w.atoms, (cx, cy, cz), (0, 1, 2),
+                       P.reshape(3, -1), U.reshape(3, -1), ph.reshape(-1), "crop")
+        oPhase, Pc, Uc, region = host_inputs(sub)
+        t0 = time.perf_counter()
+        diffraction_fp64(sub.params, w.atoms, oPhase, Pc, Uc, region)
+        return time.perf_counter() - t0
+
     cx, cy = min(crop, nx), min(crop, ny)
-    P = w.P.reshape(3, nz, ny, nx)[:, :, :cy, :cx]
-    U = w.U.reshape(3, nz, ny, nx)[:, :, :cy, :cx]
-    ph = w.phase.reshape(nz, ny, nx)[:, :cy, :cx]
-    sub = Workload(box(cx, cy, nz, w.params.dx, w.params.nPhase), w.atoms, (cx, cy, nz), (0, 1, 2),
-                   P.reshape(3, -1), U.reshape(3, -1), ph.reshape(-1), "crop")
-    oPhase, Pc, Uc, region = host_inputs(sub)
-    t0 = time.perf_counter()
-    diffraction_fp64(sub.params, w.atoms, oPhase, Pc, Uc, region)
-    dt = time.perf_counter() - t0
+    dt = timed(cx, cy, nz)
+    grids = []
+    for g in ((64, 64, 32), (128, 128, 32), (256, 256, 64)):
+        if g[0] <= nx and g[1] <= ny and g[2] <= nz:
+            t = timed(*g)
+            grids.append({"grid": list(g), "s": t, "voxels_per_s": g[0] * g[1] * g[2] / t})
     n = cx * cy * nz
     return {"value": n / dt, "unit": "voxels/s", "cores": 1, "kind": "port",
             "sample": f"FP64 NumPy oracle (restatement of the reference diffraction()) on the "
                       f"{cx}x{cy}x{nz} corner crop of the same volume, {dt:.1f} s; host has {os.cpu_count()} "
-                      f"cores, numpy.fft and the elementwise steps use 1"}
+                      f"cores, numpy.fft and the elementwise steps use 1",
+            "grids": grids}
 
 
 def max_dlog10(I, ref):
