@@ -884,8 +884,8 @@ std::string describe_launch(const Launch& L) {
                                                "split-step", "strided", "intensity", "groups", "groups",
                                                "groups", "groups", "source-wave"};
     if (L.fe)
-        std::snprintf(buf, sizeof(buf), "fast %d (%s) G%d %s", L.fe->N, L.fe->radices, L.fe->G,
-                      role_name[L.fe->role]);
+        std::snprintf(buf, sizeof(buf), "fast %d (%s) G%d %s%s", L.fe->N, L.fe->radices, L.fe->G,
+                      role_name[L.fe->role], L.fe->no_prefetch ? " np" : "");
     else
         std::snprintf(buf, sizeof(buf), "generic %s nl%d nf%d nthr%d ept%d", L.contiguous ? "contiguous" : "strided",
                       L.nl, L.nf, L.nthr, L.ept);

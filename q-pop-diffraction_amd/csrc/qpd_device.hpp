@@ -162,8 +162,48 @@ __device__ __forceinline__ void dft8(cx<T>* v) {
     v[7] = e3 - o3;
 }
 
+// cos and sin of 2 pi m / R (m = 0..R/2) at compile time, for the odd-radix DFTs in FP32:
+// Taylor series in double on |x| <= pi / 2 (x = pi - 2 pi m / R past a quarter turn), then
+// rounded to float.  The host's twiddle table rounds cosl / sinl to double and then to
+// float, so the two agree unless a value falls within a double ulp of a float rounding
+// boundary.  As literals they cost no registers and no LDS reads.
+constexpr double taylor_cos(double x) {
+    double term = 1.0, sum = 1.0;
+    for (int k = 1; k < 14; ++k) {
+        term *= -x * x / ((2.0 * k - 1.0) * (2.0 * k));
+        sum += term;
+    }
+    return sum;
+}
+constexpr double taylor_sin(double x) {
+    double term = x, sum = x;
+    for (int k = 1; k < 14; ++k) {
+        term *= -x * x / ((2.0 * k) * (2.0 * k + 1.0));
+        sum += term;
+    }
+    return sum;
+}
+
+template <int R>
+struct OddRoots {
+    float c[R / 2 + 1], s[R / 2 + 1];
+    constexpr OddRoots() : c(), s() {
+        constexpr double pi = 3.14159265358979323846264338327950288;
+        for (int m = 0; m <= R / 2; ++m) {
+            const bool past = 4 * m > R;  // angle above pi / 2
+            const double x = past ? pi * (double)(R - 2 * m) / (double)R : 2.0 * pi * (double)m / (double)R;
+            c[m] = (float)(past ? -taylor_cos(x) : taylor_cos(x));
+            s[m] = (float)taylor_sin(x);
+        }
+    }
+};
+
+template <int R>
+__device__ constexpr OddRoots<R> kOddRoots{};
+
 // Odd-length DFT y[t] = sum_s v[s] exp(-2 pi i t s / R) using the symmetric pairs
-// a_s = v[s] + v[R-s], b_s = v[s] - v[R-s]; cr[m], sr[m] = cos, sin(2 pi m / R).
+// a_s = v[s] + v[R-s], b_s = v[s] - v[R-s]; cr[m], sr[m] = cos, sin(2 pi m / R) (FP32 uses
+// the compile-time values of kOddRoots instead).
 template <int R, typename T>
 __device__ __forceinline__ void dft_odd(cx<T>* v, const T* cr, const T* sr) {
     constexpr int H = (R - 1) / 2;
@@ -185,8 +225,10 @@ __device__ __forceinline__ void dft_odd(cx<T>* v, const T* cr, const T* sr) {
         for (int s = 1; s <= H; ++s) {
             const int m = (t * s) % R;
             const int mm = m <= H ? m : R - m;
-            const T c = cr[mm];
-            const T sn = m <= H ? sr[mm] : -sr[mm];
+            constexpr bool lit = std::is_same<T, float>::value;
+            const T c = lit ? (T)kOddRoots<R>.c[mm] : cr[mm];
+            const T s1 = lit ? (T)kOddRoots<R>.s[mm] : sr[mm];
+            const T sn = m <= H ? s1 : -s1;
             A.x += a[s].x * c;
             A.y += a[s].y * c;
             B.x += b[s].x * sn;

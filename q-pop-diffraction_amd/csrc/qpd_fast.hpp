@@ -816,7 +816,9 @@ void fast_source_wave(PassArgs<T> a) {
 // Strided pass: block = G consecutive i0 (columns) x one outer index; thread
 // (column g = tid % G, j = tid / G).  INTENSITY: write fftshifted max(|A|^2, 1e-50).
 // Rows follow the split addressing of PassArgs (plain passes: nq = 1, q = 0, unit steps).
-template <typename T, class SZ, int G, int NIN, bool INTENSITY, bool SPLIT>
+// PF: the next input is prefetched into registers while the current one is transformed;
+// without it the kernel needs fewer registers and more blocks fit on a CU.
+template <typename T, class SZ, int G, int NIN, bool INTENSITY, bool SPLIT, bool PF = true>
 __global__ __launch_bounds__(G* SZ::TPL) void fast_strided(PassArgs<T> a) {
     // the NIN inputs of a chunk are transformed one after another (one line per thread)
     constexpr int N = SZ::N, R0 = SZ::rad(0), NR0 = SZ::nr(0), LP = G + 1;
@@ -844,9 +846,7 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided(PassArgs<T> a) {
     const uint32_t in0 = q * a.in_qmul, out0 = q * a.out_qmul, c0 = q * a.c_qmul;
     load_twiddles<T, N>(Wl, a.W, tid, G * SZ::TPL);
     __syncthreads();
-    // inputs are consumed in the order (chunk, input); on long lines the next one is
-    // prefetched into registers while the current one is transformed
-    constexpr bool PF = true;
+    // inputs are consumed in the order (chunk, input)
     const int nitems = a.group_first[a.ngroups] * NIN;
     auto load_item = [&](int it, cx<T>(&dst)[1][SZ::RMAX]) {
         const int c = it / NIN, f = it - (it / NIN) * NIN;

@@ -17,7 +17,7 @@ constexpr int fast_tile(int role, int tpl) {
                                  : (16 * tpl <= 512 ? 16 : (8 * tpl <= 512 ? 8 : (4 * tpl <= 512 ? 4 : 0)));
 }
 
-template <class SZ, int ROLE, int G>
+template <class SZ, int ROLE, int G, bool PF = true>
 hipError_t fast_launch(const Launch& L, const PassArgs<float>& a, hipStream_t st) {
     if constexpr (ROLE == kRoleSource)
         return launch_one(fast_source<float, SZ, G>, L, st, a);
@@ -28,7 +28,7 @@ hipError_t fast_launch(const Launch& L, const PassArgs<float>& a, hipStream_t st
     else if constexpr (ROLE == kRoleStrided)
         return launch_one(fast_strided<float, SZ, G, 3, false, false>, L, st, a);
     else if constexpr (ROLE == kRoleIntensity)
-        return launch_one(fast_strided<float, SZ, G, 2, true, false>, L, st, a);
+        return launch_one(fast_strided<float, SZ, G, 2, true, false, PF>, L, st, a);
     else if constexpr (ROLE == kRoleStrided1)
         return launch_one(fast_strided<float, SZ, G, 1, false, false>, L, st, a);
     else if constexpr (ROLE == kRoleSplit)
@@ -47,7 +47,7 @@ hipError_t fast_launch(const Launch& L, const PassArgs<float>& a, hipStream_t st
         return launch_one(fast_strided_mix<float, SZ, G, 2, true>, L, st, a);
 }
 
-template <class SZ, int ROLE, int G>
+template <class SZ, int ROLE, int G, bool PF = true>
 void add_fast(std::vector<FastEntry>& v, const char* radices) {
     if constexpr (G >= 1) {
         constexpr int F = ROLE == kRoleSource ? 4 : 1;
@@ -56,7 +56,7 @@ void add_fast(std::vector<FastEntry>& v, const char* radices) {
         constexpr size_t tile = ROLE <= kRoleContig ? (size_t)F * G * SZ::PITCH : (size_t)SZ::N * (G + 1);
         constexpr size_t lds = (2 * tile + SZ::N) * 8;
         if constexpr (nthr <= 1024 && lds <= 160 * 1024)
-            v.push_back({SZ::N, ROLE, G, nthr, SZ::TPL, lds, radices, &fast_launch<SZ, ROLE, G>, nullptr});
+            v.push_back({SZ::N, ROLE, G, nthr, SZ::TPL, lds, radices, &fast_launch<SZ, ROLE, G, PF>, nullptr, !PF});
     }
 }
 
@@ -70,7 +70,7 @@ void add_source_wave(std::vector<FastEntry>& v, const char* radices) {
         constexpr size_t lds = ((size_t)kSrcWaves * rows * SZ::PITCH + StageTw<SZ>::total()) * 8;
         v.push_back({SZ::N, kRoleSourceW, kSrcWaves * rows, 64 * kSrcWaves, SZ::TPL, lds, radices,
                      &fast_launch<SZ, kRoleSourceW, kSrcWaves * rows>,
-                     (const void*)fast_source_wave<float, SZ, rows>});
+                     (const void*)fast_source_wave<float, SZ, rows>, false});
     }
 }
 
@@ -87,8 +87,14 @@ void add_fast_variants(std::vector<FastEntry>& v, const char* radices) {
     } else if constexpr (ROLE != kRoleContig && G >= 8) {
         add_fast<SZ, ROLE, G / 2>(v, radices);
         // 256-byte rows: fewer, longer HBM bursts per line of the tile
-        if constexpr (G == 16 && 32 * SZ::TPL <= 512 && (ROLE == kRoleIntensity || ROLE == kRoleStrided))
-            add_fast<SZ, ROLE, 32>(v, radices);
+        constexpr bool wide = G == 16 && 32 * SZ::TPL <= 512 && (ROLE == kRoleIntensity || ROLE == kRoleStrided);
+        if constexpr (wide) add_fast<SZ, ROLE, 32>(v, radices);
+        // the intensity pass without the register prefetch: a second block per CU where the
+        // prefetched line was what kept it out
+        if constexpr (ROLE == kRoleIntensity) {
+            add_fast<SZ, ROLE, G, false>(v, radices);
+            if constexpr (wide) add_fast<SZ, ROLE, 32, false>(v, radices);
+        }
     }
 }
 

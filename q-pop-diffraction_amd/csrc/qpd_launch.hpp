@@ -69,6 +69,7 @@ struct FastEntry {
     const char* radices;
     FastLaunchFn launch;
     const void* kernel;  // the kernel function (for its attributes)
+    bool no_prefetch;    // strided roles: the variant without the register prefetch
 };
 
 // Raise a kernel's dynamic LDS limit once per (kernel, device) and size.

@@ -131,7 +131,7 @@ def test_split_long_strided(refl, N, dim):
     assert err <= FP32_PEAK_TOL
 
 
-@pytest.mark.parametrize("shape", [(40, 1000, 6), (24, 20, 500)])
+@pytest.mark.parametrize("shape", [(40, 1000, 6), (24, 20, 500), (64, 20, 104)])
 def test_every_schedule(refl, shape):
     import torch
     snap = workloads.twin_snapshot(("sched",) + shape, *shape, seed=1)
