@@ -10,11 +10,12 @@
 // stages go through LDS.  Each thread transforms the F inputs of a chunk at once,
 // so its twiddles are loaded once per stage and reused F times.
 //
-// LDS layouts (8-byte complex points, 32 four-byte banks):
+// LDS layouts (8-byte complex points, four-byte banks):
 //   contiguous pass: line-major, point idx of line l at l * PITCH + idx + idx / 8
 //                    (the padding breaks the stride-R pattern of stage outputs);
-//   strided pass:    point-major, point idx of line l at idx * (L + 1) + l, with the
-//                    lanes of a wave walking consecutive columns (lines).
+//   strided pass:    point-major, point idx of line l in row idx (+ idx / 8 when the first
+//                    radix is a power of two) of L (+ 1 otherwise) points, with the lanes
+//                    of a wave walking consecutive columns (lines); see Sizes::PAD8.
 #pragma once
 
 #include <type_traits>
@@ -37,6 +38,12 @@ struct Sizes {
     static constexpr int RL = rad(NST - 1);
     static constexpr int NRL = N_ / RL;
     static constexpr int PITCH = N_ + N_ / 8 + 1;  // contiguous layout: padded line pitch
+    // strided tiles of G lines: with a power-of-two first radix, rows of G points plus one
+    // spare row per 8 (its stride-R0 first-stage writes and stride-1 reads then fall in
+    // distinct banks); otherwise rows of G + 1 points (an odd pitch spreads odd strides)
+    static constexpr bool PAD8 = (R0_ & (R0_ - 1)) == 0;
+    static constexpr int strided_lp(int G) { return PAD8 ? G : G + 1; }
+    static constexpr int strided_tile(int G) { return PAD8 ? (N_ + N_ / 8 + 1) * G : N_ * (G + 1); }
     static_assert(R0_ * (R1_ > 1 ? R1_ : 1) * (R2_ > 1 ? R2_ : 1) * (R3_ > 1 ? R3_ : 1) == N_, "radices must multiply to N");
 };
 
@@ -228,12 +235,12 @@ __device__ __forceinline__ void cell_source_f(const PassArgs<float>& a, const fl
     }
 }
 
-template <bool STRIDED, int PITCH, int LP>
+template <bool STRIDED, class SZ, int LP>
 __device__ __forceinline__ int lds_addr(int line, int idx) {
     if constexpr (STRIDED)
-        return idx * LP + line;
+        return (SZ::PAD8 ? idx + (idx >> 3) : idx) * LP + line;
     else
-        return line * PITCH + idx + (idx >> 3);
+        return line * SZ::PITCH + idx + (idx >> 3);
 }
 
 // Stage s on F lines held in registers v[f][r] (inputs of butterfly j already loaded):
@@ -283,7 +290,7 @@ __device__ __forceinline__ void stage_write(cx<T>* tile, const cx<T> (&v)[F][SZ:
     for (int f = 0; f < F; ++f)
 #pragma unroll
         for (int r = 0; r < R; ++r)
-            tile[lds_addr<STRIDED, SZ::PITCH, LP>(line0 + f * dline, o + r * NS)] = v[f][r];
+            tile[lds_addr<STRIDED, SZ, LP>(line0 + f * dline, o + r * NS)] = v[f][r];
 }
 
 // Read the inputs x[j + r N/R_S] of stage S from LDS.
@@ -293,7 +300,7 @@ __device__ __forceinline__ void stage_read(const cx<T>* tile, cx<T> (&v)[F][SZ::
 #pragma unroll
     for (int f = 0; f < F; ++f)
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[f][r] = tile[lds_addr<STRIDED, SZ::PITCH, LP>(line0 + f * dline, j + r * NR)];
+        for (int r = 0; r < R; ++r) v[f][r] = tile[lds_addr<STRIDED, SZ, LP>(line0 + f * dline, j + r * NR)];
 }
 
 // Stages 0 .. NST-1 of the F lines of butterfly j, with two LDS buffers X, Y for the
@@ -494,10 +501,10 @@ __global__ __launch_bounds__(4 * G * SZ::TPL) void fast_source(PassArgs<T> a) {
                     cx<T> v0, v1, v2, v3;
                     cell_source_f(a, tabl, ci[m], cell0 + e, src, v0, v1, v2, v3);
                     const int rr = e / N, x = e - rr * N;
-                    X[lds_addr<false, SZ::PITCH, 0>(0 * G + rr, x)] = v0;
-                    X[lds_addr<false, SZ::PITCH, 0>(1 * G + rr, x)] = v1;
-                    X[lds_addr<false, SZ::PITCH, 0>(2 * G + rr, x)] = v2;
-                    X[lds_addr<false, SZ::PITCH, 0>(3 * G + rr, x)] = v3;
+                    X[lds_addr<false, SZ, 0>(0 * G + rr, x)] = v0;
+                    X[lds_addr<false, SZ, 0>(1 * G + rr, x)] = v1;
+                    X[lds_addr<false, SZ, 0>(2 * G + rr, x)] = v2;
+                    X[lds_addr<false, SZ, 0>(3 * G + rr, x)] = v3;
                 }
             }
             __syncthreads();
@@ -518,7 +525,7 @@ __global__ __launch_bounds__(4 * G * SZ::TPL) void fast_source(PassArgs<T> a) {
         __syncthreads();
         if (f == 1 && j < NRL) {
 #pragma unroll
-            for (int r = 0; r < RL; ++r) ex[lds_addr<false, SZ::PITCH, 0>(g, j + r * NRL)] = acc[r];
+            for (int r = 0; r < RL; ++r) ex[lds_addr<false, SZ, 0>(g, j + r * NRL)] = acc[r];
         }
         __syncthreads();
         if (row < nrows && j < NRL && f != 1) {
@@ -527,7 +534,7 @@ __global__ __launch_bounds__(4 * G * SZ::TPL) void fast_source(PassArgs<T> a) {
 #pragma unroll
             for (int r = 0; r < RL; ++r) {
                 cx<T> val = acc[r];
-                if (f == 0) val = val + ex[lds_addr<false, SZ::PITCH, 0>(g, j + r * NRL)];
+                if (f == 0) val = val + ex[lds_addr<false, SZ, 0>(g, j + r * NRL)];
                 o[r * NRL] = val;
             }
         }
@@ -613,19 +620,19 @@ __device__ __forceinline__ void wave_line_transform(cx<T>* line, cx<T> (&v)[SZ::
         if (a0) {
             constexpr int R = SZ::rad(0);
 #pragma unroll
-            for (int r = 0; r < R; ++r) line[lds_addr<false, SZ::PITCH, 0>(0, j * R + r)] = v[r];
+            for (int r = 0; r < R; ++r) line[lds_addr<false, SZ, 0>(0, j * R + r)] = v[r];
         }
         wave_sync();
         const bool a1 = act && j < SZ::nr(1);
         if (a1) {
             constexpr int R = SZ::rad(1), NR = SZ::nr(1), NS = SZ::ns(1);
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[r] = line[lds_addr<false, SZ::PITCH, 0>(0, j + r * NR)];
+            for (int r = 0; r < R; ++r) v[r] = line[lds_addr<false, SZ, 0>(0, j + r * NR)];
             stage_compute_tw<T, SZ, 1>(v, j, tw);
             if constexpr (NST > 2) {
                 const int o = (j / NS) * NS * R + (j % NS);
 #pragma unroll
-                for (int r = 0; r < R; ++r) line[lds_addr<false, SZ::PITCH, 0>(0, o + r * NS)] = v[r];
+                for (int r = 0; r < R; ++r) line[lds_addr<false, SZ, 0>(0, o + r * NS)] = v[r];
             }
         }
         if constexpr (NST > 2) {
@@ -634,12 +641,12 @@ __device__ __forceinline__ void wave_line_transform(cx<T>* line, cx<T> (&v)[SZ::
             if (a2) {
                 constexpr int R = SZ::rad(2), NR = SZ::nr(2), NS = SZ::ns(2);
 #pragma unroll
-                for (int r = 0; r < R; ++r) v[r] = line[lds_addr<false, SZ::PITCH, 0>(0, j + r * NR)];
+                for (int r = 0; r < R; ++r) v[r] = line[lds_addr<false, SZ, 0>(0, j + r * NR)];
                 stage_compute_tw<T, SZ, 2>(v, j, tw);
                 if constexpr (NST > 3) {
                     const int o = (j / NS) * NS * R + (j % NS);
 #pragma unroll
-                    for (int r = 0; r < R; ++r) line[lds_addr<false, SZ::PITCH, 0>(0, o + r * NS)] = v[r];
+                    for (int r = 0; r < R; ++r) line[lds_addr<false, SZ, 0>(0, o + r * NS)] = v[r];
                 }
             }
             if constexpr (NST > 3) {
@@ -648,7 +655,7 @@ __device__ __forceinline__ void wave_line_transform(cx<T>* line, cx<T> (&v)[SZ::
                 if (a3) {
                     constexpr int R = SZ::rad(3), NR = SZ::nr(3);
 #pragma unroll
-                    for (int r = 0; r < R; ++r) v[r] = line[lds_addr<false, SZ::PITCH, 0>(0, j + r * NR)];
+                    for (int r = 0; r < R; ++r) v[r] = line[lds_addr<false, SZ, 0>(0, j + r * NR)];
                     stage_compute_tw<T, SZ, 3>(v, j, tw);
                 }
             }
@@ -821,9 +828,9 @@ void fast_source_wave(PassArgs<T> a) {
 template <typename T, class SZ, int G, int NIN, bool INTENSITY, bool SPLIT, bool PF = true>
 __global__ __launch_bounds__(G* SZ::TPL) void fast_strided(PassArgs<T> a) {
     // the NIN inputs of a chunk are transformed one after another (one line per thread)
-    constexpr int N = SZ::N, R0 = SZ::rad(0), NR0 = SZ::nr(0), LP = G + 1;
+    constexpr int N = SZ::N, R0 = SZ::rad(0), NR0 = SZ::nr(0), LP = SZ::strided_lp(G);
     constexpr int NOUT = NIN > 1 ? NIN - 1 : 1;
-    constexpr int TILE = N * LP;
+    constexpr int TILE = SZ::strided_tile(G);
     constexpr bool ALT = (chunk_writes<SZ, false>() & 1) == 1;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* bufA = reinterpret_cast<cx<T>*>(smem_raw);
@@ -932,8 +939,8 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided(PassArgs<T> a) {
 // the same outputs as the class form.  Each group's three inputs are transformed once.
 template <typename T, class SZ, int G, int NZ, bool SPLIT>
 __global__ __launch_bounds__(G* SZ::TPL) void fast_strided_mix(PassArgs<T> a) {
-    constexpr int N = SZ::N, R0 = SZ::rad(0), NR0 = SZ::nr(0), LP = G + 1, RL = SZ::RL, NRL = SZ::NRL;
-    constexpr int TILE = N * LP;
+    constexpr int N = SZ::N, R0 = SZ::rad(0), NR0 = SZ::nr(0), LP = SZ::strided_lp(G), RL = SZ::RL, NRL = SZ::NRL;
+    constexpr int TILE = SZ::strided_tile(G);
     constexpr bool ALT = (chunk_writes<SZ, false>() & 1) == 1;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* bufA = reinterpret_cast<cx<T>*>(smem_raw);
@@ -1032,8 +1039,8 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided_mix(PassArgs<T> a) {
 // Each block reads all of a line before it writes it, and blocks own disjoint lines.
 template <typename T, class SZ, int G>
 __global__ __launch_bounds__(G* SZ::TPL) void fast_split(PassArgs<T> a) {
-    constexpr int N = SZ::N, R0 = SZ::rad(0), NR0 = SZ::nr(0), LP = G + 1;
-    constexpr int TILE = N * LP;
+    constexpr int N = SZ::N, R0 = SZ::rad(0), NR0 = SZ::nr(0), LP = SZ::strided_lp(G);
+    constexpr int TILE = SZ::strided_tile(G);
     constexpr bool ALT = (chunk_writes<SZ, false>() & 1) == 1;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* bufA = reinterpret_cast<cx<T>*>(smem_raw);

@@ -53,7 +53,7 @@ void add_fast(std::vector<FastEntry>& v, const char* radices) {
         constexpr int F = ROLE == kRoleSource ? 4 : 1;
         constexpr int nthr = F * G * SZ::TPL;
         // two stage buffers and the twiddle table
-        constexpr size_t tile = ROLE <= kRoleContig ? (size_t)F * G * SZ::PITCH : (size_t)SZ::N * (G + 1);
+        constexpr size_t tile = ROLE <= kRoleContig ? (size_t)F * G * SZ::PITCH : (size_t)SZ::strided_tile(G);
         constexpr size_t lds = (2 * tile + SZ::N) * 8;
         if constexpr (nthr <= 1024 && lds <= 160 * 1024)
             v.push_back({SZ::N, ROLE, G, nthr, SZ::TPL, lds, radices, &fast_launch<SZ, ROLE, G, PF>, nullptr, !PF});
