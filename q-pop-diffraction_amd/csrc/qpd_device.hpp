@@ -141,6 +141,36 @@ __device__ __forceinline__ void dft4(cx<T>& v0, cx<T>& v1, cx<T>& v2, cx<T>& v3)
     v3 = t1 - t3;
 }
 
+// 3-point DFT in place (w = exp(-2 pi i / 3) = -1/2 - i sqrt(3)/2).
+template <typename T>
+__device__ __forceinline__ void dft3(cx<T>& a, cx<T>& b, cx<T>& c) {
+    const T h = (T)0.866025403784438646763723170752936183;  // sqrt(3) / 2
+    const cx<T> t1 = b + c, t2 = b - c;
+    const cx<T> m = {a.x - (T)0.5 * t1.x, a.y - (T)0.5 * t1.y};
+    a = a + t1;
+    b = {m.x + h * t2.y, m.y - h * t2.x};
+    c = {m.x - h * t2.y, m.y + h * t2.x};
+}
+
+// 6-point DFT: 3-point DFTs of the even and odd inputs, the odd ones times
+// exp(-2 pi i k / 6) (k = 1, 2), joined by a radix-2 step.
+template <typename T>
+__device__ __forceinline__ void dft6(cx<T>* v) {
+    const T h = (T)0.866025403784438646763723170752936183;  // sqrt(3) / 2
+    cx<T> e0 = v[0], e1 = v[2], e2 = v[4];
+    cx<T> o0 = v[1], o1 = v[3], o2 = v[5];
+    dft3(e0, e1, e2);
+    dft3(o0, o1, o2);
+    o1 = {(T)0.5 * o1.x + h * o1.y, (T)0.5 * o1.y - h * o1.x};
+    o2 = {-(T)0.5 * o2.x + h * o2.y, -(T)0.5 * o2.y - h * o2.x};
+    v[0] = e0 + o0;
+    v[3] = e0 - o0;
+    v[1] = e1 + o1;
+    v[4] = e1 - o1;
+    v[2] = e2 + o2;
+    v[5] = e2 - o2;
+}
+
 template <typename T>
 __device__ __forceinline__ void dft8(cx<T>* v) {
     const T h = (T)0.707106781186547524400844362104849039;  // sqrt(1/2)
@@ -247,6 +277,8 @@ __device__ __forceinline__ void dft(cx<T>* v, const T* cr, const T* sr) {
         dft2(v);
     } else if constexpr (R == 4) {
         dft4(v[0], v[1], v[2], v[3]);
+    } else if constexpr (R == 6) {
+        dft6(v);
     } else if constexpr (R == 8) {
         dft8(v);
     } else {

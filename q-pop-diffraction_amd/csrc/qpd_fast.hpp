@@ -553,9 +553,10 @@ __global__ __launch_bounds__(4 * G * SZ::TPL) void fast_source(PassArgs<T> a) {
 // fields then run one after another through the row's own LDS line.  Every hand-off is
 // between lanes of one wave, and the LDS serves a wave's instructions in program order,
 // so a compiler-level fence (wave_sync) orders them and no workgroup barrier is needed.
-// F[E] stays in registers until F[U_d0] is formed; A, B and C are stored from registers.
-// The twiddles of stage s sit in LDS as entry (r - 1) NS + k, so the lanes of a wave read
-// consecutive entries; odd radices keep their DFT constants after them.
+// A, B and C are stored from registers.  With one-hot phases both groups of a pair come
+// from one read of the cells (see the one-hot branch); fractional phases take one group
+// per read.  The twiddles of stage s sit in LDS as entry (r - 1) NS + k, so the lanes of a
+// wave read consecutive entries; odd radices keep their DFT constants after them.
 constexpr int kSrcWaves = 4;  // waves per block of the one-wave-per-row source pass
 
 __device__ __forceinline__ void wave_sync() {
@@ -706,7 +707,8 @@ void fast_source_wave(PassArgs<T> a) {
 
     const bool has0 = live && j < NR0;  // the lane owns stage-0 inputs
     const T* kj = a.k + j;
-    // transform group grp's four fields held in v and store A, B, C
+    // fractional phases: transform group grp's four fields held in v (F[E] waits in
+    // registers for F[U_d0]) and store A, B, C
     auto emit = [&](cx<T>(&v)[4][SZ::RMAX], int grp) {
         cx<T>* oA = a.out + (size_t)grp * 3 * a.ncell + (size_t)cell0 + (size_t)j;
         cx<T>* oB = oA + a.ncell;
@@ -728,8 +730,8 @@ void fast_source_wave(PassArgs<T> a) {
             for (int r = 0; r < RL; ++r) oC[r * NRL] = v[3][r];
         }
     };
-    // Every global load of the row's cells is issued first, then the cells are formed one at
-    // a time (the scheduling barriers keep each cell's atom-table reads next to its use).
+    // The cells are formed one at a time after their loads; the scheduling barriers keep each
+    // cell's atom-table reads next to its use.
     if (!a.fraction) {
         // One-hot phases: a group's fields are e and e u_d, so a pair of groups is formed
         // from one read of the cells and only e and u_d (5 values a cell) are kept.  Each

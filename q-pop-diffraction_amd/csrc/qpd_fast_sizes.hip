@@ -68,8 +68,10 @@ void QPD_CAT(register_fast_part, QPD_FAST_PART)(std::vector<FastEntry>& v) {
     add_if_mine<37, Sizes<224, 8, 4, 7>, kRolesContig>(v, "8x4x7");
     add_if_mine<38, Sizes<240, 4, 4, 3, 5>, kRolesContig>(v, "4x4x3x5");
     add_if_mine<39, Sizes<256, 4, 8, 8>, kRolesContig>(v, "4x8x8");
+    add_if_mine<112, Sizes<288, 8, 6, 6>, kRolesContig>(v, "8x6x6");  // 48 threads a line
     add_if_mine<40, Sizes<288, 8, 4, 3, 3>, kRolesContig>(v, "8x4x3x3");
     add_if_mine<41, Sizes<320, 8, 8, 5>, kRolesContig>(v, "8x8x5");
+    add_if_mine<114, Sizes<384, 8, 8, 6>, kRolesContig>(v, "8x8x6");  // 64 threads a line
     add_if_mine<42, Sizes<384, 4, 4, 8, 3>, kRolesContig>(v, "4x4x8x3");
     add_if_mine<43, Sizes<400, 4, 4, 5, 5>, kRolesContig>(v, "4x4x5x5");
     add_if_mine<44, Sizes<448, 8, 8, 7>, kRolesContig>(v, "8x8x7");
@@ -121,8 +123,10 @@ void QPD_CAT(register_fast_part, QPD_FAST_PART)(std::vector<FastEntry>& v) {
     add_if_mine<90, Sizes<224, 8, 7, 4>, kRolesStrided>(v, "8x7x4");
     add_if_mine<91, Sizes<240, 5, 4, 4, 3>, kRolesStrided>(v, "5x4x4x3");
     add_if_mine<92, Sizes<256, 8, 8, 4>, kRolesStrided>(v, "8x8x4");
+    add_if_mine<109, Sizes<288, 8, 6, 6>, kRolesStrided>(v, "8x6x6");
     add_if_mine<93, Sizes<288, 8, 4, 3, 3>, kRolesStrided>(v, "8x4x3x3");
     add_if_mine<94, Sizes<320, 8, 8, 5>, kRolesStrided>(v, "8x8x5");
+    add_if_mine<111, Sizes<384, 8, 8, 6>, kRolesStrided>(v, "8x8x6");
     add_if_mine<95, Sizes<384, 8, 4, 4, 3>, kRolesStrided>(v, "8x4x4x3");
     add_if_mine<96, Sizes<400, 5, 5, 4, 4>, kRolesStrided>(v, "5x5x4x4");
     add_if_mine<97, Sizes<448, 8, 8, 7>, kRolesStrided>(v, "8x8x7");
