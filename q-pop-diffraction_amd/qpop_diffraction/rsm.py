@@ -56,20 +56,45 @@ def axis_sums(I, dims, axis):
     return out
 
 
+def _bins(values, edges):
+    """Bin of each value as numpy.histogramdd assigns it (right-open bins, the last one closed;
+    0 and len(edges) mark values below and above the edges)."""
+    idx = np.searchsorted(edges, values, side="right")
+    idx[values == edges[-1]] -= 1
+    return idx
+
+
+def _histogram2d_grid(qa_1d, qb_1d, ea, eb, weights):
+    """numpy.histogram2d of the grid points (qa_1d[i], qb_1d[j]) in (i, j) C order with weights
+    (n_a, n_b): the same bins and the same order of additions (one bincount over the points),
+    with the binning done on the 1-D axes."""
+    ia, ib = _bins(qa_1d, ea), _bins(qb_1d, eb)
+    na, nb = len(ea) + 1, len(eb) + 1
+    flat = (ia[:, None] * nb + ib[None, :]).ravel()
+    hist = np.bincount(flat, weights=None if weights is None else np.asarray(weights, np.float64).ravel(),
+                       minlength=na * nb).reshape(na, nb)
+    return hist[1:-1, 1:-1].astype(np.float64)
+
+
 def maps_from_sums(sums, q, n_bins=None, aggregation="sum"):
-    """The three maps from the axis sums and the 1-D q axes (rsm.py:88-113)."""
+    """The three maps from the axis sums and the 1-D q axes (rsm.py:88-113).
+
+    The reference histograms the meshgrid of two q axes; its edges come from the unique values
+    of that meshgrid, which are the unique values of the 1-D axis, and its histogram adds the
+    points in meshgrid order, which _histogram2d_grid reproduces from the 1-D axes.
+    """
     n = [len(v) for v in q]
     maps = {}
     for a, b, stem in PAIR_DEFINITIONS:
         c = 3 - a - b
-        qa, qb = np.meshgrid(q[a], q[b], indexing="ij")
-        ea, ca = build_edges(qa.ravel(), n_bins)
-        eb, cb = build_edges(qb.ravel(), n_bins)
-        total, _, _ = np.histogram2d(qa.ravel(), qb.ravel(), bins=(ea, eb), weights=sums[c].ravel())
+        qa_1d, qb_1d = np.asarray(q[a], np.float64), np.asarray(q[b], np.float64)
+        ea, ca = build_edges(qa_1d, n_bins)
+        eb, cb = build_edges(qb_1d, n_bins)
+        total = _histogram2d_grid(qa_1d, qb_1d, ea, eb, sums[c])
         if aggregation == "sum":
             data = total
         else:
-            counts, _, _ = np.histogram2d(qa.ravel(), qb.ravel(), bins=(ea, eb))
+            counts = _histogram2d_grid(qa_1d, qb_1d, ea, eb, None)
             counts *= n[c]  # every (a, b) point stands for a whole column along c
             data = np.divide(total, counts, out=np.zeros_like(total), where=counts > 0)
         maps[stem] = (ca, cb, data)
