@@ -546,6 +546,14 @@ size_t kernel_scratch_bytes(const void* kernel) {
     return bytes;
 }
 
+// The strided fast kernels address a row as a wave-uniform base plus a 32-bit lane byte
+// offset of up to (64/G - 1) step S + G elements (rows step S apart, G columns): a
+// schedule whose offsets could reach 4 GiB is not offered.
+bool lane_offsets_fit(int G, int64_t step, int64_t S, size_t csize) {
+    const int64_t lanes = 64 / std::max(G, 1) - 1;
+    return (double)(lanes * step * S + G) * (double)csize < 4294967296.0;
+}
+
 const FastEntry* fast_lookup(int64_t N, int role) {
     for (const FastEntry& e : fast_registry())
         if (e.N == N && e.role == role) return &e;
@@ -603,6 +611,7 @@ int pass_candidates(const AxisFFT& ax, int d, const int64_t dims[3], int nin, si
     const int64_t N = dims[d];
     const int64_t lines = contig ? dims[1] * dims[2] : 0;
     const int64_t n_other = d == 1 ? dims[2] : dims[1];
+    const int64_t row_stride = d == 1 ? dims[0] : dims[0] * dims[1];
     if (csize == 8 && fast_enabled()) {
         int role = -1;
         if (contig)
@@ -628,6 +637,7 @@ int pass_candidates(const AxisFFT& ax, int d, const int64_t dims[3], int nin, si
                     const bool src = rl == kRoleSource || rl == kRoleSourceW;
                     const size_t lds = e.lds + (src ? (size_t)4 * atomtab_entries : 0);
                     if ((int)lds > hard) continue;
+                    if (!contig && !lane_offsets_fit(e.G, 1, row_stride, csize)) continue;
                     PassPlan pp;
                     set_fast(pp.L[0], &e, lds);
                     pp.L[0].blocks = contig ? (lines + e.G - 1) / e.G : (dims[0] + e.G - 1) / e.G * n_other;
@@ -649,6 +659,7 @@ int pass_candidates(const AxisFFT& ax, int d, const int64_t dims[3], int nin, si
                                                                     : kRoleMix2B;
                         for (const FastEntry& eb : fast_registry()) {
                             if (eb.N != n1 || eb.role != role_b || (int)eb.lds > hard) continue;
+                            if (!lane_offsets_fit(eb.G, n2, row_stride, csize)) continue;
                             PassPlan pp;
                             pp.nlaunch = 2;
                             pp.N1 = (int)n1;

@@ -235,6 +235,50 @@ __device__ __forceinline__ void cell_source_f(const PassArgs<float>& a, const fl
     }
 }
 
+// Workgroup barrier that orders LDS only.  __syncthreads() is a release/acquire fence
+// on every address space, so the compiler waits for all outstanding global loads
+// (vmcnt 0) before each barrier, which ends a register prefetch at the first LDS
+// hand-off.  These kernels exchange only LDS data across barriers: every global load
+// whose value a barrier publishes feeds an LDS write first, and the loads left in
+// flight (the next input line) never alias the stores made meanwhile (distinct
+// fields or buffers), so the fence is narrowed to the LDS address space.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// p[byte_off] with p wave-uniform and byte_off a 32-bit lane offset, as a buffer access: a
+// resource (scalar registers) over the 4 GiB from p and the lane offset as its 32-bit
+// vector operand.  A plain p + byte_off leaves the optimiser free to regroup the sum as
+// (row offset + lane offset) + base and keep a 64-bit vector address per row, and to
+// widen the offset outside a loop, after which the loop's accesses cannot use the 32-bit
+// form; the buffer form fixes the operands.  The resource's last word is untyped data
+// (format 32); the range check (offset < 2^32 - 1) never triggers.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc_at(const void* p) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)0xFFFFFFFF, 0x00020000);
+}
+template <typename V>
+__device__ __forceinline__ V load_at(const V* p, uint32_t byte_off) {
+    static_assert(sizeof(V) == 4 || sizeof(V) == 8 || sizeof(V) == 16, "4, 8 or 16 bytes");
+    if constexpr (sizeof(V) == 4)
+        return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b32(rsrc_at(p), byte_off, 0, 0));
+    else if constexpr (sizeof(V) == 8)
+        return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b64(rsrc_at(p), byte_off, 0, 0));
+    else
+        return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(rsrc_at(p), byte_off, 0, 0));
+}
+template <typename V>
+__device__ __forceinline__ void store_at(V* p, uint32_t byte_off, const V& v) {
+    static_assert(sizeof(V) == 8 || sizeof(V) == 16, "8 or 16 bytes");
+    if constexpr (sizeof(V) == 8)
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v),
+                                              rsrc_at(p), byte_off, 0, 0);
+    else
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(__attribute__((ext_vector_type(4))) unsigned, v),
+                                               rsrc_at(p), byte_off, 0, 0);
+}
+
 template <bool STRIDED, class SZ, int LP>
 __device__ __forceinline__ int lds_addr(int line, int idx) {
     if constexpr (STRIDED)
@@ -328,7 +372,7 @@ __device__ __forceinline__ void transform(cx<T>* X, cx<T>* Y, cx<T> (&v)[F][SZ::
     if (a0) stage_compute<T, SZ, 0, F>(v, j, W);
     if constexpr (SZ::NST > 1) {
         if (a0) stage_write<T, SZ, 0, F, STRIDED, LP>(wr, v, j, line0, dline);
-        __syncthreads();
+        lds_barrier();
         rd = wr;
         wr = (wr == X) ? Y : X;
         const bool a1 = j < SZ::nr(1);
@@ -338,7 +382,7 @@ __device__ __forceinline__ void transform(cx<T>* X, cx<T>* Y, cx<T> (&v)[F][SZ::
         }
         if constexpr (SZ::NST > 2) {
             if (a1) stage_write<T, SZ, 1, F, STRIDED, LP>(wr, v, j, line0, dline);
-            __syncthreads();
+            lds_barrier();
             rd = wr;
             wr = (wr == X) ? Y : X;
             const bool a2 = j < SZ::nr(2);
@@ -348,7 +392,7 @@ __device__ __forceinline__ void transform(cx<T>* X, cx<T>* Y, cx<T> (&v)[F][SZ::
             }
             if constexpr (SZ::NST > 3) {
                 if (a2) stage_write<T, SZ, 2, F, STRIDED, LP>(wr, v, j, line0, dline);
-                __syncthreads();
+                lds_barrier();
                 rd = wr;
                 const bool a3 = j < SZ::nr(3);
                 if (a3) {
@@ -408,7 +452,7 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_contig(PassArgs<T> a) {
     const bool row_ok = row < nrows;
     const uint32_t cell0 = row * (uint32_t)N;
     load_twiddles<T, N>(Wl, a.W, tid, G * TPL);
-    __syncthreads();
+    lds_barrier();
     int par = 0;
     for (int grp = 0; grp < a.ngroups; ++grp) {
         cx<T> acc[NOUT][SZ::RL];
@@ -481,7 +525,7 @@ __global__ __launch_bounds__(4 * G * SZ::TPL) void fast_source(PassArgs<T> a) {
         const int e = tid + m * NTHR;
         cell_load_f(a, cell0 + e, e < G * N && row0 + e / N < nrows, ci[m]);
     }
-    __syncthreads();
+    lds_barrier();
 
     int par = 0;
     for (int grp = 0; grp < a.ngroups; ++grp) {
@@ -507,7 +551,7 @@ __global__ __launch_bounds__(4 * G * SZ::TPL) void fast_source(PassArgs<T> a) {
                     X[lds_addr<false, SZ, 0>(3 * G + rr, x)] = v3;
                 }
             }
-            __syncthreads();
+            lds_barrier();
             cx<T> v[1][SZ::RMAX];
             transform<T, SZ, 1, false, 0, true>(X, Y, v, j, f * G + g, G, Wl);
             if (j < NRL) {
@@ -522,12 +566,12 @@ __global__ __launch_bounds__(4 * G * SZ::TPL) void fast_source(PassArgs<T> a) {
         }
         // out0 = (f = 0) + (f = 1): the f = 1 threads pass their sums through LDS
         cx<T>* ex = bufA;
-        __syncthreads();
+        lds_barrier();
         if (f == 1 && j < NRL) {
 #pragma unroll
             for (int r = 0; r < RL; ++r) ex[lds_addr<false, SZ, 0>(g, j + r * NRL)] = acc[r];
         }
-        __syncthreads();
+        lds_barrier();
         if (row < nrows && j < NRL && f != 1) {
             const int q = f == 0 ? 0 : f - 1;
             cx<T>* o = a.out + ((size_t)grp * 3 + q) * a.ncell + cell0 + (uint32_t)g * N + j;
@@ -538,7 +582,7 @@ __global__ __launch_bounds__(4 * G * SZ::TPL) void fast_source(PassArgs<T> a) {
                 o[r * NRL] = val;
             }
         }
-        __syncthreads();
+        lds_barrier();
         par = 0;
     }
 }
@@ -701,7 +745,7 @@ void fast_source_wave(PassArgs<T> a) {
         }
         tabl[m] = a.atomtab[src];
     }
-    __syncthreads();
+    lds_barrier();
     cx<T>* line = lines + (wave * ROWS + (inrow ? g : 0)) * SZ::PITCH;
     const uint32_t cell0 = row * (uint32_t)N;
 
@@ -851,19 +895,29 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided(PassArgs<T> a) {
     const uint32_t c_step = SPLIT ? a.c_step : 1u;
     const uint32_t i0 = t0 * G + g;
     const bool col_ok = i0 < a.n0;
-    const uint32_t colbase = i0 + outer * a.OS;
     const uint32_t in0 = q * a.in_qmul, out0 = q * a.out_qmul, c0 = q * a.c_qmul;
+    // j = jw + jl, where jw is the same on every lane of a wave (G divides 64): a row
+    // address is a wave-uniform pointer (scalar registers) plus one 32-bit lane byte
+    // offset (the host keeps these offsets below 4 GiB).  The idle columns of a partial
+    // tile read the tile's last column: each column is transformed on its own, and
+    // theirs are never stored.
+    static_assert(64 % G == 0, "a wave holds whole rows of the tile");
+    const uint32_t jw = (uint32_t)__builtin_amdgcn_readfirstlane(tid / 64) * (64 / G);
+    const uint32_t jl = (uint32_t)(tid % 64) / G;
+    const size_t tile0 = (size_t)t0 * G + (size_t)outer * a.OS;
+    const uint32_t in_lane = ((col_ok ? i0 : a.n0 - 1) - t0 * G + jl * in_step * a.S) * (uint32_t)sizeof(cx<T>);
+    const uint32_t c_lane = jl * c_step;
     load_twiddles<T, N>(Wl, a.W, tid, G * SZ::TPL);
-    __syncthreads();
+    lds_barrier();
     // inputs are consumed in the order (chunk, input)
     const int nitems = a.group_first[a.ngroups] * NIN;
     auto load_item = [&](int it, cx<T>(&dst)[1][SZ::RMAX]) {
         const int c = it / NIN, f = it - (it / NIN) * NIN;
-        const cx<T>* in = a.in + ((size_t)a.chunk_src[c] * a.nin + f) * a.ncell + colbase;
+        const cx<T>* in = a.in + ((size_t)a.chunk_src[c] * a.nin + f) * a.ncell + tile0;
         if (j < NR0) {
 #pragma unroll
             for (int r = 0; r < R0; ++r)
-                dst[0][r] = col_ok ? in[(in0 + (uint32_t)(j + r * NR0) * in_step) * a.S] : cx<T>{(T)0, (T)0};
+                dst[0][r] = load_at(in + (size_t)((in0 + (jw + r * NR0) * in_step) * a.S), in_lane);
         }
     };
     cx<T> nxt[1][SZ::RMAX];
@@ -878,6 +932,7 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided(PassArgs<T> a) {
             for (int r = 0; r < SZ::RL; ++r) acc[o][r] = {(T)0, (T)0};
         for (int c = a.group_first[grp]; c < a.group_first[grp + 1]; ++c) {
             const int ct = a.chunk_coef[c];
+            const cx<T>* coef = a.coef + (ct >= 0 ? (size_t)ct * a.Nc : 0);
 #pragma unroll
             for (int f = 0; f < NIN; ++f, ++it) {
                 cx<T> v[1][SZ::RMAX];
@@ -891,11 +946,15 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided(PassArgs<T> a) {
                 transform<T, SZ, 1, true, LP, false>(par ? bufB : bufA, par ? bufA : bufB, v, j, g, 0, Wl);
                 if (ALT) par ^= 1;
                 if (j < SZ::NRL) {
+                    const uint32_t coff = c_lane * (uint32_t)sizeof(cx<T>);
+                    const uint32_t koff = c_lane * (uint32_t)sizeof(T);
 #pragma unroll
                     for (int r = 0; r < SZ::RL; ++r) {
-                        const int idx = (int)(c0 + (uint32_t)(j + r * SZ::NRL) * c_step);
-                        cx<T> t = cmul(chunk_coef(a, ct, idx), v[0][r]);
-                        if (f == 1) t = cscale(negi(t), a.k[idx]);
+                        // coefficient index c0 + (j + r NRL) c_step, uniform part + lane part
+                        const uint32_t iu = c0 + (jw + r * SZ::NRL) * c_step;
+                        const cx<T> cf = ct >= 0 ? load_at(coef + iu, coff) : cx<T>{(T)1, (T)0};
+                        cx<T> t = cmul(cf, v[0][r]);
+                        if (f == 1) t = cscale(negi(t), load_at(a.k + iu, koff));
                         const int oq = f <= 1 ? 0 : f - 1;
                         acc[oq][r] = acc[oq][r] + t;
                     }
@@ -918,12 +977,14 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided(PassArgs<T> a) {
                 a.I[(size_t)s0 + (size_t)a.n0 * ((size_t)s1 + (size_t)a.n1 * s2)] = fmax(re * re + im * im, 1e-50);
             }
         } else {
-            cx<T>* o = a.out + (size_t)grp * NOUT * a.ncell + colbase;
+            cx<T>* o = a.out + (size_t)grp * NOUT * a.ncell + tile0;
+            const uint32_t out_lane = (g + jl * out_step * a.S) * (uint32_t)sizeof(cx<T>);
 #pragma unroll
             for (int qq = 0; qq < NOUT; ++qq)
 #pragma unroll
                 for (int r = 0; r < SZ::RL; ++r)
-                    o[(size_t)qq * a.ncell + (out0 + (uint32_t)(j + r * SZ::NRL) * out_step) * a.S] = acc[qq][r];
+                    store_at(o + (size_t)qq * a.ncell + (size_t)((out0 + (jw + r * SZ::NRL) * out_step) * a.S), out_lane,
+                             acc[qq][r]);
         }
     }
 }
@@ -961,17 +1022,23 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided_mix(PassArgs<T> a) {
     const uint32_t in0 = q * a.in_qmul, out0 = q * a.out_qmul, c0 = q * a.c_qmul;
     const uint32_t i0 = t0 * G + g;
     const bool col_ok = i0 < a.n0;
-    const uint32_t colbase = i0 + outer * a.OS;
+    // row addresses: wave-uniform pointer + 32-bit lane offset, as in fast_strided
+    static_assert(64 % G == 0, "a wave holds whole rows of the tile");
+    const uint32_t jw = (uint32_t)__builtin_amdgcn_readfirstlane(tid / 64) * (64 / G);
+    const uint32_t jl = (uint32_t)(tid % 64) / G;
+    const size_t tile0 = (size_t)t0 * G + (size_t)outer * a.OS;
+    const uint32_t in_lane = ((col_ok ? i0 : a.n0 - 1) - t0 * G + jl * in_step * a.S) * (uint32_t)sizeof(cx<T>);
+    const uint32_t c_lane = jl * c_step;
     load_twiddles<T, N>(Wl, a.W, tid, G * SZ::TPL);
-    __syncthreads();
+    lds_barrier();
     constexpr bool PF = true;
     const int nitems = 3 * a.nsrc;
     auto load_item = [&](int it, cx<T>(&dst)[1][SZ::RMAX]) {
-        const cx<T>* in = a.in + (size_t)it * a.ncell + colbase;
+        const cx<T>* in = a.in + (size_t)it * a.ncell + tile0;
         if (j < NR0) {
 #pragma unroll
             for (int r = 0; r < R0; ++r)
-                dst[0][r] = col_ok ? in[(in0 + (uint32_t)(j + r * NR0) * in_step) * a.S] : cx<T>{(T)0, (T)0};
+                dst[0][r] = load_at(in + (size_t)((in0 + (jw + r * NR0) * in_step) * a.S), in_lane);
         }
     };
     cx<T> acc[NZ][2][RL];
@@ -998,23 +1065,30 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided_mix(PassArgs<T> a) {
             transform<T, SZ, 1, true, LP, false>(par ? bufB : bufA, par ? bufA : bufB, v, j, g, 0, Wl);
             if (ALT) par ^= 1;
             if (j < NRL) {
+                const uint32_t koff = c_lane * (uint32_t)sizeof(T);
 #pragma unroll
                 for (int r = 0; r < RL; ++r) {
                     if (f == 0) tA[r] = v[0][r];
-                    if (f == 1) tA[r] = tA[r] + cscale(negi(v[0][r]), a.k[c0 + (uint32_t)(j + r * NRL) * c_step]);
+                    if (f == 1) tA[r] = tA[r] + cscale(negi(v[0][r]), load_at(a.k + c0 + (jw + r * NRL) * c_step, koff));
                     if (f == 2) tC[r] = v[0][r];
                 }
             }
         }
-        if (j < NRL && col_ok) {
+        // the pair lists are walked by the whole wave (their entries stay in scalar
+        // registers); only the lanes of stored points accumulate
 #pragma unroll
-            for (int z = 0; z < NZ; ++z) {
-                for (int p = a.pair_first[s * NZ + z]; p < a.pair_first[s * NZ + z + 1]; ++p) {
-                    const int y = a.pair_y[p];
-                    const cx<T> xc = a.mixc[(size_t)(y * a.nsrc + s) * a.n0 + i0];
+        for (int z = 0; z < NZ; ++z) {
+            const int p1 = __builtin_amdgcn_readfirstlane(a.pair_first[s * NZ + z + 1]);
+            for (int p = __builtin_amdgcn_readfirstlane(a.pair_first[s * NZ + z]); p < p1; ++p) {
+                const int y = __builtin_amdgcn_readfirstlane(a.pair_y[p]);
+                const cx<T>* xrow = a.mixc + (size_t)(y * a.nsrc + s) * a.n0 + t0 * G;
+                const cx<T>* cy = a.coef + (size_t)y * a.Nc + c0;
+                if (j < NRL && col_ok) {
+                    const cx<T> xc = load_at(xrow, g * (uint32_t)sizeof(cx<T>));
+                    const uint32_t coff = c_lane * (uint32_t)sizeof(cx<T>);
 #pragma unroll
                     for (int r = 0; r < RL; ++r) {
-                        const cx<T> cf = cmul(a.coef[y * a.Nc + c0 + (uint32_t)(j + r * NRL) * c_step], xc);
+                        const cx<T> cf = cmul(load_at(cy + (jw + r * NRL) * c_step, coff), xc);
                         acc[z][0][r] = acc[z][0][r] + cmul(cf, tA[r]);
                         acc[z][1][r] = acc[z][1][r] + cmul(cf, tC[r]);
                     }
@@ -1023,13 +1097,15 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided_mix(PassArgs<T> a) {
         }
     }
     if (!col_ok || j >= NRL) return;
+    const uint32_t out_lane = (g + jl * out_step * a.S) * (uint32_t)sizeof(cx<T>);
 #pragma unroll
     for (int z = 0; z < NZ; ++z)
 #pragma unroll
         for (int oq = 0; oq < 2; ++oq) {
-            cx<T>* o = a.out + (size_t)(2 * z + oq) * a.ncell + colbase;
+            cx<T>* o = a.out + (size_t)(2 * z + oq) * a.ncell + tile0;
 #pragma unroll
-            for (int r = 0; r < RL; ++r) o[(out0 + (uint32_t)(j + r * NRL) * out_step) * a.S] = acc[z][oq][r];
+            for (int r = 0; r < RL; ++r)
+                store_at(o + (size_t)((out0 + (jw + r * NRL) * out_step) * a.S), out_lane, acc[z][oq][r]);
         }
 }
 
@@ -1060,7 +1136,7 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_split(PassArgs<T> a) {
     const uint32_t colbase = i0 + outer * a.OS + q * a.in_qmul * a.S;
     const uint32_t rstride = a.in_step * a.S;
     load_twiddles<T, N>(Wl, a.W, tid, G * SZ::TPL);
-    __syncthreads();
+    lds_barrier();
     constexpr bool PF = true;
     auto load_field = [&](int f, cx<T>(&dst)[1][SZ::RMAX]) {
         const cx<T>* in = a.out + (size_t)f * a.ncell + colbase;
