@@ -712,7 +712,7 @@ __device__ __forceinline__ void wave_line_transform(cx<T>* line, cx<T> (&v)[SZ::
 // inputs and the four fields' stage-0 values with at most a couple of spilled values
 // (measured faster than two waves without spills or four with more spills).
 template <typename T, class SZ, int ROWS>
-__global__ __launch_bounds__(64 * kSrcWaves) __attribute__((amdgpu_waves_per_eu(3)))
+__global__ __launch_bounds__(64 * kSrcWaves) __attribute__((amdgpu_waves_per_eu(SZ::RMAX >= 11 ? 2 : 3)))
 void fast_source_wave(PassArgs<T> a) {
     static_assert(std::is_same<T, float>::value, "the fast source pass is FP32");
     constexpr int N = SZ::N, TPL = SZ::TPL, R0 = SZ::rad(0), NR0 = SZ::nr(0), RL = SZ::RL, NRL = SZ::NRL;
