@@ -624,8 +624,8 @@ int pass_candidates(const AxisFFT& ax, int d, const int64_t dims[3], int nin, si
             role = kRoleStrided1;
         if (role >= 0) {
             // source-group plans (mix_role >= 0) run the one-wave-per-row source pass first,
-            // unless its registers spill (more than 32 bytes a lane: lengths with a radix-11,
-            // 13 or 17 stage), in which case it follows the block-per-row kernels
+            // unless its registers spill more than 32 bytes a lane, in which case it follows
+            // the block-per-row kernels
             const bool wave = role == kRoleSource && mix_role >= 0;
             const int roles[3] = {wave ? kRoleSourceW : -1, role, wave ? kRoleSourceW : -1};
             for (int pos = 0; pos < 3; ++pos) {

@@ -710,7 +710,9 @@ __device__ __forceinline__ void wave_line_transform(cx<T>* line, cx<T> (&v)[SZ::
 
 // Three waves per SIMD: the register budget (168) that holds a row's eight cells of raw
 // inputs and the four fields' stage-0 values with at most a couple of spilled values
-// (measured faster than two waves without spills or four with more spills).
+// (measured faster than two waves without spills or four with more spills).  Lengths with
+// a radix-11, 13 or 17 stage, whose DFTs need more registers, run two waves per SIMD
+// instead of spilling.
 template <typename T, class SZ, int ROWS>
 __global__ __launch_bounds__(64 * kSrcWaves) __attribute__((amdgpu_waves_per_eu(SZ::RMAX >= 11 ? 2 : 3)))
 void fast_source_wave(PassArgs<T> a) {
