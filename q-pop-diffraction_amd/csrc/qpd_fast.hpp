@@ -404,6 +404,43 @@ __device__ __forceinline__ void transform(cx<T>* X, cx<T>* Y, cx<T> (&v)[F][SZ::
     }
 }
 
+// transform() for nlines contiguous lines whose stage-0 inputs are in X in natural order
+// (written before a barrier), one line per thread group: the stages before the last
+// run with their butterflies packed over the block's threads (thread t takes butterfly
+// t mod nr(s) of line t / nr(s)), so a stage with fewer butterflies than threads per line
+// (a large radix after a small one's worth of threads) keeps whole waves busy and leaves
+// the rest idle instead of running every wave with a few active lanes; the last stage
+// runs in the thread's own line (line, j) and leaves X[j + r NRL] in v.  Same buffers,
+// writes and barriers as transform<..., LOAD0 = true>.
+template <typename T, class SZ>
+__device__ __forceinline__ void transform_packed(cx<T>* X, cx<T>* Y, cx<T> (&v)[1][SZ::RMAX], int tid, int nlines,
+                                                 int j, int line, const cx<T>* __restrict__ W) {
+    constexpr int SL = SZ::NST - 1;
+    cx<T>* rd = X;
+    cx<T>* wr = Y;
+    auto packed_stage = [&](auto s_) {
+        constexpr int S = decltype(s_)::value;
+        constexpr int NR = SZ::nr(S);
+        const int ln = tid / NR, jj = tid - ln * NR;
+        if (ln < nlines) {
+            stage_read<T, SZ, S, 1, false, 0>(rd, v, jj, ln, 0);
+            stage_compute<T, SZ, S, 1>(v, jj, W);
+            stage_write<T, SZ, S, 1, false, 0>(wr, v, jj, ln, 0);
+        }
+        lds_barrier();
+        cx<T>* t = rd;
+        rd = wr;
+        wr = t;
+    };
+    if constexpr (SL > 0) packed_stage(std::integral_constant<int, 0>{});
+    if constexpr (SL > 1) packed_stage(std::integral_constant<int, 1>{});
+    if constexpr (SL > 2) packed_stage(std::integral_constant<int, 2>{});
+    if (j < SZ::nr(SL)) {
+        stage_read<T, SZ, SL, 1, false, 0>(rd, v, j, line, 0);
+        stage_compute<T, SZ, SL, 1>(v, j, W);
+    }
+}
+
 // Copy the twiddle table W[0, N) into LDS (caller synchronises before use).
 template <typename T, int N>
 __device__ __forceinline__ void load_twiddles(cx<T>* dst, const cx<T>* __restrict__ src, int tid, int nthr) {
@@ -553,7 +590,7 @@ __global__ __launch_bounds__(4 * G * SZ::TPL) void fast_source(PassArgs<T> a) {
             }
             lds_barrier();
             cx<T> v[1][SZ::RMAX];
-            transform<T, SZ, 1, false, 0, true>(X, Y, v, j, f * G + g, G, Wl);
+            transform_packed<T, SZ>(X, Y, v, tid, 4 * G, j, f * G + g, Wl);
             if (j < NRL) {
 #pragma unroll
                 for (int r = 0; r < RL; ++r) {
