@@ -653,6 +653,7 @@ int pass_candidates(const AxisFFT& ax, int d, const int64_t dims[3], int nin, si
                     const int64_t n2 = N / n1;
                     for (const FastEntry& ea : fast_registry()) {
                         if (ea.N != n2 || ea.role != kRoleSplit || (int)ea.lds > hard) continue;
+                        if (!lane_offsets_fit(ea.G, n1, row_stride, csize)) continue;
                         const int role_b = role == kRoleStrided     ? kRoleStridedB
                                            : role == kRoleIntensity ? kRoleIntensityB
                                            : role == kRoleMix1      ? kRoleMix1B

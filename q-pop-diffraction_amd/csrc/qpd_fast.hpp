@@ -1172,16 +1172,21 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_split(PassArgs<T> a) {
     const uint32_t q = outer_q - outer * a.nq;
     const uint32_t i0 = t0 * G + g;
     const bool col_ok = i0 < a.n0;
-    const uint32_t colbase = i0 + outer * a.OS + q * a.in_qmul * a.S;
     const uint32_t rstride = a.in_step * a.S;
+    // row addresses: wave-uniform pointer + 32-bit lane offset, as in fast_strided
+    static_assert(64 % G == 0, "a wave holds whole rows of the tile");
+    const uint32_t jw = (uint32_t)__builtin_amdgcn_readfirstlane(tid / 64) * (64 / G);
+    const uint32_t jl = (uint32_t)(tid % 64) / G;
+    const size_t tile0 = (size_t)t0 * G + (size_t)outer * a.OS + (size_t)q * a.in_qmul * a.S;
+    const uint32_t in_lane = ((col_ok ? i0 : a.n0 - 1) - t0 * G + jl * rstride) * (uint32_t)sizeof(cx<T>);
     load_twiddles<T, N>(Wl, a.W, tid, G * SZ::TPL);
     lds_barrier();
     constexpr bool PF = true;
     auto load_field = [&](int f, cx<T>(&dst)[1][SZ::RMAX]) {
-        const cx<T>* in = a.out + (size_t)f * a.ncell + colbase;
+        const cx<T>* in = a.out + (size_t)f * a.ncell + tile0;
         if (j < NR0) {
 #pragma unroll
-            for (int r = 0; r < R0; ++r) dst[0][r] = col_ok ? in[(uint32_t)(j + r * NR0) * rstride] : cx<T>{(T)0, (T)0};
+            for (int r = 0; r < R0; ++r) dst[0][r] = load_at(in + (size_t)(jw + r * NR0) * rstride, in_lane);
         }
     };
     cx<T> nxt[1][SZ::RMAX];
@@ -1199,11 +1204,14 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_split(PassArgs<T> a) {
         transform<T, SZ, 1, true, LP, false>(par ? bufB : bufA, par ? bufA : bufB, v, j, g, 0, Wl);
         if (ALT) par ^= 1;
         if (col_ok && j < SZ::NRL) {
-            cx<T>* o = a.out + (size_t)f * a.ncell + colbase;
+            cx<T>* o = a.out + (size_t)f * a.ncell + tile0;
+            const uint32_t out_lane = (g + jl * rstride) * (uint32_t)sizeof(cx<T>);
+            const uint32_t w_lane = q * jl * (uint32_t)sizeof(cx<T>);
 #pragma unroll
             for (int r = 0; r < SZ::RL; ++r) {
-                const uint32_t k2 = (uint32_t)(j + r * SZ::NRL);
-                o[k2 * rstride] = cmul(v[0][r], a.Wsplit[q * k2]);
+                // row k2 = j + r NRL: its twiddle W^{q k2} and its output row
+                const uint32_t k2u = jw + (uint32_t)(r * SZ::NRL);
+                store_at(o + (size_t)k2u * rstride, out_lane, cmul(v[0][r], load_at(a.Wsplit + q * k2u, w_lane)));
             }
         }
     }
