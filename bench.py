@@ -545,6 +545,8 @@ def run():
         w64 = workloads.two_phase(nx, ny, nz, precision=64)
         vol64 = Volume(w64.dims, w64.axis, w64.P, w64.U, w64.phase, precision=64)
         out64 = torch.empty(n, dtype=torch.float64, device="cuda")
+        if not args.no_tune:
+            engine.tune(vol64, cache=ScheduleCache(None), out=out64)
         log("timing FP64")
         sec64 = time_steps(lambda: engine.intensity(vol64, out64), max(3, args.steps // 4), 1)
         dl, relpk = max_dlog10(I32, out64)

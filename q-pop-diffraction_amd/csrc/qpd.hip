@@ -145,7 +145,7 @@ int round_up(int v, int m) { return (v + m - 1) / m * m; }
 
 // Pick (lines per field, inputs per fill, threads, EPT) for a pass over lines of length N.
 bool choose_launch(const AxisFFT& ax, int nin, bool contiguous, size_t csize, int64_t n_lines_total,
-                   int lds_soft, int lds_hard, int ept_max, Launch& L) {
+                   int lds_soft, int lds_hard, int ept_max, Launch& L, int tx_cap = 0) {
     const int N = ax.N;
     std::vector<int> nfs;
     for (int nf : {4, 3, 2, 1})
@@ -154,7 +154,7 @@ bool choose_launch(const AxisFFT& ax, int nin, bool contiguous, size_t csize, in
     if (contiguous) {
         for (int nl = 64; nl >= 1; nl /= 2) per.push_back(nl);
     } else {
-        const int tx_pref = csize == 8 ? 16 : 8;
+        const int tx_pref = tx_cap > 0 ? std::min(tx_cap, csize == 8 ? 16 : 8) : (csize == 8 ? 16 : 8);
         for (int tx = tx_pref; tx >= 1; tx /= 2) per.push_back(tx);
     }
     // contiguous passes prefer holding all inputs of a chunk at once; strided passes
@@ -189,7 +189,7 @@ bool choose_launch(const AxisFFT& ax, int nin, bool contiguous, size_t csize, in
                 L.nf = nf;
                 L.nthr = nthr;
                 const int need = (int)((elems + nthr - 1) / nthr);
-                L.ept = need <= 4 ? 4 : 8;
+                L.ept = need <= 2 ? 2 : (need <= 4 ? 4 : 8);
                 if (L.ept > ept_max) continue;
                 L.lds = lds;
                 return true;
@@ -581,20 +581,33 @@ hipError_t launch_pass(const Launch& L, const PassArgs<T>& a, bool source, bool 
     }
     if (L.contiguous) {
         if (source) {
+            // two elements a thread where they suffice: the FP64 source kernel holds each
+            // element's cell inputs and accumulators across the groups
             if (a.nout != 3 || L.ept > 4) return hipErrorInvalidValue;
-            return launch_one(pass_contig<T, 4, 3, true>, L, st, a);
+            return L.ept <= 2 ? launch_one(pass_contig<T, 2, 3, true>, L, st, a)
+                              : launch_one(pass_contig<T, 4, 3, true>, L, st, a);
         }
         if (a.nout != 1 || L.ept > 8) return hipErrorInvalidValue;
         return L.ept <= 4 ? launch_one(pass_contig<T, 4, 1, false>, L, st, a) : launch_one(pass_contig<T, 8, 1, false>, L, st, a);
     }
     if (L.ept > 8) return hipErrorInvalidValue;
+    // four elements a thread where they suffice: fewer registers (FP64 kernels spill)
+    const bool e4 = L.ept <= 4;
+    if (a.pair_first) {  // pass 1 over source groups: one or two z-classes
+        if (a.ngroups == 1)
+            return e4 ? launch_one(pass_strided_mix<T, 4, 1>, L, st, a) : launch_one(pass_strided_mix<T, 8, 1>, L, st, a);
+        if (a.ngroups == 2)
+            return e4 ? launch_one(pass_strided_mix<T, 4, 2>, L, st, a) : launch_one(pass_strided_mix<T, 8, 2>, L, st, a);
+        return hipErrorInvalidValue;
+    }
     if (intensity) {
         if (a.nout != 1) return hipErrorInvalidValue;
-        return launch_one(pass_strided<T, 8, 1, true>, L, st, a);
+        return e4 ? launch_one(pass_strided<T, 4, 1, true>, L, st, a) : launch_one(pass_strided<T, 8, 1, true>, L, st, a);
     }
-    if (a.nout == 2) return launch_one(pass_strided<T, 8, 2, false>, L, st, a);
+    if (a.nout == 2)
+        return e4 ? launch_one(pass_strided<T, 4, 2, false>, L, st, a) : launch_one(pass_strided<T, 8, 2, false>, L, st, a);
     if (a.nout != 1) return hipErrorInvalidValue;
-    return launch_one(pass_strided<T, 8, 1, false>, L, st, a);
+    return e4 ? launch_one(pass_strided<T, 4, 1, false>, L, st, a) : launch_one(pass_strided<T, 8, 1, false>, L, st, a);
 }
 
 // Schedules of pass d with nin inputs per chunk, the default first: the registered
@@ -684,10 +697,19 @@ int pass_candidates(const AxisFFT& ax, int d, const int64_t dims[3], int nin, si
     PassPlan gp;
     Launch& L = gp.L[0];
     const int ept_max = contig ? (nin > 1 ? 4 : 8) : 8;
-    // the generic kernels have no source-group form of pass 1
-    if (!(d == 1 && mix_role >= 0) && choose_launch(ax, nin, contig, csize, lines, soft, hard, ept_max, L)) {
+    if (choose_launch(ax, nin, contig, csize, lines, soft, hard, ept_max, L)) {
         L.blocks = contig ? (lines + L.nl - 1) / L.nl : (dims[0] + L.nl - 1) / L.nl * n_other;
         if (L.blocks <= 0x7fffffffLL) out.push_back(gp);
+        // strided passes: the half-width tile as a second generic schedule (fewer elements
+        // and registers a thread, narrower rows), for the autotuner to weigh
+        if (!contig && L.nl > 1) {
+            PassPlan gp2;
+            Launch& L2 = gp2.L[0];
+            if (choose_launch(ax, nin, contig, csize, lines, soft, hard, ept_max, L2, L.nl / 2)) {
+                L2.blocks = (dims[0] + L2.nl - 1) / L2.nl * n_other;
+                if (L2.blocks <= 0x7fffffffLL) out.push_back(gp2);
+            }
+        }
     }
     if (out.empty())
         return fail(QPD_ERR_UNSUPPORTED, "no launch shape fits dim " + std::to_string(d) + " of length " + std::to_string(N));
@@ -1019,13 +1041,12 @@ int qpd_plan_create(qpd_plan** plan, const qpd_grid* grid, const qpd_crystal* c,
     const int nin[3] = {4, 3, 2};
     classify(p, grid, c);
     // source groups: when they are fewer than the y-classes, pass 0 transforms one
-    // representative per group and pass 1 mixes them (FP32 compile-time kernels with one
-    // or two z-classes; QPD_CLASS_FORM=1 keeps the class form for comparison runs)
+    // representative per group and pass 1 mixes them (one or two z-classes; compile-time
+    // FP32 kernels or the run-time-radix pass_strided_mix; QPD_CLASS_FORM=1 keeps the class
+    // form for comparison runs)
     const int mix_role = p->nZ == 1 ? kRoleMix1 : (p->nZ == 2 ? kRoleMix2 : -1);
     const char* cf = std::getenv("QPD_CLASS_FORM");
-    p->grouped = (precision == 32 && fast_enabled() && mix_role >= 0 && p->nS < p->nY && !(cf && cf[0] == '1'))
-                     ? 1
-                     : 0;
+    p->grouped = (mix_role >= 0 && p->nS < p->nY && !(cf && cf[0] == '1')) ? 1 : 0;
     int rc = QPD_OK;
     for (int d = 0; d < 3 && rc == QPD_OK; ++d) {
         rc = pass_candidates(p->fft[d], d, p->dims, nin[d], p->csize, true, c->n_atom * c->n_phase * kAtomTab, true,

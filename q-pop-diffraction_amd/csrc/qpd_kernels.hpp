@@ -357,4 +357,100 @@ __global__ __launch_bounds__(512) void pass_strided(PassArgs<T> a) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Pass 1 over source groups for any length and precision (the run-time-radix form of
+// fast_strided_mix): the three inputs [A_s, B_s, C_s] of every source group s are
+// transformed in the tile, and every z-class z receives
+//   out_{z,0} = sum_s sum_{Y in z} c_Y(k1) X_{Y,s}(k0) [F1 A_s - i k1 F1 B_s],
+//   out_{z,1} = sum_s sum_{Y in z} c_Y(k1) X_{Y,s}(k0) F1 C_s,
+// the outputs of the class form's pass 1.
+template <typename T, int EPT, int NZ>
+__global__ __launch_bounds__(512) void pass_strided_mix(PassArgs<T> a) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T>* tile = reinterpret_cast<cx<T>*>(smem_raw);
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int N = a.N;
+    const int TX = 1 << a.log2TX;
+    const int stride = a.nf * TX;
+    const uint32_t t0 = blockIdx.x % a.ntiles0;
+    const uint32_t outer = blockIdx.x / a.ntiles0;
+    const int tx = tid & (TX - 1);
+    const int r0 = tid >> a.log2TX;
+    const int rstep = nthr >> a.log2TX;
+    const uint32_t i0 = t0 * (uint32_t)TX + (uint32_t)tx;
+    const bool col_ok = i0 < a.n0;
+    const uint32_t colbase = i0 + outer * a.OS;
+
+    cx<T> acc[EPT][NZ][2];
+#pragma unroll
+    for (int m = 0; m < EPT; ++m)
+#pragma unroll
+        for (int z = 0; z < NZ; ++z) acc[m][z][0] = acc[m][z][1] = {(T)0, (T)0};
+    for (int s = 0; s < a.nsrc; ++s) {
+        cx<T> tA[EPT], tC[EPT];
+#pragma unroll
+        for (int m = 0; m < EPT; ++m) tA[m] = tC[m] = {(T)0, (T)0};
+        for (int j0 = 0; j0 < 3; j0 += a.nf) {
+            const int nfc = min(a.nf, 3 - j0);
+            const cx<T>* inbase = a.in + ((size_t)s * 3 + j0) * a.ncell;
+#pragma unroll
+            for (int m = 0; m < EPT; ++m) {
+                const int idx = r0 + m * rstep;
+                if (idx >= N) continue;
+                const uint32_t off = colbase + (uint32_t)idx * a.S;
+                cx<T>* dst = tile + idx * stride + tx;
+                for (int f = 0; f < nfc; ++f)
+                    dst[f * TX] = col_ok ? inbase[(size_t)f * a.ncell + off] : cx<T>{(T)0, (T)0};
+            }
+            __syncthreads();
+            fft_tile(tile, a.nst, a.stages, a.W, stride, nfc * TX, lines_div(a, nfc), tid, nthr);
+#pragma unroll
+            for (int m = 0; m < EPT; ++m) {
+                const int idx = r0 + m * rstep;
+                if (idx >= N) continue;
+                const T kv = a.k[idx];
+                const cx<T>* srcp = tile + idx * stride + tx;
+                for (int f = 0; f < nfc; ++f) {
+                    const int jj = j0 + f;
+                    const cx<T> v = srcp[f * TX];
+                    if (jj == 0)
+                        tA[m] = tA[m] + v;
+                    else if (jj == 1)
+                        tA[m] = tA[m] + cscale(negi(v), kv);
+                    else
+                        tC[m] = v;
+                }
+            }
+            __syncthreads();
+        }
+        if (!col_ok) continue;  // no barrier follows within this group
+#pragma unroll
+        for (int z = 0; z < NZ; ++z) {
+            for (int p = a.pair_first[s * NZ + z]; p < a.pair_first[s * NZ + z + 1]; ++p) {
+                const int y = a.pair_y[p];
+                const cx<T> xc = a.mixc[(size_t)(y * a.nsrc + s) * a.n0 + i0];
+#pragma unroll
+                for (int m = 0; m < EPT; ++m) {
+                    const int idx = r0 + m * rstep;
+                    if (idx >= N) continue;
+                    const cx<T> cf = cmul(a.coef[(size_t)y * a.Nc + idx], xc);
+                    acc[m][z][0] = acc[m][z][0] + cmul(cf, tA[m]);
+                    acc[m][z][1] = acc[m][z][1] + cmul(cf, tC[m]);
+                }
+            }
+        }
+    }
+    if (!col_ok) return;
+#pragma unroll
+    for (int m = 0; m < EPT; ++m) {
+        const int idx = r0 + m * rstep;
+        if (idx >= N) continue;
+        const uint32_t off = colbase + (uint32_t)idx * a.S;
+#pragma unroll
+        for (int z = 0; z < NZ; ++z)
+#pragma unroll
+            for (int oq = 0; oq < 2; ++oq) a.out[(size_t)(2 * z + oq) * a.ncell + off] = acc[m][z][oq];
+    }
+}
+
 }  // namespace qpd

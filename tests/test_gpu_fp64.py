@@ -1,0 +1,66 @@
+"""FP64 plans: source groups and every launch schedule against the FP64 oracle.
+
+PbTiO3 has two source groups (Pb; Ti and the three O share su) and four y-classes, so FP64
+plans, like FP32 ones, transform the groups in pass 0 and mix them in pass 1 (the
+run-time-radix pass_strided_mix); QPD_CLASS_FORM=1 keeps the per-class form.  Every schedule
+of every pass, in both forms, must meet the reference's parity bound between its two
+implementations (rtol 1e-9 with an absolute floor of 1e-12 of the peak,
+tests/test_mgpu.py:51-53 of the reference) on voxels of the full volume.
+"""
+
+import numpy as np
+import pytest
+
+from qpop_diffraction import workloads
+from qpop_diffraction.batch import PlanCache, Reflection, resident_job
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def refl():
+    return Reflection((1, 0, 3), workloads.pto_atoms((1, 0, 3)))
+
+
+def run_fp64(refl, shape, schedules=None):
+    import torch
+    nx, ny, nz = shape
+    snap = workloads.twin_snapshot(("fp64",) + shape, nx, ny, nz, seed=5, precision=64)
+    plans = PlanCache(64)
+    plan = plans.get(snap, 0, refl)
+    for d, idx in (schedules or {}).items():
+        plan.set_schedule(d, idx)
+    out = torch.empty(nx * ny * nz, dtype=torch.float64, device="cuda")
+    work = torch.empty(plan.workspace_bytes, dtype=torch.uint8, device="cuda")
+    resident_job(plans, snap, 0, refl, out, work)
+    I = out.cpu().numpy().reshape(nz, ny, nx).transpose(2, 1, 0)
+    return I, snap, plan
+
+
+@pytest.mark.parametrize("shape", [(48, 40, 24), (104, 64, 30)])
+@pytest.mark.parametrize("form", ["groups", "class"])
+def test_fp64_every_schedule(refl, shape, form, monkeypatch):
+    from qpd_oracle import diffraction_fp64
+    if form == "class":
+        monkeypatch.setenv("QPD_CLASS_FORM", "1")
+    else:
+        monkeypatch.delenv("QPD_CLASS_FORM", raising=False)
+    I0, snap, plan = run_fp64(refl, shape)
+    ref = diffraction_fp64(snap.params, refl.atoms, *workloads.host_inputs(snap))[0]
+    atol = 1e-12 * ref.max()
+    np.testing.assert_allclose(I0, ref, rtol=1e-9, atol=atol)
+    names = [plan.schedules(d) for d in range(3)]
+    print(f"{shape} {form}: workspace {plan.workspace_bytes} B, schedules {names}", flush=True)
+    for d in range(3):
+        for idx in range(1, len(names[d])):
+            I, _, _ = run_fp64(refl, shape, {d: idx})
+            np.testing.assert_allclose(I, ref, rtol=1e-9, atol=atol, err_msg=f"pass {d}: {names[d][idx]}")
+
+
+def test_fp64_groups_use_less_workspace(refl, monkeypatch):
+    """Source groups: pass 0 writes 3 fields per group (2 groups) instead of 3 per y-class (4)."""
+    monkeypatch.delenv("QPD_CLASS_FORM", raising=False)
+    _, _, grouped = run_fp64(refl, (48, 40, 24))
+    monkeypatch.setenv("QPD_CLASS_FORM", "1")
+    _, _, classes = run_fp64(refl, (48, 40, 24))
+    assert grouped.workspace_bytes < classes.workspace_bytes
