@@ -64,3 +64,12 @@ def test_fp64_groups_use_less_workspace(refl, monkeypatch):
     monkeypatch.setenv("QPD_CLASS_FORM", "1")
     _, _, classes = run_fp64(refl, (48, 40, 24))
     assert grouped.workspace_bytes < classes.workspace_bytes
+
+
+@pytest.mark.parametrize("shape", [(64, 48, 1), (1, 32, 20)])
+def test_fp64_thin_volumes(refl, shape):
+    """FP64 volumes one cell thick along an axis, at the reference's parity bound."""
+    from qpd_oracle import diffraction_fp64
+    I, snap, _ = run_fp64(refl, shape)
+    ref = diffraction_fp64(snap.params, refl.atoms, *workloads.host_inputs(snap))[0]
+    np.testing.assert_allclose(I, ref, rtol=1e-9, atol=1e-12 * ref.max())

@@ -228,3 +228,12 @@ def test_source_schedules_with_weights(case):
         err = rel_to_peak(I, ref)
         print(f"{case} pass 0 schedule {name}: max|dI|/peak {err:.2e}", flush=True)
         assert err <= FP32_PEAK_TOL and np.argmax(I) == np.argmax(ref), (name, err)
+
+
+@pytest.mark.parametrize("shape", [(64, 48, 1), (40, 1, 24), (1, 32, 20), (1, 1, 16)])
+def test_thin_volumes(refl, shape):
+    """Volumes one cell thick along an axis (the reference's 2-D runs, e.g. its 400x400x1
+    polar-vortex example): a length-1 transform is the identity, FP32 against the oracle."""
+    err = volume_check(refl, *shape, seed=3)
+    print(f"{shape}: {err:.2e}", flush=True)
+    assert err <= FP32_PEAK_TOL, (shape, err)
