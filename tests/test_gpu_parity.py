@@ -65,6 +65,31 @@ def test_example_two_variant_twin(tmp_path, precision):
         check_fp32(I, ref)
 
 
+REFERENCE_EXAMPLES = ["1_two_phase_mixture/anisotropic", "1_two_phase_mixture/isotropic",
+                      "3_three_variant_hexagonal_twin", "4_polar_vortex_PbTiO3/ordered",
+                      "4_polar_vortex_PbTiO3/disordered"]
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+@pytest.mark.parametrize("name", REFERENCE_EXAMPLES)
+def test_reference_examples(tmp_path, name, precision):
+    """The reference's other examples (each of which its tests/test_examples.py:34-37 runs),
+    from their own input files: 800 x 800 x 1 two-phase mixtures with phase-fraction and
+    region files, a 1024 x 1024 x 1 three-variant hexagonal twin with displacements, and
+    1 x 400 x 400 polar vortices.  The CLI against the oracle: FP64 at the reference's
+    parity bound, FP32 at the bulk bound."""
+    print(f"example {name} precision {precision}", flush=True)
+    path = copy_example(name, tmp_path / "ex")
+    I, q = run_cli(path, precision=precision)
+    ref, qref = oracle_run(path)
+    if precision == 64:
+        np.testing.assert_allclose(I, ref, rtol=1e-9, atol=1e-12 * ref.max())
+        for a, b in zip(q, qref):
+            np.testing.assert_allclose(a, b, rtol=1e-14)
+    else:
+        check_fp32(I, ref)
+
+
 @pytest.mark.parametrize("precision", [64, 32])
 def test_dropin_diffraction_layout_matches_cli(tmp_path, precision):
     """diffraction() (z-fastest arrays) and the CLI (x-fastest files) give the same intensity."""
