@@ -5,7 +5,8 @@ restated from the kinematic theory and the FerroDyn setting rules.  The PbTiO3 c
 planted golden cases (complex form factors, anisotropic P -> shift coefficients) on a
 40 x 30 x 20 grid of 1 nm cells: with |G_200| / 2 pi = 5 / nm and e = 0.02, e L |G| / 2 pi is
 an integer on every axis, so a uniform strain u = e r is periodic in exp(-i G.u).
-  * a uniform P puts the peak at G; u_z = e z moves the 002 peak to G (1 - e) (u in metres:
+  * a uniform P along each of the six variants puts the peak at G with I(G) = |F(G)|^2 / V^2,
+    F(G) = sum_a f_a exp(-i G.(R_a + d_a)) computed here; u_z = e z moves the 002 peak to G (1 - e) (u in metres:
     a missing m -> nm conversion would leave it at G); a spontaneous strain e along an axis
     moves the peak to G / (1 + e) along it (first order in u: G (1 - e));
   * 180-degree P_z stripes of period L along x give satellites at +-2 pi / L on q_x only;
@@ -77,6 +78,28 @@ def compute(path, how):
 def floor(how):
     """Relative level below which a voxel counts as empty: FP64 round-off or the FP32 bulk bound."""
     return 1e-5 if how == 32 else 1e-12
+
+
+def structure_factor(q, P):
+    """sum_a f_a exp(-i q.(R_a + d_a)) of a uniform domain, positions in nm, P in C/m^2."""
+    r = np.array(golden_cases.PLANT_POS) * A + golden_cases.PLANT_SHIFT * np.asarray(P) / 1e-9
+    return np.sum(np.array(golden_cases.PLANT_FORM) * np.exp(-1j * (r @ q)))
+
+
+@pytest.mark.parametrize("how", RUNNERS)
+@pytest.mark.parametrize("hkl", [(0, 0, 2), (2, 0, 0), (0, 2, 0), (1, 1, 1)], ids=lambda h: "".join(map(str, h)))
+@pytest.mark.parametrize("variant", list(VARIANTS))
+def test_structure_factor(tmp_path, variant, hkl, how):
+    """I(G) = |sum_a f_a exp(-i G.(R_a + d_a))|^2 / V^2 for a uniform domain, peak at G."""
+    P = 0.75 * np.array(VARIANTS[variant], dtype=float)
+    Pf = zeros()
+    for d in range(3):
+        Pf[d] = P[d]
+    write(tmp_path, Pf, zeros(), hkl=hkl)
+    I, q = compute(tmp_path, how)
+    G = reciprocal(hkl)
+    np.testing.assert_allclose(peak(I, q), G, atol=1e-9)
+    np.testing.assert_allclose(at(I, q, G), abs(structure_factor(G, P)) ** 2 / A ** 6, rtol=2e-5 if how == 32 else 1e-9)
 
 
 @pytest.mark.parametrize("how", RUNNERS)
