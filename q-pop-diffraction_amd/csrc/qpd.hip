@@ -613,8 +613,8 @@ hipError_t launch_pass(const Launch& L, const PassArgs<T>& a, bool source, bool 
 // Schedules of pass d with nin inputs per chunk, the default first: the registered
 // compile-time kernels of the length and role (FP32), then, for strided lengths, every
 // split N1 x N2 into registered step lengths (most even first; only where allow_split),
-// then the generic kernel.  A split is the default only where the length has no kernel
-// of its own.
+// then the generic kernel (for strided passes at two tile widths).  A split is the default
+// only where the length has no kernel of its own.
 int pass_candidates(const AxisFFT& ax, int d, const int64_t dims[3], int nin, size_t csize, bool intensity_last,
                     int atomtab_entries, bool allow_split, std::vector<PassPlan>& out, int mix_role = -1) {
     out.clear();
