@@ -1,14 +1,17 @@
 """libqpd.so loads and exports every entry point include/qpd.h declares (CPU, no compute)."""
 
 import ctypes
+import os
 import re
 from pathlib import Path
 
+import numpy as np
 import pytest
 
 from qpop_diffraction import _native
 
-HEADER = Path(__file__).resolve().parents[1] / "include" / "qpd.h"
+ROOT = Path(__file__).resolve().parents[1]
+HEADER = ROOT / "include" / "qpd.h"
 
 
 def declared():
@@ -77,3 +80,39 @@ def test_ingest_rejects_bad_arguments():
     arg.U = (ctypes.c_void_p * 3)(0x1000, None, None)
     assert lib.qpd_ingest_fields(ctypes.byref(arg), 32, fake, fake, fake, None) == -1
     assert b"three components" in lib.qpd_last_error()
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    """Without libqpd.so the package raises (with the build command); there is no CPU fallback."""
+    import subprocess
+    import sys
+    code = (
+        "import sys\n"
+        f"sys.path.insert(0, {str(ROOT / 'q-pop-diffraction_amd')!r})\n"
+        "from qpop_diffraction._native import load_library\n"
+        "try:\n"
+        "    load_library()\n"
+        "except ImportError as e:\n"
+        "    print('raised:', e)\n"
+        "    sys.exit(0)\n"
+        "sys.exit(1)\n"
+    )
+    env = dict(os.environ, QPD_LIB=str(tmp_path / "absent" / "libqpd.so"))
+    r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "build" in r.stdout
+
+
+def test_no_gpu_fails_loudly(tmp_path):
+    """On a machine without a GPU the drop-in diffraction() raises instead of computing elsewhere."""
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    import golden_cases
+    from helpers import oracle_inputs
+    from qpop_diffraction import diffraction
+    from qpop_diffraction.h5lite import open_file
+    golden_cases.case_bulk(open_file, str(tmp_path))
+    params, atoms, oPhase, oStruc, u, region, _ = oracle_inputs(tmp_path)
+    with pytest.raises(RuntimeError, match="ROCm GPU"):
+        diffraction(params, atoms, oPhase, oStruc, u, region)
