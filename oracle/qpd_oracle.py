@@ -14,13 +14,16 @@ k-space phase and first-order term (:107-109), and I = fftshift(max(|S amp|^2,
 1e-50)) (:111).  ``dft_direct`` is a brute-force O(N^2)-per-axis DFT used to pin
 numpy.fft on tiny sizes.
 
-How each function is pinned (tests/test_oracle.py):
+How each function is pinned (tests/test_oracle.py, tests/test_planted_forms.py,
+tests/test_a1a2_domains.py):
   * reciprocal_centre, q_axes: closed forms and the reference's known answers
     (tests/test_planted.py:141-194 of the reference);
-  * diffraction_fp64: the analytic structure-factor, strain, satellite and
-    thickness-fringe answers of tests/test_single_domain.py and
-    tests/test_a1a2_domains.py of the reference, linearity in the weights, and
-    golden vectors written by the reference's own NumPy path
+  * diffraction_fp64: the analytic answers of the reference's tests/test_single_domain.py,
+    tests/test_planted.py and tests/test_a1a2_domains.py restated from the kinematic theory
+    (single-domain structure factors for six variants at four reflections, cubic variant
+    symmetry, peak shifts under a u_z ramp, a spontaneous strain and a film lattice,
+    stripe satellites on one axis, thickness fringes and domain satellites of a1/a2 films,
+    weight scaling), and golden vectors written by the reference's own NumPy path
     (tests/golden/make_golden.py);
   * dft_direct: the DFT definition, checked against numpy.fft;
   * rsm_map: the reciprocal-space map of src/python/rsm.py:66-113 (bin edges from the
