@@ -138,12 +138,22 @@ class Slot:
         items = self.buffers.items() if grid is None else [(grid, self.buffers[grid])] if grid in self.buffers else []
         return sum(w.numel() + o.numel() * 8 for _, (w, o, h) in items)
 
+    def extra_bytes(self, grid, work_bytes, n):
+        """Device bytes that ensure(grid, work_bytes, n) would add."""
+        if grid not in self.buffers:
+            return max(1, work_bytes) + 8 * n
+        return max(0, work_bytes - self.buffers[grid][0].numel())
+
     def ensure(self, grid, work_bytes, n):
+        """The grid's buffers, the workspace grown if this plan needs more than the cached one."""
         import torch
         if grid not in self.buffers:
             self.buffers[grid] = (torch.empty(max(1, work_bytes), dtype=torch.uint8, device="cuda"),
                                   torch.empty(n, dtype=torch.float64, device="cuda"),
                                   torch.empty(n, dtype=torch.float64, pin_memory=True))
+        elif self.buffers[grid][0].numel() < work_bytes:
+            _, out, host = self.buffers[grid]
+            self.buffers[grid] = (torch.empty(work_bytes, dtype=torch.uint8, device="cuda"), out, host)
         return self.buffers[grid]
 
     def trim(self, keep=None):
@@ -182,10 +192,7 @@ class Server:
         return self.plans.get(self.store.spec(req.key), req.reflection, self.reflections[req.reflection])
 
     def _need(self, req, slot, plan):
-        need = self.store.footprint(req.key)
-        if req.grid not in slot.buffers:
-            need += plan.workspace_bytes + 8 * plan.n
-        return need
+        return self.store.footprint(req.key) + slot.extra_bytes(req.grid, plan.workspace_bytes, plan.n)
 
     def _fits(self, req, slot, plan):
         """Whether the job fits the budget in this slot, freeing idle buffers and unused snapshots if needed."""

@@ -147,14 +147,41 @@ def test_served_same_alone_and_in_mix():
 
 
 def test_served_under_tight_budget():
+    """A budget that holds the largest job but not every snapshot the requests read plus the
+    smallest job's buffers: whatever order the slots finish in, the store must evict."""
     reqs = poisson_requests(16, rate=0.0, grids=SERVE_GRIDS, n_snapshots=6, n_reflections=4, seed=11)
-    big = workloads.snapshot_spec(*SERVE_GRIDS[-1]).nbytes
-    n = int(np.prod(SERVE_GRIDS[-1]))
-    budget = 2 * big + 2 * (128 * n + 8 * n)  # two big snapshots and two slots' buffers
-    print(f"serve: tight budget {budget / 1e6:.1f} MB", flush=True)
+    refl = [Reflection(h, workloads.pto_atoms(h)) for h in workloads.REFLECTIONS]
+    plans = PlanCache(32)
+    snap_bytes = lambda grid: workloads.snapshot_spec(*grid).nbytes
+
+    def buffer_bytes(grid):  # the largest workspace of any request on the grid, and the output
+        spec = workloads.snapshot_spec(*grid)
+        work = max(plans.get(spec, r.reflection, refl[r.reflection]).workspace_bytes for r in reqs if r.grid == grid)
+        return work + 8 * int(np.prod(grid))
+
+    grids = {r.grid for r in reqs}
+    budget = int(1.02 * max(snap_bytes(g) + buffer_bytes(g) for g in grids))
+    all_snaps = sum(snap_bytes(k[0]) for k in {r.key for r in reqs})
+    assert all_snaps + min(buffer_bytes(g) for g in grids) > budget, "the case would not force an eviction"
+    print(f"serve: tight budget {budget / 1e6:.1f} MB, snapshots {all_snaps / 1e6:.1f} MB", flush=True)
     tight, done, srv = serve(reqs, n_slots=3, budget=budget, realtime=False)
+    assert sorted(tight) == list(range(16))
     assert srv.peak_bytes <= budget
     assert srv.store.evictions > 0
     roomy, _, _ = serve(reqs, n_slots=3, budget=200e9, realtime=False)
     for r in reqs:
         assert np.array_equal(tight[r.rid].numpy(), roomy[r.rid].numpy()), f"request {r.rid}"
+
+
+def test_slot_workspace_grows():
+    """A plan needing more workspace than the grid's cached buffer gets a larger one; the output
+    buffers are kept."""
+    from qpop_diffraction.serve import Slot
+    slot = Slot(0)
+    g = (8, 8, 8)
+    work, out, host = slot.ensure(g, 1000, 512)
+    assert slot.extra_bytes(g, 800, 512) == 0 and slot.extra_bytes(g, 1500, 512) == 500
+    assert slot.extra_bytes((4, 4, 4), 100, 64) == 100 + 8 * 64
+    work2, out2, host2 = slot.ensure(g, 1500, 512)
+    assert work2.numel() == 1500 and out2 is out and host2 is host
+    assert slot.ensure(g, 800, 512)[0] is work2
