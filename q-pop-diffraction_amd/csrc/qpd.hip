@@ -131,10 +131,12 @@ int max_lds_bytes() {
 }
 
 
-bool stage_fits(const AxisFFT& ax, int lines, int nthr) {
+// Whether every stage's butterflies over `lines` lines fit nthr threads at the butterflies
+// a thread holds (fft_stage: max_regs of the real type over R).
+bool stage_fits(const AxisFFT& ax, int lines, int nthr, size_t csize) {
     for (int s = 0; s < ax.nst; ++s) {
         const int R = ax.st[s].R;
-        const int mb = std::max(1, kMaxRegs / R);
+        const int mb = std::max(1, max_regs(csize / 2) / R);
         const int64_t total = (int64_t)(ax.N / R) * lines;
         if ((total + nthr - 1) / nthr > mb) return false;
     }
@@ -143,7 +145,8 @@ bool stage_fits(const AxisFFT& ax, int lines, int nthr) {
 
 int round_up(int v, int m) { return (v + m - 1) / m * m; }
 
-// Pick (lines per field, inputs per fill, threads, EPT) for a pass over lines of length N.
+// Pick (lines per field, inputs per fill, threads, EPT) for a pass over lines of length N;
+// tx_cap > 0 caps the lines per field (rows of a contiguous pass, columns of a strided one).
 bool choose_launch(const AxisFFT& ax, int nin, bool contiguous, size_t csize, int64_t n_lines_total,
                    int lds_soft, int lds_hard, int ept_max, Launch& L, int tx_cap = 0) {
     const int N = ax.N;
@@ -152,7 +155,8 @@ bool choose_launch(const AxisFFT& ax, int nin, bool contiguous, size_t csize, in
         if (nf <= nin) nfs.push_back(nf);
     std::vector<int> per;  // lines per field candidates
     if (contiguous) {
-        for (int nl = 64; nl >= 1; nl /= 2) per.push_back(nl);
+        for (int nl = 64; nl >= 1; nl /= 2)
+            if (tx_cap <= 0 || nl <= tx_cap) per.push_back(nl);
     } else {
         const int tx_pref = tx_cap > 0 ? std::min(tx_cap, csize == 8 ? 16 : 8) : (csize == 8 ? 16 : 8);
         for (int tx = tx_pref; tx >= 1; tx /= 2) per.push_back(tx);
@@ -181,8 +185,8 @@ bool choose_launch(const AxisFFT& ax, int nin, bool contiguous, size_t csize, in
                 nthr = std::max(64, std::min(512, nthr));
                 while (nthr < 512 && (elems + nthr - 1) / nthr > ept_max) nthr += 64;
                 if ((elems + nthr - 1) / nthr > ept_max) continue;
-                while (nthr < 512 && !stage_fits(ax, nf * p, nthr)) nthr += 64;
-                if (!stage_fits(ax, nf * p, nthr)) continue;
+                while (nthr < 512 && !stage_fits(ax, nf * p, nthr, csize)) nthr += 64;
+                if (!stage_fits(ax, nf * p, nthr, csize)) continue;
                 L.fe = nullptr;
                 L.contiguous = contiguous;
                 L.nl = p;
@@ -191,6 +195,8 @@ bool choose_launch(const AxisFFT& ax, int nin, bool contiguous, size_t csize, in
                 const int need = (int)((elems + nthr - 1) / nthr);
                 L.ept = need <= 2 ? 2 : (need <= 4 ? 4 : 8);
                 if (L.ept > ept_max) continue;
+                L.pow2 = csize == 16;
+                for (int s = 0; s < ax.nst; ++s) L.pow2 = L.pow2 && (ax.st[s].R == 2 || ax.st[s].R == 4 || ax.st[s].R == 8);
                 L.lds = lds;
                 return true;
             }
@@ -573,41 +579,56 @@ void set_fast(Launch& L, const FastEntry* fe, size_t lds) {
 // Kernel instance for a pass: contiguous passes compute the source (3 outputs) or
 // transform fields (1 output); strided passes have 2 outputs, or 1 when they
 // write the intensity or run a plain transform.
-template <typename T>
-hipError_t launch_pass(const Launch& L, const PassArgs<T>& a, bool source, bool intensity, hipStream_t st) {
-    if (L.fe) {
-        if constexpr (std::is_same<T, float>::value) return L.fe->launch(L, a, st);
-        return hipErrorInvalidValue;
-    }
+// The generic kernels of one pass; POW2 instantiations only for FP64 (their FP32 forms
+// do not spill).
+template <typename T, bool POW2>
+hipError_t launch_generic(const Launch& L, const PassArgs<T>& a, bool source, bool intensity, hipStream_t st) {
     if (L.contiguous) {
         if (source) {
             // two elements a thread where they suffice: the FP64 source kernel holds each
             // element's cell inputs and accumulators across the groups
             if (a.nout != 3 || L.ept > 4) return hipErrorInvalidValue;
-            return L.ept <= 2 ? launch_one(pass_contig<T, 2, 3, true>, L, st, a)
-                              : launch_one(pass_contig<T, 4, 3, true>, L, st, a);
+            return L.ept <= 2 ? launch_one(pass_contig<T, 2, 3, true, POW2>, L, st, a)
+                              : launch_one(pass_contig<T, 4, 3, true, POW2>, L, st, a);
         }
         if (a.nout != 1 || L.ept > 8) return hipErrorInvalidValue;
-        return L.ept <= 4 ? launch_one(pass_contig<T, 4, 1, false>, L, st, a) : launch_one(pass_contig<T, 8, 1, false>, L, st, a);
+        return L.ept <= 4 ? launch_one(pass_contig<T, 4, 1, false, POW2>, L, st, a)
+                          : launch_one(pass_contig<T, 8, 1, false, POW2>, L, st, a);
     }
     if (L.ept > 8) return hipErrorInvalidValue;
     // four elements a thread where they suffice: fewer registers (FP64 kernels spill)
     const bool e4 = L.ept <= 4;
     if (a.pair_first) {  // pass 1 over source groups: one or two z-classes
         if (a.ngroups == 1)
-            return e4 ? launch_one(pass_strided_mix<T, 4, 1>, L, st, a) : launch_one(pass_strided_mix<T, 8, 1>, L, st, a);
+            return e4 ? launch_one(pass_strided_mix<T, 4, 1, POW2>, L, st, a)
+                      : launch_one(pass_strided_mix<T, 8, 1, POW2>, L, st, a);
         if (a.ngroups == 2)
-            return e4 ? launch_one(pass_strided_mix<T, 4, 2>, L, st, a) : launch_one(pass_strided_mix<T, 8, 2>, L, st, a);
+            return e4 ? launch_one(pass_strided_mix<T, 4, 2, POW2>, L, st, a)
+                      : launch_one(pass_strided_mix<T, 8, 2, POW2>, L, st, a);
         return hipErrorInvalidValue;
     }
     if (intensity) {
         if (a.nout != 1) return hipErrorInvalidValue;
-        return e4 ? launch_one(pass_strided<T, 4, 1, true>, L, st, a) : launch_one(pass_strided<T, 8, 1, true>, L, st, a);
+        return e4 ? launch_one(pass_strided<T, 4, 1, true, POW2>, L, st, a)
+                  : launch_one(pass_strided<T, 8, 1, true, POW2>, L, st, a);
     }
     if (a.nout == 2)
-        return e4 ? launch_one(pass_strided<T, 4, 2, false>, L, st, a) : launch_one(pass_strided<T, 8, 2, false>, L, st, a);
+        return e4 ? launch_one(pass_strided<T, 4, 2, false, POW2>, L, st, a)
+                  : launch_one(pass_strided<T, 8, 2, false, POW2>, L, st, a);
     if (a.nout != 1) return hipErrorInvalidValue;
-    return e4 ? launch_one(pass_strided<T, 4, 1, false>, L, st, a) : launch_one(pass_strided<T, 8, 1, false>, L, st, a);
+    return e4 ? launch_one(pass_strided<T, 4, 1, false, POW2>, L, st, a)
+              : launch_one(pass_strided<T, 8, 1, false, POW2>, L, st, a);
+}
+
+template <typename T>
+hipError_t launch_pass(const Launch& L, const PassArgs<T>& a, bool source, bool intensity, hipStream_t st) {
+    if (L.fe) {
+        if constexpr (std::is_same<T, float>::value) return L.fe->launch(L, a, st);
+        return hipErrorInvalidValue;
+    }
+    if constexpr (std::is_same<T, double>::value)
+        if (L.pow2) return launch_generic<T, true>(L, a, source, intensity, st);
+    return launch_generic<T, false>(L, a, source, intensity, st);
 }
 
 // Schedules of pass d with nin inputs per chunk, the default first: the registered
@@ -700,14 +721,18 @@ int pass_candidates(const AxisFFT& ax, int d, const int64_t dims[3], int nin, si
     if (choose_launch(ax, nin, contig, csize, lines, soft, hard, ept_max, L)) {
         L.blocks = contig ? (lines + L.nl - 1) / L.nl : (dims[0] + L.nl - 1) / L.nl * n_other;
         if (L.blocks <= 0x7fffffffLL) out.push_back(gp);
-        // strided passes: the half-width tile as a second generic schedule (fewer elements
-        // and registers a thread, narrower rows), for the autotuner to weigh
-        if (!contig && L.nl > 1) {
+        // the half-width tile as a second generic schedule, for the autotuner to weigh:
+        // strided passes get fewer elements and registers a thread and narrower rows,
+        // contiguous ones half the rows (half the LDS and threads: more blocks per CU)
+        if (L.nl > 1) {
             PassPlan gp2;
             Launch& L2 = gp2.L[0];
             if (choose_launch(ax, nin, contig, csize, lines, soft, hard, ept_max, L2, L.nl / 2)) {
-                L2.blocks = (dims[0] + L2.nl - 1) / L2.nl * n_other;
-                if (L2.blocks <= 0x7fffffffLL) out.push_back(gp2);
+                L2.blocks = contig ? (lines + L2.nl - 1) / L2.nl : (dims[0] + L2.nl - 1) / L2.nl * n_other;
+                // FP64 source passes default to the half tile: at about 165 VGPRs a 512-thread
+                // block of 64 KB runs alone on its CU (3.55 against 2.60 ms at 512x512x104)
+                const bool first = contig && csize == 16 && atomtab_entries > 0;
+                if (L2.blocks <= 0x7fffffffLL) out.insert(first ? out.end() - 1 : out.end(), gp2);
             }
         }
     }
@@ -921,8 +946,8 @@ std::string describe_launch(const Launch& L) {
         std::snprintf(buf, sizeof(buf), "fast %d (%s) G%d %s%s", L.fe->N, L.fe->radices, L.fe->G,
                       role_name[L.fe->role], L.fe->no_prefetch ? " np" : "");
     else
-        std::snprintf(buf, sizeof(buf), "generic %s nl%d nf%d nthr%d ept%d", L.contiguous ? "contiguous" : "strided",
-                      L.nl, L.nf, L.nthr, L.ept);
+        std::snprintf(buf, sizeof(buf), "generic %s nl%d nf%d nthr%d ept%d%s", L.contiguous ? "contiguous" : "strided",
+                      L.nl, L.nf, L.nthr, L.ept, L.pow2 ? " pow2" : "");
     return buf;
 }
 

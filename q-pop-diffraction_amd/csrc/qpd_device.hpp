@@ -107,7 +107,10 @@ __device__ __forceinline__ void phase_sincos(double t, double& s, double& c) { s
 // table W[m] = exp(-2 pi i m / N).  The stage list lives in device memory and
 // is read with uniform (scalar) loads.
 constexpr int kMaxStages = 16;
-constexpr int kMaxRegs = 16;  // complex values a thread holds during one stage
+// Complex values a thread holds during one stage: 16 in FP32, 8 in FP64 (the same 64
+// VGPRs; FP64 kernels spill to scratch with 16).
+constexpr int kMaxRegs = 16;
+constexpr int max_regs(size_t real_bytes) { return real_bytes == 4 ? kMaxRegs : kMaxRegs / 2; }
 
 struct Stage {
     int R;
@@ -295,7 +298,7 @@ template <int R, typename T>
 __device__ __forceinline__ void fft_stage(cx<T>* __restrict__ tile, const cx<T>* __restrict__ W, int NR,
                                           int stride, int Lc, FastDiv divLc, int Ns, FastDiv divNs, int tws,
                                           int tid, int nthr) {
-    constexpr int MB = (kMaxRegs / R) > 0 ? (kMaxRegs / R) : 1;
+    constexpr int MB = (max_regs(sizeof(T)) / R) > 0 ? (max_regs(sizeof(T)) / R) : 1;
     const int total = NR * Lc;
     T cr[R / 2 + 1], sr[R / 2 + 1];
     if constexpr ((R & 1) == 1) {
@@ -341,7 +344,10 @@ __device__ __forceinline__ void fft_stage(cx<T>* __restrict__ tile, const cx<T>*
 }
 
 // Forward transform of lines [0, Lc) of an LDS tile (see fft_stage), natural order in and out.
-template <typename T>
+// POW2: every stage has radix 2, 4 or 8, and the odd-radix stages are not compiled in (a
+// kernel's registers are those of its largest stage: the FP64 radix-17 stage alone holds
+// about 200 VGPRs).
+template <typename T, bool POW2 = false>
 __device__ __forceinline__ void fft_tile(cx<T>* tile, int nst, const Stage* __restrict__ stages,
                                          const cx<T>* __restrict__ W, int stride, int Lc, FastDiv divLc, int tid,
                                          int nthr) {
@@ -349,17 +355,26 @@ __device__ __forceinline__ void fft_tile(cx<T>* tile, int nst, const Stage* __re
         const Stage sd = stages[s];
         const FastDiv dn = {(uint32_t)sd.Ns, sd.ns_m};
         const int Ns = sd.Ns, tws = sd.tws, NR = sd.NR;
-        switch (sd.R) {
-            case 2: fft_stage<2, T>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
-            case 3: fft_stage<3, T>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
-            case 4: fft_stage<4, T>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
-            case 5: fft_stage<5, T>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
-            case 7: fft_stage<7, T>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
-            case 8: fft_stage<8, T>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
-            case 11: fft_stage<11, T>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
-            case 13: fft_stage<13, T>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
-            case 17: fft_stage<17, T>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
-            default: break;  // the host accepts no other radix
+        if constexpr (POW2) {
+            switch (sd.R) {
+                case 2: fft_stage<2, T>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
+                case 4: fft_stage<4, T>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
+                case 8: fft_stage<8, T>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
+                default: break;  // the host sets POW2 only for plans of these radices
+            }
+        } else {
+            switch (sd.R) {
+                case 2: fft_stage<2, T>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
+                case 3: fft_stage<3, T>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
+                case 4: fft_stage<4, T>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
+                case 5: fft_stage<5, T>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
+                case 7: fft_stage<7, T>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
+                case 8: fft_stage<8, T>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
+                case 11: fft_stage<11, T>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
+                case 13: fft_stage<13, T>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
+                case 17: fft_stage<17, T>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
+                default: break;  // the host accepts no other radix
+            }
         }
     }
 }

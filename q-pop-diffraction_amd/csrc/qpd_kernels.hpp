@@ -165,10 +165,11 @@ __device__ __forceinline__ cx<T> chunk_coef(const PassArgs<T>& a, int ct, int id
 }
 
 // ---------------------------------------------------------------------------
-// Pass along the contiguous dim: block = nl consecutive rows of n0 points.
+// Pass along the contiguous dim: block = nl consecutive rows of n0 points.  POW2: the
+// axis has only radix-2/4/8 stages (fft_tile).
 // LDS tile: point idx of line (f, row) at tile[idx * (nf*nl) + f*nl + row].
 // Element e = tid + m*nthr of a field's tile is (row, idx) = divmod(e, N).
-template <typename T, int EPT, int NOUT, bool SOURCE>
+template <typename T, int EPT, int NOUT, bool SOURCE, bool POW2 = false>
 __global__ __launch_bounds__(512) void pass_contig(PassArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* tile = reinterpret_cast<cx<T>*>(smem_raw);
@@ -240,7 +241,7 @@ __global__ __launch_bounds__(512) void pass_contig(PassArgs<T> a) {
                     }
                 }
                 __syncthreads();
-                fft_tile(tile, a.nst, a.stages, a.W, stride, nfc * nl, lines_div(a, nfc), tid, nthr);
+                fft_tile<T, POW2>(tile, a.nst, a.stages, a.W, stride, nfc * nl, lines_div(a, nfc), tid, nthr);
 #pragma unroll
                 for (int m = 0; m < EPT; ++m) {
                     const int e = tid + m * nthr;
@@ -273,7 +274,7 @@ __global__ __launch_bounds__(512) void pass_contig(PassArgs<T> a) {
 // LDS tile: point idx of line (f, tx) at tile[idx * (nf*TX) + f*TX + tx].  A thread
 // keeps one column tx = tid % TX and the points idx = tid / TX + m * (nthr / TX).
 // INTENSITY: the single output is the amplitude; write fftshifted max(|A|^2, 1e-50).
-template <typename T, int EPT, int NOUT, bool INTENSITY>
+template <typename T, int EPT, int NOUT, bool INTENSITY, bool POW2 = false>
 __global__ __launch_bounds__(512) void pass_strided(PassArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* tile = reinterpret_cast<cx<T>*>(smem_raw);
@@ -313,7 +314,7 @@ __global__ __launch_bounds__(512) void pass_strided(PassArgs<T> a) {
                         dst[f * TX] = col_ok ? inbase[(size_t)f * a.ncell + off] : cx<T>{(T)0, (T)0};
                 }
                 __syncthreads();
-                fft_tile(tile, a.nst, a.stages, a.W, stride, nfc * TX, lines_div(a, nfc), tid, nthr);
+                fft_tile<T, POW2>(tile, a.nst, a.stages, a.W, stride, nfc * TX, lines_div(a, nfc), tid, nthr);
 #pragma unroll
                 for (int m = 0; m < EPT; ++m) {
                     const int idx = r0 + m * rstep;
@@ -364,7 +365,7 @@ __global__ __launch_bounds__(512) void pass_strided(PassArgs<T> a) {
 //   out_{z,0} = sum_s sum_{Y in z} c_Y(k1) X_{Y,s}(k0) [F1 A_s - i k1 F1 B_s],
 //   out_{z,1} = sum_s sum_{Y in z} c_Y(k1) X_{Y,s}(k0) F1 C_s,
 // the outputs of the class form's pass 1.
-template <typename T, int EPT, int NZ>
+template <typename T, int EPT, int NZ, bool POW2 = false>
 __global__ __launch_bounds__(512) void pass_strided_mix(PassArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* tile = reinterpret_cast<cx<T>*>(smem_raw);
@@ -403,7 +404,7 @@ __global__ __launch_bounds__(512) void pass_strided_mix(PassArgs<T> a) {
                     dst[f * TX] = col_ok ? inbase[(size_t)f * a.ncell + off] : cx<T>{(T)0, (T)0};
             }
             __syncthreads();
-            fft_tile(tile, a.nst, a.stages, a.W, stride, nfc * TX, lines_div(a, nfc), tid, nthr);
+            fft_tile<T, POW2>(tile, a.nst, a.stages, a.W, stride, nfc * TX, lines_div(a, nfc), tid, nthr);
 #pragma unroll
             for (int m = 0; m < EPT; ++m) {
                 const int idx = r0 + m * rstep;

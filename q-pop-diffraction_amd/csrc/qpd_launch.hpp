@@ -44,6 +44,7 @@ struct Launch {
     int nf = 0;                     // inputs per LDS fill (generic)
     int nthr = 0;
     int ept = 0;                    // elements per thread, 4 / 8 (generic)
+    int pow2 = 0;                   // generic FP64: every stage radix 2, 4 or 8 (fft_tile POW2)
     size_t lds = 0;
     int64_t blocks = 0;
 };
