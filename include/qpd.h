@@ -151,7 +151,7 @@ int qpd_plan_schedule_name(const qpd_plan* plan, int pass, int index, char* buf,
 int qpd_plan_set_schedule(qpd_plan* plan, int pass, int index);
 int qpd_plan_get_schedule(const qpd_plan* plan, int pass);   /* the selected index */
 
-/* Lengths with a compile-time kernel for a role (0 source pass, 1 contiguous transform,
+/* Lengths with a compile-time FP32 kernel for a role (0 source pass, 1 contiguous transform,
  * 2 strided 3 -> 2 pass, 3 strided pass to the intensity, 4 strided transform, 5 first
  * step of a split length, 6 and 7 roles 2 and 3 as the second step of a split length,
  * 8 and 9 the strided pass over source groups with one or two z-classes, 10 and 11 those

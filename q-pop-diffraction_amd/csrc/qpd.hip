@@ -560,12 +560,6 @@ bool lane_offsets_fit(int G, int64_t step, int64_t S, size_t csize) {
     return (double)(lanes * step * S + G) * (double)csize < 4294967296.0;
 }
 
-const FastEntry* fast_lookup(int64_t N, int role) {
-    for (const FastEntry& e : fast_registry())
-        if (e.N == N && e.role == role) return &e;
-    return nullptr;
-}
-
 void set_fast(Launch& L, const FastEntry* fe, size_t lds) {
     L = Launch();
     L.fe = fe;
@@ -623,8 +617,10 @@ hipError_t launch_generic(const Launch& L, const PassArgs<T>& a, bool source, bo
 template <typename T>
 hipError_t launch_pass(const Launch& L, const PassArgs<T>& a, bool source, bool intensity, hipStream_t st) {
     if (L.fe) {
-        if constexpr (std::is_same<T, float>::value) return L.fe->launch(L, a, st);
-        return hipErrorInvalidValue;
+        if constexpr (std::is_same<T, float>::value)
+            return L.fe->launch ? L.fe->launch(L, a, st) : hipErrorInvalidValue;
+        else
+            return L.fe->launch64 ? L.fe->launch64(L, a, st) : hipErrorInvalidValue;
     }
     if constexpr (std::is_same<T, double>::value)
         if (L.pow2) return launch_generic<T, true>(L, a, source, intensity, st);
@@ -646,7 +642,8 @@ int pass_candidates(const AxisFFT& ax, int d, const int64_t dims[3], int nin, si
     const int64_t lines = contig ? dims[1] * dims[2] : 0;
     const int64_t n_other = d == 1 ? dims[2] : dims[1];
     const int64_t row_stride = d == 1 ? dims[0] : dims[0] * dims[1];
-    if (csize == 8 && fast_enabled()) {
+    // compile-time kernels: every role in FP32, the strided roles in FP64
+    if (fast_enabled() && (csize == 8 || !contig)) {
         int role = -1;
         if (contig)
             role = (nin == 4 && atomtab_entries <= kMaxAtomTab) ? kRoleSource : (nin == 1 ? kRoleContig : -1);
@@ -666,7 +663,7 @@ int pass_candidates(const AxisFFT& ax, int d, const int64_t dims[3], int nin, si
                 const int rl = roles[pos];
                 if (rl < 0) continue;
                 for (const FastEntry& e : fast_registry()) {
-                    if (e.N != N || e.role != rl) continue;
+                    if (e.N != N || e.role != rl || (size_t)e.real_bytes * 2 != csize) continue;
                     if (rl == kRoleSourceW && (pos == 0) != (kernel_scratch_bytes(e.kernel) <= 32)) continue;
                     const bool src = rl == kRoleSource || rl == kRoleSourceW;
                     const size_t lds = e.lds + (src ? (size_t)4 * atomtab_entries : 0);
@@ -680,7 +677,7 @@ int pass_candidates(const AxisFFT& ax, int d, const int64_t dims[3], int nin, si
             }
             const bool splittable =
                 role == kRoleStrided || role == kRoleIntensity || role == kRoleMix1 || role == kRoleMix2;
-            if (!contig && allow_split && splittable) {
+            if (!contig && allow_split && splittable && csize == 8) {
                 std::vector<std::pair<double, PassPlan>> splits;
                 for (int64_t n1 = 2; n1 <= N / 2; ++n1) {
                     if (N % n1) continue;
@@ -943,8 +940,8 @@ std::string describe_launch(const Launch& L) {
                                                "split-step", "strided", "intensity", "groups", "groups",
                                                "groups", "groups", "source-wave"};
     if (L.fe)
-        std::snprintf(buf, sizeof(buf), "fast %d (%s) G%d %s%s", L.fe->N, L.fe->radices, L.fe->G,
-                      role_name[L.fe->role], L.fe->no_prefetch ? " np" : "");
+        std::snprintf(buf, sizeof(buf), "fast %d (%s) G%d %s%s%s", L.fe->N, L.fe->radices, L.fe->G,
+                      role_name[L.fe->role], L.fe->no_prefetch ? " np" : "", L.fe->real_bytes == 8 ? " f64" : "");
     else
         std::snprintf(buf, sizeof(buf), "generic %s nl%d nf%d nthr%d ept%d%s", L.contiguous ? "contiguous" : "strided",
                       L.nl, L.nf, L.nthr, L.ept, L.pow2 ? " pow2" : "");
@@ -1195,7 +1192,8 @@ int qpd_fast_lengths(int role, int64_t* lengths, int cap) {
     if (role < 0 || role >= kNumRoles) return fail(QPD_ERR_ARG, "role must be 0..12");
     std::vector<int64_t> ls;
     for (const FastEntry& e : fast_registry())
-        if (e.role == role && std::find(ls.begin(), ls.end(), (int64_t)e.N) == ls.end()) ls.push_back(e.N);
+        if (e.role == role && e.real_bytes == 4 && std::find(ls.begin(), ls.end(), (int64_t)e.N) == ls.end())
+            ls.push_back(e.N);
     std::sort(ls.begin(), ls.end());
     for (int i = 0; i < cap && i < (int)ls.size() && lengths; ++i) lengths[i] = ls[i];
     return (int)ls.size();

@@ -59,6 +59,7 @@ struct PassPlan {
 };
 
 using FastLaunchFn = hipError_t (*)(const Launch&, const PassArgs<float>&, hipStream_t);
+using FastLaunchFn64 = hipError_t (*)(const Launch&, const PassArgs<double>&, hipStream_t);
 
 struct FastEntry {
     int N;
@@ -71,6 +72,8 @@ struct FastEntry {
     FastLaunchFn launch;
     const void* kernel;  // the kernel function (for its attributes)
     bool no_prefetch;    // strided roles: the variant without the register prefetch
+    int real_bytes = 4;  // 4: an FP32 kernel (launch), 8: an FP64 one (launch64)
+    FastLaunchFn64 launch64 = nullptr;
 };
 
 // Raise a kernel's dynamic LDS limit once per (kernel, device) and size.
