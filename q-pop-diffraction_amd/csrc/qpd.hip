@@ -642,8 +642,9 @@ int pass_candidates(const AxisFFT& ax, int d, const int64_t dims[3], int nin, si
     const int64_t lines = contig ? dims[1] * dims[2] : 0;
     const int64_t n_other = d == 1 ? dims[2] : dims[1];
     const int64_t row_stride = d == 1 ? dims[0] : dims[0] * dims[1];
-    // compile-time kernels: every role in FP32, the strided roles in FP64
-    if (fast_enabled() && (csize == 8 || !contig)) {
+    // compile-time kernels: every role in FP32; the strided roles and the block-per-row
+    // source pass in FP64
+    if (fast_enabled()) {
         int role = -1;
         if (contig)
             role = (nin == 4 && atomtab_entries <= kMaxAtomTab) ? kRoleSource : (nin == 1 ? kRoleContig : -1);
@@ -666,7 +667,7 @@ int pass_candidates(const AxisFFT& ax, int d, const int64_t dims[3], int nin, si
                     if (e.N != N || e.role != rl || (size_t)e.real_bytes * 2 != csize) continue;
                     if (rl == kRoleSourceW && (pos == 0) != (kernel_scratch_bytes(e.kernel) <= 32)) continue;
                     const bool src = rl == kRoleSource || rl == kRoleSourceW;
-                    const size_t lds = e.lds + (src ? (size_t)4 * atomtab_entries : 0);
+                    const size_t lds = e.lds + (src ? csize / 2 * (size_t)atomtab_entries : 0);
                     if ((int)lds > hard) continue;
                     if (!contig && !lane_offsets_fit(e.G, 1, row_stride, csize)) continue;
                     PassPlan pp;

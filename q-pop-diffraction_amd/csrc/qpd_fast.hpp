@@ -554,13 +554,19 @@ __global__ __launch_bounds__(4 * G * SZ::TPL) void fast_source(PassArgs<T> a) {
     const int ntab = a.n_atom_tab;
     for (int m = tid; m < ntab; m += NTHR) tabl[m] = a.atomtab[m];
 
-    // source cells of this thread: block cell e = tid + m * NTHR -> (row e / N, x e % N)
-    static_assert(std::is_same<T, float>::value, "the fast source pass is FP32");
-    CellInputsF ci[CPT];
+    // source cells of this thread: block cell e = tid + m * NTHR -> (row e / N, x e % N);
+    // FP32 splits the phase (CellInputsF), FP64 forms it whole per atom (cell_source)
+    constexpr bool F32 = std::is_same<T, float>::value;
+    using CellIn = std::conditional_t<F32, CellInputsF, CellInputs<T>>;
+    CellIn ci[CPT];
 #pragma unroll
     for (int m = 0; m < CPT; ++m) {
         const int e = tid + m * NTHR;
-        cell_load_f(a, cell0 + e, e < G * N && row0 + e / N < nrows, ci[m]);
+        const bool valid = e < G * N && row0 + e / N < nrows;
+        if constexpr (F32)
+            cell_load_f(a, cell0 + e, valid, ci[m]);
+        else
+            cell_load(a, cell0 + e, valid, ci[m]);
     }
     lds_barrier();
 
@@ -580,7 +586,10 @@ __global__ __launch_bounds__(4 * G * SZ::TPL) void fast_source(PassArgs<T> a) {
                 const int e = tid + m * NTHR;
                 if (e < G * N) {
                     cx<T> v0, v1, v2, v3;
-                    cell_source_f(a, tabl, ci[m], cell0 + e, src, v0, v1, v2, v3);
+                    if constexpr (F32)
+                        cell_source_f(a, tabl, ci[m], cell0 + e, src, v0, v1, v2, v3);
+                    else
+                        cell_source(a, tabl, ci[m], cell0 + e, src, v0, v1, v2, v3);
                     const int rr = e / N, x = e - rr * N;
                     X[lds_addr<false, SZ, 0>(0 * G + rr, x)] = v0;
                     X[lds_addr<false, SZ, 0>(1 * G + rr, x)] = v1;

@@ -60,13 +60,15 @@ void add_fast(std::vector<FastEntry>& v, const char* radices) {
     }
 }
 
-// FP64 kernels of the strided roles: the same code over complex FP64.  A thread holds
-// the same points as in FP32 at twice the registers, so the tiles take half the columns
-// (rows of the same bytes) and at most 256 threads: a wave may then use 512 registers
-// (AGPRs included) instead of spilling to scratch.
+// FP64 kernels of the strided roles and the block-per-row source pass: the same code over
+// complex FP64.  A thread holds the same points as in FP32 at twice the registers, so the
+// strided tiles take half the columns (rows of the same bytes), and every block has at most
+// 256 threads: a wave may then use 512 registers (AGPRs included) instead of spilling.
 template <class SZ, int ROLE, int G, bool PF = true>
 hipError_t fast_launch64(const Launch& L, const PassArgs<double>& a, hipStream_t st) {
-    if constexpr (ROLE == kRoleStrided)
+    if constexpr (ROLE == kRoleSource)
+        return launch_one(fast_source<double, SZ, G>, L, st, a);
+    else if constexpr (ROLE == kRoleStrided)
         return launch_one(fast_strided<double, SZ, G, 3, false, false>, L, st, a);
     else if constexpr (ROLE == kRoleIntensity)
         return launch_one(fast_strided<double, SZ, G, 2, true, false, PF>, L, st, a);
@@ -78,22 +80,24 @@ hipError_t fast_launch64(const Launch& L, const PassArgs<double>& a, hipStream_t
 
 template <class SZ, int ROLE, int G, bool PF = true>
 void add_fast64(std::vector<FastEntry>& v, const char* radices) {
-    if constexpr (G >= 4 && G * SZ::TPL <= 256) {
-        constexpr size_t lds = (2 * (size_t)SZ::strided_tile(G) + SZ::N) * sizeof(cx<double>);
+    constexpr bool src = ROLE == kRoleSource;
+    if constexpr (G >= (src ? 1 : 4) && (src ? 4 : 1) * G * SZ::TPL <= 256) {
+        constexpr size_t tile = src ? (size_t)4 * G * SZ::PITCH : (size_t)SZ::strided_tile(G);
+        constexpr size_t lds = (2 * tile + SZ::N) * sizeof(cx<double>);
         if constexpr (lds <= 160 * 1024)
-            v.push_back({SZ::N, ROLE, G, G * SZ::TPL, SZ::TPL, lds, radices, nullptr, nullptr, !PF, 8,
-                         &fast_launch64<SZ, ROLE, G, PF>});
+            v.push_back({SZ::N, ROLE, G, (src ? 4 : 1) * G * SZ::TPL, SZ::TPL, lds, radices, nullptr, nullptr, !PF,
+                         8, &fast_launch64<SZ, ROLE, G, PF>});
     }
 }
 
 template <class SZ, int ROLE>
 void add_fast64_variants(std::vector<FastEntry>& v, const char* radices) {
-    constexpr int G = fast_tile(ROLE, SZ::TPL) / 2;
+    constexpr int G = fast_tile(ROLE, SZ::TPL) / (ROLE == kRoleSource ? 1 : 2);
     add_fast64<SZ, ROLE, G>(v, radices);
     if constexpr (ROLE == kRoleIntensity) {
         add_fast64<SZ, ROLE, G, false>(v, radices);
         add_fast64<SZ, ROLE, G / 2, false>(v, radices);
-    } else {
+    } else if constexpr (ROLE != kRoleSource) {
         add_fast64<SZ, ROLE, G / 2>(v, radices);
     }
 }
@@ -160,6 +164,7 @@ void add_fast_roles(std::vector<FastEntry>& v, const char* radices) {
         add_fast<SZ, kRoleIntensityB, fast_tile(kRoleIntensityB, SZ::TPL)>(v, radices);
     if constexpr ((ROLES >> kRoleMix1) & 1u) add_fast_variants<SZ, kRoleMix1>(v, radices);
     if constexpr ((ROLES >> kRoleMix2) & 1u) add_fast_variants<SZ, kRoleMix2>(v, radices);
+    if constexpr ((ROLES >> kRoleSource) & 1u) add_fast64_variants<SZ, kRoleSource>(v, radices);
     if constexpr ((ROLES >> kRoleStrided) & 1u) add_fast64_variants<SZ, kRoleStrided>(v, radices);
     if constexpr ((ROLES >> kRoleIntensity) & 1u) add_fast64_variants<SZ, kRoleIntensity>(v, radices);
     if constexpr ((ROLES >> kRoleMix1) & 1u) add_fast64_variants<SZ, kRoleMix1>(v, radices);

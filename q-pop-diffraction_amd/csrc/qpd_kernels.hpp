@@ -115,6 +115,23 @@ struct CellInputs {
 };
 
 template <typename T>
+__device__ __forceinline__ void cell_load(const PassArgs<T>& a, uint32_t off, bool valid, CellInputs<T>& c) {
+    c.ph = -1;
+    c.rg = (T)1;
+    c.P0 = c.P1 = c.P2 = (T)0;
+    c.U0 = c.U1 = c.U2 = 0.0;
+    if (!valid) return;
+    c.ph = a.phase[off];
+    if (a.region) c.rg = a.region[off];
+    c.P0 = a.P[off];
+    c.P1 = a.P[a.ncell + off];
+    c.P2 = a.P[(size_t)2 * a.ncell + off];
+    c.U0 = a.U[off];
+    c.U1 = a.U[a.ncell + off];
+    c.U2 = a.U[(size_t)2 * a.ncell + off];
+}
+
+template <typename T>
 __device__ __forceinline__ void cell_source(const PassArgs<T>& a, const T* atomtab, const CellInputs<T>& ci,
                                             uint32_t off, int atom, cx<T>& E, cx<T>& V0, cx<T>& V1, cx<T>& V2) {
     E = V0 = V1 = V2 = {(T)0, (T)0};
@@ -188,22 +205,7 @@ __global__ __launch_bounds__(512) void pass_contig(PassArgs<T> a) {
         for (int m = 0; m < EPT; ++m) {
             const int e = tid + m * nthr;
             const int row = (int)fdiv((uint32_t)e, a.divN);
-            const bool valid = e < tileN && row0 + row < nrows;
-            ci[m].ph = -1;
-            ci[m].rg = (T)1;
-            ci[m].P0 = ci[m].P1 = ci[m].P2 = (T)0;
-            ci[m].U0 = ci[m].U1 = ci[m].U2 = 0.0;
-            if (valid) {
-                const uint32_t off = cell0 + (uint32_t)e;
-                ci[m].ph = a.phase[off];
-                if (a.region) ci[m].rg = a.region[off];
-                ci[m].P0 = a.P[off];
-                ci[m].P1 = a.P[a.ncell + off];
-                ci[m].P2 = a.P[(size_t)2 * a.ncell + off];
-                ci[m].U0 = a.U[off];
-                ci[m].U1 = a.U[a.ncell + off];
-                ci[m].U2 = a.U[(size_t)2 * a.ncell + off];
-            }
+            cell_load(a, cell0 + (uint32_t)e, e < tileN && row0 + row < nrows, ci[m]);
         }
     }
 
