@@ -757,12 +757,12 @@ __device__ __forceinline__ void wave_line_transform(cx<T>* line, cx<T> (&v)[SZ::
 // Three waves per SIMD: the register budget (168) that holds a row's eight cells of raw
 // inputs and the four fields' stage-0 values with at most a couple of spilled values
 // (measured faster than two waves without spills or four with more spills).  Lengths with
-// a radix-11, 13 or 17 stage, whose DFTs need more registers, run two waves per SIMD
-// instead of spilling.
+// a radix-11, 13 or 17 stage, whose DFTs need more registers, and FP64 run two waves per
+// SIMD instead of spilling.
 template <typename T, class SZ, int ROWS>
-__global__ __launch_bounds__(64 * kSrcWaves) __attribute__((amdgpu_waves_per_eu(SZ::RMAX >= 11 ? 2 : 3)))
-void fast_source_wave(PassArgs<T> a) {
-    static_assert(std::is_same<T, float>::value, "the fast source pass is FP32");
+__global__ __launch_bounds__(64 * kSrcWaves)
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 1 : (SZ::RMAX >= 11 ? 2 : 3)))) void fast_source_wave(PassArgs<T> a) {
+    constexpr bool F32 = std::is_same<T, float>::value;
     constexpr int N = SZ::N, TPL = SZ::TPL, R0 = SZ::rad(0), NR0 = SZ::nr(0), RL = SZ::RL, NRL = SZ::NRL;
     constexpr int NTHR = 64 * kSrcWaves;
     constexpr int NTW = StageTw<SZ>::total();
@@ -782,11 +782,12 @@ void fast_source_wave(PassArgs<T> a) {
     fill_stage_tw<T, SZ, 1>(tw, a.W, tid, NTHR);
     fill_stage_tw<T, SZ, 2>(tw, a.W, tid, NTHR);
     fill_stage_tw<T, SZ, 3>(tw, a.W, tid, NTHR);
-    // atom table with the su rows in memory-dim order: entry 2 + 3 m + k <- 2 + 3 ax_m + k
+    // FP32: the atom table with the su rows in memory-dim order (entry 2 + 3 m + k <-
+    // 2 + 3 ax_m + k); FP64 keeps the table's own order (cell_source picks the axes)
     for (int m = tid; m < a.n_atom_tab; m += NTHR) {
         const int e = m % kAtomTab;
         int src = m;
-        if (e >= 2 && e < 11) {
+        if (F32 && e >= 2 && e < 11) {
             const int dm = (e - 2) / 3;
             const int ax = dm == 0 ? a.ax0 : (dm == 1 ? a.ax1 : a.ax2);
             src = m - e + 2 + 3 * ax + (e - 2) % 3;
@@ -822,43 +823,81 @@ void fast_source_wave(PassArgs<T> a) {
             for (int r = 0; r < RL; ++r) oC[r * NRL] = v[3][r];
         }
     };
+    // one-hot phases: a group's fields are e and e u_d; each field e u_d is formed just
+    // before its transform, and F[e u_0] waits for F[e], which is transformed last, in place
+    auto emit_eu = [&](cx<T>(&e)[SZ::RMAX], T(&u)[3][R0], int grp, uint32_t lane_cell) {
+        cx<T>* oA = a.out + (size_t)grp * 3 * a.ncell + (size_t)lane_cell;
+        cx<T>* oB = oA + a.ncell;
+        cx<T>* oC = oB + a.ncell;
+        cx<T> w[SZ::RMAX], f0[RL];
+#pragma unroll
+        for (int r = 0; r < R0; ++r) w[r] = cscale(e[r], u[0][r]);
+        wave_line_transform<T, SZ>(line, w, j, inrow, tw);
+#pragma unroll
+        for (int r = 0; r < RL; ++r) f0[r] = w[r];
+#pragma unroll
+        for (int r = 0; r < R0; ++r) w[r] = cscale(e[r], u[1][r]);
+        wave_line_transform<T, SZ>(line, w, j, inrow, tw);
+        if (live && j < NRL) {
+#pragma unroll
+            for (int r = 0; r < RL; ++r) oB[r * NRL] = w[r];
+        }
+#pragma unroll
+        for (int r = 0; r < R0; ++r) w[r] = cscale(e[r], u[2][r]);
+        wave_line_transform<T, SZ>(line, w, j, inrow, tw);
+        if (live && j < NRL) {
+#pragma unroll
+            for (int r = 0; r < RL; ++r) oC[r * NRL] = w[r];
+        }
+        wave_line_transform<T, SZ>(line, e, j, inrow, tw);
+        if (live && j < NRL) {
+#pragma unroll
+            for (int r = 0; r < RL; ++r) oA[r * NRL] = e[r] + cscale(negi(f0[r]), kj[r * NRL]);
+        }
+    };
+    if constexpr (!F32) {
+        // FP64: one group per read of the cells, the phase of each atom formed whole in FP64;
+        // one-hot phases keep e and u_d (5 values a cell), fractional ones the four fields
+        for (int grp = 0; grp < a.ngroups; ++grp) {
+            const int src = a.chunk_src[a.group_first[grp]];
+            // at most kAhead cells' loads in flight: cell r + kAhead is requested once cell r
+            // has been formed (the scheduling barriers keep the compiler from hoisting them all)
+            constexpr int kAhead = R0 < 2 ? R0 : 2;
+            CellInputs<T> ci[R0];
+#pragma unroll
+            for (int r = 0; r < kAhead; ++r) cell_load(a, cell0 + (uint32_t)(j + r * NR0), has0, ci[r]);
+            if (!a.fraction) {
+                cx<T> e[SZ::RMAX];
+                T u[3][R0];
+#pragma unroll
+                for (int r = 0; r < R0; ++r) {
+                    __builtin_amdgcn_sched_barrier(0);
+                    if (r + kAhead < R0) cell_load(a, cell0 + (uint32_t)(j + (r + kAhead) * NR0), has0, ci[r + kAhead]);
+                    cell_source_eu(a, tabl, ci[r], src, e[r], u[0][r], u[1][r], u[2][r]);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+                emit_eu(e, u, grp, cell0 + (uint32_t)j);
+            } else {
+                cx<T> v[4][SZ::RMAX];
+#pragma unroll
+                for (int r = 0; r < R0; ++r) {
+                    __builtin_amdgcn_sched_barrier(0);
+                    if (r + kAhead < R0) cell_load(a, cell0 + (uint32_t)(j + (r + kAhead) * NR0), has0, ci[r + kAhead]);
+                    cell_source(a, tabl, ci[r], cell0 + (uint32_t)(j + r * NR0), src, v[0][r], v[1][r], v[2][r], v[3][r]);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+                emit(v, grp);
+            }
+        }
+        return;
+    }
     // The cells are formed one at a time after their loads; the scheduling barriers keep each
     // cell's atom-table reads next to its use.
-    if (!a.fraction) {
+    else if (!a.fraction) {
         // One-hot phases: a group's fields are e and e u_d, so a pair of groups is formed
         // from one read of the cells and only e and u_d (5 values a cell) are kept.  Each
         // field e u_d is formed just before its transform; F[e u_0] waits for F[e], which is
         // transformed last, in place.
-        auto emit_eu = [&](cx<T>(&e)[SZ::RMAX], T(&u)[3][R0], int grp, uint32_t lane_cell) {
-            cx<T>* oA = a.out + (size_t)grp * 3 * a.ncell + (size_t)lane_cell;
-            cx<T>* oB = oA + a.ncell;
-            cx<T>* oC = oB + a.ncell;
-            cx<T> w[SZ::RMAX], f0[RL];
-#pragma unroll
-            for (int r = 0; r < R0; ++r) w[r] = cscale(e[r], u[0][r]);
-            wave_line_transform<T, SZ>(line, w, j, inrow, tw);
-#pragma unroll
-            for (int r = 0; r < RL; ++r) f0[r] = w[r];
-#pragma unroll
-            for (int r = 0; r < R0; ++r) w[r] = cscale(e[r], u[1][r]);
-            wave_line_transform<T, SZ>(line, w, j, inrow, tw);
-            if (live && j < NRL) {
-#pragma unroll
-                for (int r = 0; r < RL; ++r) oB[r * NRL] = w[r];
-            }
-#pragma unroll
-            for (int r = 0; r < R0; ++r) w[r] = cscale(e[r], u[2][r]);
-            wave_line_transform<T, SZ>(line, w, j, inrow, tw);
-            if (live && j < NRL) {
-#pragma unroll
-                for (int r = 0; r < RL; ++r) oC[r * NRL] = w[r];
-            }
-            wave_line_transform<T, SZ>(line, e, j, inrow, tw);
-            if (live && j < NRL) {
-#pragma unroll
-                for (int r = 0; r < RL; ++r) oA[r * NRL] = e[r] + cscale(negi(f0[r]), kj[r * NRL]);
-            }
-        };
         for (int grp = 0; grp < a.ngroups; grp += 2) {
             const bool pair = grp + 1 < a.ngroups;
             const int srcA = a.chunk_src[a.group_first[grp]];

@@ -68,6 +68,8 @@ template <class SZ, int ROLE, int G, bool PF = true>
 hipError_t fast_launch64(const Launch& L, const PassArgs<double>& a, hipStream_t st) {
     if constexpr (ROLE == kRoleSource)
         return launch_one(fast_source<double, SZ, G>, L, st, a);
+    else if constexpr (ROLE == kRoleSourceW)
+        return launch_one(fast_source_wave<double, SZ, G / kSrcWaves>, L, st, a);
     else if constexpr (ROLE == kRoleStrided)
         return launch_one(fast_strided<double, SZ, G, 3, false, false>, L, st, a);
     else if constexpr (ROLE == kRoleIntensity)
@@ -109,10 +111,13 @@ template <class SZ>
 void add_source_wave(std::vector<FastEntry>& v, const char* radices) {
     if constexpr (SZ::TPL <= 64 && SZ::rad(0) <= 8) {
         constexpr int rows = 64 / SZ::TPL;
-        constexpr size_t lds = ((size_t)kSrcWaves * rows * SZ::PITCH + StageTw<SZ>::total()) * 8;
-        v.push_back({SZ::N, kRoleSourceW, kSrcWaves * rows, 64 * kSrcWaves, SZ::TPL, lds, radices,
+        constexpr size_t cells = (size_t)kSrcWaves * rows * SZ::PITCH + StageTw<SZ>::total();
+        v.push_back({SZ::N, kRoleSourceW, kSrcWaves * rows, 64 * kSrcWaves, SZ::TPL, cells * 8, radices,
                      &fast_launch<SZ, kRoleSourceW, kSrcWaves * rows>,
                      (const void*)fast_source_wave<float, SZ, rows>, false});
+        v.push_back({SZ::N, kRoleSourceW, kSrcWaves * rows, 64 * kSrcWaves, SZ::TPL, cells * 16, radices, nullptr,
+                     (const void*)fast_source_wave<double, SZ, rows>, false, 8,
+                     &fast_launch64<SZ, kRoleSourceW, kSrcWaves * rows>});
     }
 }
 

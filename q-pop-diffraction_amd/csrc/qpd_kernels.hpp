@@ -159,6 +159,29 @@ __device__ __forceinline__ void cell_source(const PassArgs<T>& a, const T* atomt
     }
 }
 
+// The same source for one-hot phases (no fraction field) in factored form: the fields are
+// e, e u_d0, e u_d1, e u_d2 with e = w f e^{-i t} and u_d the displacement along memory
+// dim d (cell_source forms exactly these products).
+template <typename T>
+__device__ __forceinline__ void cell_source_eu(const PassArgs<T>& a, const T* atomtab, const CellInputs<T>& ci,
+                                               int atom, cx<T>& e, T& u0, T& u1, T& u2) {
+    e = {(T)0, (T)0};
+    u0 = u1 = u2 = (T)0;
+    if (ci.ph < 0) return;
+    const T* tab = atomtab + (atom * a.n_phase + ci.ph) * kAtomTab;
+    const double ux = ci.U0 + (double)(tab[2] * ci.P0 + tab[3] * ci.P1 + tab[4] * ci.P2);
+    const double uy = ci.U1 + (double)(tab[5] * ci.P0 + tab[6] * ci.P1 + tab[7] * ci.P2);
+    const double uz = ci.U2 + (double)(tab[8] * ci.P0 + tab[9] * ci.P1 + tab[10] * ci.P2);
+    const double t = a.qC0 * ux + a.qC1 * uy + a.qC2 * uz;
+    T sn, cs;
+    phase_sincos(t, sn, cs);
+    const T fr = tab[0] * ci.rg, fi = tab[1] * ci.rg;
+    e = {fr * cs + fi * sn, fi * cs - fr * sn};
+    u0 = (T)pick3(ux, uy, uz, a.ax0);
+    u1 = (T)pick3(ux, uy, uz, a.ax1);
+    u2 = (T)pick3(ux, uy, uz, a.ax2);
+}
+
 // Input jj of a chunk, already multiplied by the chunk coefficient, into the outputs:
 // 0 -> out0, 1 -> out0 times -i k, j >= 2 -> out_{j-1}.
 template <typename T, int NOUT>
