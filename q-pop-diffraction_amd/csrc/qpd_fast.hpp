@@ -856,8 +856,9 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 1 : (SZ::RMAX >= 11 ? 2 : 3)
         }
     };
     if constexpr (!F32) {
-        // FP64: one group per read of the cells, the phase of each atom formed whole in FP64;
-        // one-hot phases keep e and u_d (5 values a cell), fractional ones the four fields
+        // FP64: the phase of each atom formed whole in FP64; one-hot phases keep e and u_d
+        // (5 values a cell) of a pair of groups per read of the cells, fractional ones the
+        // four fields of one group
         for (int grp = 0; grp < a.ngroups; ++grp) {
             const int src = a.chunk_src[a.group_first[grp]];
             // at most kAhead cells' loads in flight: cell r + kAhead is requested once cell r
@@ -867,16 +868,22 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 1 : (SZ::RMAX >= 11 ? 2 : 3)
 #pragma unroll
             for (int r = 0; r < kAhead; ++r) cell_load(a, cell0 + (uint32_t)(j + r * NR0), has0, ci[r]);
             if (!a.fraction) {
-                cx<T> e[SZ::RMAX];
-                T u[3][R0];
+                // a pair of groups from one read of the cells
+                const bool pair = grp + 1 < a.ngroups;
+                const int srcB = pair ? a.chunk_src[a.group_first[grp + 1]] : src;
+                cx<T> eA[SZ::RMAX], eB[SZ::RMAX];
+                T uA[3][R0], uB[3][R0];
 #pragma unroll
                 for (int r = 0; r < R0; ++r) {
                     __builtin_amdgcn_sched_barrier(0);
                     if (r + kAhead < R0) cell_load(a, cell0 + (uint32_t)(j + (r + kAhead) * NR0), has0, ci[r + kAhead]);
-                    cell_source_eu(a, tabl, ci[r], src, e[r], u[0][r], u[1][r], u[2][r]);
+                    cell_source_eu(a, tabl, ci[r], src, eA[r], uA[0][r], uA[1][r], uA[2][r]);
+                    if (pair) cell_source_eu(a, tabl, ci[r], srcB, eB[r], uB[0][r], uB[1][r], uB[2][r]);
                 }
                 __builtin_amdgcn_sched_barrier(0);
-                emit_eu(e, u, grp, cell0 + (uint32_t)j);
+                emit_eu(eA, uA, grp, cell0 + (uint32_t)j);
+                if (pair) emit_eu(eB, uB, grp + 1, cell0 + (uint32_t)j);
+                grp += pair ? 1 : 0;
             } else {
                 cx<T> v[4][SZ::RMAX];
 #pragma unroll
