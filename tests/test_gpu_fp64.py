@@ -37,7 +37,7 @@ def run_fp64(refl, shape, schedules=None):
     return I, snap, plan
 
 
-@pytest.mark.parametrize("shape", [(48, 40, 24), (104, 64, 30)])
+@pytest.mark.parametrize("shape", [(48, 40, 24), (104, 64, 30), (40, 24, 104)])
 @pytest.mark.parametrize("form", ["groups", "class"])
 def test_fp64_every_schedule(refl, shape, form, monkeypatch):
     from qpd_oracle import diffraction_fp64
