@@ -237,7 +237,7 @@ def fft3d(x, precision=None, stream=None):
 
 
 def fast_lengths(role):
-    """Transform lengths with a compile-time kernel of this build for a role (name or number)."""
+    """Transform lengths with a compile-time FP32 kernel of this build for a role (name or number)."""
     lib = load_library()
     r = ROLES[role] if isinstance(role, str) else int(role)
     n = lib.qpd_fast_lengths(r, None, 0)

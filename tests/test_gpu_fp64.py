@@ -12,6 +12,7 @@ import numpy as np
 import pytest
 
 from qpop_diffraction import workloads
+from qpop_diffraction._native import fast_lengths
 from qpop_diffraction.batch import PlanCache, Reflection, resident_job
 
 pytestmark = pytest.mark.gpu
@@ -22,14 +23,18 @@ def refl():
     return Reflection((1, 0, 3), workloads.pto_atoms((1, 0, 3)))
 
 
-def run_fp64(refl, shape, schedules=None):
+def run_fp64(refl, shape, schedules=None, pick=None):
+    """FP64 intensity of a twin snapshot; pick: {pass: suffix} selects the first schedule
+    whose name ends with it."""
     import torch
     nx, ny, nz = shape
-    snap = workloads.twin_snapshot(("fp64",) + shape, nx, ny, nz, seed=5, precision=64)
+    snap = workloads.twin_snapshot(("fp64",) + tuple(shape), nx, ny, nz, seed=5, precision=64)
     plans = PlanCache(64)
     plan = plans.get(snap, 0, refl)
     for d, idx in (schedules or {}).items():
         plan.set_schedule(d, idx)
+    for d, suffix in (pick or {}).items():
+        plan.set_schedule(d, next(i for i, nm in enumerate(plan.schedules(d)) if nm.endswith(suffix)))
     out = torch.empty(nx * ny * nz, dtype=torch.float64, device="cuda")
     work = torch.empty(plan.workspace_bytes, dtype=torch.uint8, device="cuda")
     resident_job(plans, snap, 0, refl, out, work)
@@ -73,3 +78,34 @@ def test_fp64_thin_volumes(refl, shape):
     I, snap, _ = run_fp64(refl, shape)
     ref = diffraction_fp64(snap.params, refl.atoms, *workloads.host_inputs(snap))[0]
     np.testing.assert_allclose(I, ref, rtol=1e-9, atol=1e-12 * ref.max())
+
+
+@pytest.mark.parametrize("role,dim,form", [("source_wave", 0, "groups"), ("source", 0, "class"),
+                                           ("strided", 1, "class"), ("groups", 1, "groups"), ("intensity", 2, "groups")])
+def test_fp64_every_length(refl, role, dim, form, monkeypatch):
+    """The compile-time FP64 kernels of a role on every length that has an FP32 kernel of the
+    role, at the reference's bound.  A length whose FP64 tile does not fit (more than 256
+    threads or 160 KB of LDS) runs the generic kernel, which must meet the same bound."""
+    from qpd_oracle import diffraction_fp64
+    if form == "class":
+        monkeypatch.setenv("QPD_CLASS_FORM", "1")
+    else:
+        monkeypatch.delenv("QPD_CLASS_FORM", raising=False)
+    lengths = fast_lengths("strided_groups2" if role == "groups" else role)
+    ran_f64 = 0
+    for N in lengths:
+        # pass 1 of the source-group form needs a length with a source-group kernel on dim 1 (20)
+        shape = [6, 20, 25] if role == "source_wave" else [6, 5, 4]
+        shape[dim] = N
+        plan_names = PlanCache(64).get(
+            workloads.twin_snapshot(("names",) + tuple(shape), *shape, seed=5, precision=64), 0, refl)
+        names = plan_names.schedules(dim)
+        wanted = " source-wave f64" if role == "source_wave" else " f64"
+        pick = {dim: wanted} if any(nm.endswith(wanted) for nm in names) else None
+        print(f"FP64 {role} N={N} grid {shape}: {'fast' if pick else 'generic'}", flush=True)
+        I, snap, plan = run_fp64(refl, shape, pick=pick)
+        ran_f64 += pick is not None
+        ref = diffraction_fp64(snap.params, refl.atoms, *workloads.host_inputs(snap))[0]
+        np.testing.assert_allclose(I, ref, rtol=1e-9, atol=1e-12 * ref.max(), err_msg=f"{role} N={N}")
+    print(f"{role}: {ran_f64} of {len(lengths)} lengths on an FP64 kernel")
+    assert ran_f64 >= len(lengths) // 2
