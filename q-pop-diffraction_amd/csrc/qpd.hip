@@ -132,11 +132,11 @@ int max_lds_bytes() {
 
 
 // Whether every stage's butterflies over `lines` lines fit nthr threads at the butterflies
-// a thread holds (fft_stage: max_regs of the real type over R).
-bool stage_fits(const AxisFFT& ax, int lines, int nthr, size_t csize) {
+// a thread holds (fft_stage: mr values, mr / R butterflies).
+bool stage_fits(const AxisFFT& ax, int lines, int nthr, int mr) {
     for (int s = 0; s < ax.nst; ++s) {
         const int R = ax.st[s].R;
-        const int mb = std::max(1, max_regs(csize / 2) / R);
+        const int mb = std::max(1, mr / R);
         const int64_t total = (int64_t)(ax.N / R) * lines;
         if ((total + nthr - 1) / nthr > mb) return false;
     }
@@ -171,34 +171,41 @@ bool choose_launch(const AxisFFT& ax, int nin, bool contiguous, size_t csize, in
         for (int p : per)
             for (int nf : nfs) order.push_back({nf, p});
     }
-    for (int pass = 0; pass < 2; ++pass) {
-        const int cap = pass == 0 ? lds_soft : lds_hard;
-        for (auto [nf, p] : order) {
-            {
-                if (contiguous && p > 1 && (int64_t)p > n_lines_total) continue;
-                const int64_t elems = (int64_t)p * N;      // one field's tile
-                if (contiguous && elems > 4096 && p > 1) continue;
-                const size_t lds = (size_t)nf * elems * csize;
-                if (lds > (size_t)cap) continue;
-                // threads: aim at ~ept_max/2 elements per thread, at most 512
-                int nthr = round_up((int)std::min<int64_t>(512, (elems + ept_max / 2 - 1) / (ept_max / 2)), 64);
-                nthr = std::max(64, std::min(512, nthr));
-                while (nthr < 512 && (elems + nthr - 1) / nthr > ept_max) nthr += 64;
-                if ((elems + nthr - 1) / nthr > ept_max) continue;
-                while (nthr < 512 && !stage_fits(ax, nf * p, nthr, csize)) nthr += 64;
-                if (!stage_fits(ax, nf * p, nthr, csize)) continue;
-                L.fe = nullptr;
-                L.contiguous = contiguous;
-                L.nl = p;
-                L.nf = nf;
-                L.nthr = nthr;
-                const int need = (int)((elems + nthr - 1) / nthr);
-                L.ept = need <= 2 ? 2 : (need <= 4 ? 4 : 8);
-                if (L.ept > ept_max) continue;
-                L.pow2 = csize == 16;
-                for (int s = 0; s < ax.nst; ++s) L.pow2 = L.pow2 && (ax.st[s].R == 2 || ax.st[s].R == 4 || ax.st[s].R == 8);
-                L.lds = lds;
-                return true;
+    // FP64 axes whose stages are all radix 2, 4 or 8 run the POW2 kernels (8 values a thread)
+    // where a shape fits them, else the all-radix ones (16)
+    bool pow2_axis = csize == 16;
+    for (int s = 0; s < ax.nst; ++s) pow2_axis = pow2_axis && (ax.st[s].R == 2 || ax.st[s].R == 4 || ax.st[s].R == 8);
+    for (int attempt = pow2_axis ? 0 : 1; attempt < 2; ++attempt) {
+        const bool pow2 = attempt == 0;
+        const int mr = pow2 ? kMaxRegsPow2 : kMaxRegs;
+        for (int pass = 0; pass < 2; ++pass) {
+            const int cap = pass == 0 ? lds_soft : lds_hard;
+            for (auto [nf, p] : order) {
+                {
+                    if (contiguous && p > 1 && (int64_t)p > n_lines_total) continue;
+                    const int64_t elems = (int64_t)p * N;      // one field's tile
+                    if (contiguous && elems > 4096 && p > 1) continue;
+                    const size_t lds = (size_t)nf * elems * csize;
+                    if (lds > (size_t)cap) continue;
+                    // threads: aim at ~ept_max/2 elements per thread, at most 512
+                    int nthr = round_up((int)std::min<int64_t>(512, (elems + ept_max / 2 - 1) / (ept_max / 2)), 64);
+                    nthr = std::max(64, std::min(512, nthr));
+                    while (nthr < 512 && (elems + nthr - 1) / nthr > ept_max) nthr += 64;
+                    if ((elems + nthr - 1) / nthr > ept_max) continue;
+                    while (nthr < 512 && !stage_fits(ax, nf * p, nthr, mr)) nthr += 64;
+                    if (!stage_fits(ax, nf * p, nthr, mr)) continue;
+                    L.fe = nullptr;
+                    L.contiguous = contiguous;
+                    L.nl = p;
+                    L.nf = nf;
+                    L.nthr = nthr;
+                    const int need = (int)((elems + nthr - 1) / nthr);
+                    L.ept = need <= 2 ? 2 : (need <= 4 ? 4 : 8);
+                    if (L.ept > ept_max) continue;
+                    L.pow2 = pow2 ? 1 : 0;
+                    L.lds = lds;
+                    return true;
+                }
             }
         }
     }
@@ -580,10 +587,12 @@ hipError_t launch_generic(const Launch& L, const PassArgs<T>& a, bool source, bo
     if (L.contiguous) {
         if (source) {
             // two elements a thread where they suffice: the FP64 source kernel holds each
-            // element's cell inputs and accumulators across the groups
-            if (a.nout != 3 || L.ept > 4) return hipErrorInvalidValue;
-            return L.ept <= 2 ? launch_one(pass_contig<T, 2, 3, true, POW2>, L, st, a)
-                              : launch_one(pass_contig<T, 4, 3, true, POW2>, L, st, a);
+            // element's cell inputs and accumulators across the groups; eight only for rows
+            // longer than 2048 cells
+            if (a.nout != 3 || L.ept > 8) return hipErrorInvalidValue;
+            return L.ept <= 2   ? launch_one(pass_contig<T, 2, 3, true, POW2>, L, st, a)
+                   : L.ept <= 4 ? launch_one(pass_contig<T, 4, 3, true, POW2>, L, st, a)
+                                : launch_one(pass_contig<T, 8, 3, true, POW2>, L, st, a);
         }
         if (a.nout != 1 || L.ept > 8) return hipErrorInvalidValue;
         return L.ept <= 4 ? launch_one(pass_contig<T, 4, 1, false, POW2>, L, st, a)
@@ -715,10 +724,18 @@ int pass_candidates(const AxisFFT& ax, int d, const int64_t dims[3], int nin, si
     }
     PassPlan gp;
     Launch& L = gp.L[0];
-    const int ept_max = contig ? (nin > 1 ? 4 : 8) : 8;
-    if (choose_launch(ax, nin, contig, csize, lines, soft, hard, ept_max, L)) {
+    // the source pass holds at most four cells a thread, or eight where a row of more than
+    // 2048 cells leaves no other shape (512 threads a block)
+    int ept_max = contig ? (nin > 1 ? 4 : 8) : 8;
+    bool fits = choose_launch(ax, nin, contig, csize, lines, soft, hard, ept_max, L);
+    if (!fits && contig && nin > 1) {
+        ept_max = 8;
+        fits = choose_launch(ax, nin, contig, csize, lines, soft, hard, ept_max, L);
+    }
+    if (fits) {
         L.blocks = contig ? (lines + L.nl - 1) / L.nl : (dims[0] + L.nl - 1) / L.nl * n_other;
-        if (L.blocks <= 0x7fffffffLL) out.push_back(gp);
+        const bool pushed = L.blocks <= 0x7fffffffLL;
+        if (pushed) out.push_back(gp);
         // the half-width tile as a second generic schedule, for the autotuner to weigh:
         // strided passes get fewer elements and registers a thread and narrower rows,
         // contiguous ones half the rows (half the LDS and threads: more blocks per CU)
@@ -729,7 +746,7 @@ int pass_candidates(const AxisFFT& ax, int d, const int64_t dims[3], int nin, si
                 L2.blocks = contig ? (lines + L2.nl - 1) / L2.nl : (dims[0] + L2.nl - 1) / L2.nl * n_other;
                 // FP64 source passes default to the half tile: at about 165 VGPRs a 512-thread
                 // block of 64 KB runs alone on its CU (3.55 against 2.60 ms at 512x512x104)
-                const bool first = contig && csize == 16 && atomtab_entries > 0;
+                const bool first = pushed && contig && csize == 16 && atomtab_entries > 0;
                 if (L2.blocks <= 0x7fffffffLL) out.insert(first ? out.end() - 1 : out.end(), gp2);
             }
         }

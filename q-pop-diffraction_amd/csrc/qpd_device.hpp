@@ -107,10 +107,10 @@ __device__ __forceinline__ void phase_sincos(double t, double& s, double& c) { s
 // table W[m] = exp(-2 pi i m / N).  The stage list lives in device memory and
 // is read with uniform (scalar) loads.
 constexpr int kMaxStages = 16;
-// Complex values a thread holds during one stage: 16 in FP32, 8 in FP64 (the same 64
-// VGPRs; FP64 kernels spill to scratch with 16).
+// Complex values a thread holds during one stage: 16, or 8 in the POW2 instantiations of
+// fft_tile (FP64 only: the same 64 VGPRs as 16 FP32 values; they spilled to scratch with 16).
 constexpr int kMaxRegs = 16;
-constexpr int max_regs(size_t real_bytes) { return real_bytes == 4 ? kMaxRegs : kMaxRegs / 2; }
+constexpr int kMaxRegsPow2 = 8;
 
 struct Stage {
     int R;
@@ -291,14 +291,14 @@ __device__ __forceinline__ void dft(cx<T>* v, const T* cr, const T* sr) {
 
 // One Stockham stage, in place in LDS.  The tile holds L lines of N points,
 // point j of line l at tile[j * stride + l]; lines [0, Lc) are transformed.
-// Every thread first loads all its butterflies (at most kMaxRegs values), the
+// Every thread first loads all its butterflies (at most MR values), the
 // block synchronises, then the butterflies are twiddled, transformed and
 // written back in autosort order.
-template <int R, typename T>
+template <int R, typename T, int MR = kMaxRegs>
 __device__ __forceinline__ void fft_stage(cx<T>* __restrict__ tile, const cx<T>* __restrict__ W, int NR,
                                           int stride, int Lc, FastDiv divLc, int Ns, FastDiv divNs, int tws,
                                           int tid, int nthr) {
-    constexpr int MB = (max_regs(sizeof(T)) / R) > 0 ? (max_regs(sizeof(T)) / R) : 1;
+    constexpr int MB = (MR / R) > 0 ? (MR / R) : 1;
     const int total = NR * Lc;
     T cr[R / 2 + 1], sr[R / 2 + 1];
     if constexpr ((R & 1) == 1) {
@@ -357,9 +357,9 @@ __device__ __forceinline__ void fft_tile(cx<T>* tile, int nst, const Stage* __re
         const int Ns = sd.Ns, tws = sd.tws, NR = sd.NR;
         if constexpr (POW2) {
             switch (sd.R) {
-                case 2: fft_stage<2, T>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
-                case 4: fft_stage<4, T>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
-                case 8: fft_stage<8, T>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
+                case 2: fft_stage<2, T, kMaxRegsPow2>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
+                case 4: fft_stage<4, T, kMaxRegsPow2>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
+                case 8: fft_stage<8, T, kMaxRegsPow2>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
                 default: break;  // the host sets POW2 only for plans of these radices
             }
         } else {

@@ -109,3 +109,16 @@ def test_fp64_every_length(refl, role, dim, form, monkeypatch):
         np.testing.assert_allclose(I, ref, rtol=1e-9, atol=1e-12 * ref.max(), err_msg=f"{role} N={N}")
     print(f"{role}: {ran_f64} of {len(lengths)} lengths on an FP64 kernel")
     assert ran_f64 >= len(lengths) // 2
+
+
+@pytest.mark.parametrize("shape", [(6, 5, 4000), (6, 4000, 4), (6, 5, 4096), (4000, 3, 2)])
+def test_fp64_long_axes(refl, shape):
+    """Long FP64 axes: strided ones (FP64 plans do not split them) of 4000 (radix-5 stages, 800
+    butterflies a line, which the all-radix kernels hold at 3 a thread) and 4096, the longest
+    (radix 8 only: 512 butterflies a line, one a thread in the 8-values-a-thread kernels), and
+    a contiguous 4000 (the source pass at eight cells a thread)."""
+    from qpd_oracle import diffraction_fp64
+    print(f"FP64 long axis {shape}", flush=True)
+    I, snap, _ = run_fp64(refl, shape)
+    ref = diffraction_fp64(snap.params, refl.atoms, *workloads.host_inputs(snap))[0]
+    np.testing.assert_allclose(I, ref, rtol=1e-9, atol=1e-12 * ref.max())

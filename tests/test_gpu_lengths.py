@@ -131,6 +131,16 @@ def test_split_long_strided(refl, N, dim):
     assert err <= FP32_PEAK_TOL
 
 
+@pytest.mark.parametrize("N", [3000, 4000])
+def test_long_contiguous_rows(refl, N):
+    """Rows longer than the compile-time lengths and than 2048 cells (the generic source pass
+    at eight cells a thread), as in a 4000 x 4000 x 68 volume laid out x fastest by the CLI."""
+    print(f"contiguous N={N}", flush=True)
+    err = volume_check(refl, N, 3, 2, expect={0: "generic"})
+    print(f"  max|dI|/peak {err:.2e}")
+    assert err <= FP32_PEAK_TOL
+
+
 @pytest.mark.parametrize("shape", [(40, 1000, 6), (24, 20, 500), (64, 20, 104)])
 def test_every_schedule(refl, shape):
     import torch
