@@ -137,7 +137,8 @@ size_t qpd_workspace_bytes(const qpd_plan* plan);
 int qpd_launches_per_volume(const qpd_plan* plan);
 
 /* Launch schedules.  Each pass of a plan can run one of several schedules: a
- * compile-time-length kernel of this build (FP32), a long strided length split into
+ * compile-time-length kernel of this build (every role in FP32; the source, strided and
+ * intensity roles in FP64, named "... f64"), in FP32 a long strided length split into
  * two launches of registered lengths N1 x N2 (N2-point sub-transforms with the twiddle
  * in place, then N1-point transforms with the pass's accumulation), or the generic
  * run-time-radix kernel.  Schedule 0 is the default; an autotuner times the others
