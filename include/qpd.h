@@ -121,6 +121,15 @@ int qpd_ingest_fields(const qpd_ingest* in, int precision, void* P_out, double* 
  * over dims[2]; a NULL output is skipped.  Each sum runs in a fixed order. */
 int qpd_project(const int64_t dims[3], const double* A, double* S0, double* S1, double* S2, void* stream);
 
+/* Reverse the axis order of n_comp fields of a grid: src holds n_comp arrays of dims[0] x
+ * dims[1] x dims[2] elements (dims[0] fastest, elem_bytes 1, 4 or 8 each), dst receives them
+ * with dims[2] fastest (element (i0, i1, i2) moves from i0 + dims[0] (i1 + dims[1] i2) to
+ * i2 + dims[2] (i1 + dims[1] i0)).  Device memory; src and dst must not overlap.  The CLI uses
+ * it to run x-fastest fields z fastest when the x rows are longer than every compile-time
+ * source kernel, and to return the intensity to x fastest.  No counterpart in the reference,
+ * whose arrays keep one layout (cli.py:189-217). */
+int qpd_reverse_axes(const int64_t dims[3], int n_comp, int elem_bytes, const void* src, void* dst, void* stream);
+
 /* Create a plan for one grid, crystal and precision (32: complex FP32 bulk with FP64
  * phase reduction and FP64-generated twiddles; 64: complex FP64 throughout).
  * Replaces the per-call setup of diffraction.py:75-94 and the HeFFTe plan of

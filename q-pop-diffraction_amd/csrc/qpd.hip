@@ -22,6 +22,7 @@
 #include "qpd_launch.hpp"
 #include "qpd_ingest.hpp"
 #include "qpd_project.hpp"
+#include "qpd_layout.hpp"
 
 using namespace qpd;
 
@@ -1203,6 +1204,39 @@ int qpd_project(const int64_t dims[3], const double* A, double* S0, double* S1, 
         hipLaunchKernelGGL(project_outer, dim3((d0 * d1 + 255u) / 256u), dim3(256), 0, st, A, S2, d0, d1, d2);
         QPD_HIP(hipGetLastError());
     }
+    return QPD_OK;
+}
+
+int qpd_reverse_axes(const int64_t dims[3], int n_comp, int elem_bytes, const void* src, void* dst, void* stream) {
+    g_error.clear();
+    if (!dims || !src || !dst) return fail(QPD_ERR_ARG, "null argument");
+    if (n_comp < 1) return fail(QPD_ERR_ARG, "n_comp must be >= 1");
+    for (int d = 0; d < 3; ++d)
+        if (dims[d] < 1) return fail(QPD_ERR_ARG, "dims must be >= 1");
+    const int64_t n = dims[0] * dims[1] * dims[2];
+    if (n >= (int64_t)1 << 31) return fail(QPD_ERR_UNSUPPORTED, "grids of 2^31 cells or more are not supported");
+    const char* s = static_cast<const char*>(src);
+    const char* d = static_cast<const char*>(dst);
+    const int64_t bytes = n * n_comp * elem_bytes;
+    if (s < d + bytes && d < s + bytes) return fail(QPD_ERR_ARG, "src and dst overlap");
+    const uint32_t n0 = (uint32_t)dims[0], n1 = (uint32_t)dims[1], n2 = (uint32_t)dims[2];
+    const uint32_t t0 = (n0 + kTrTile - 1) / kTrTile, t2 = (n2 + kTrTile - 1) / kTrTile;
+    const uint64_t blocks = (uint64_t)t0 * t2 * n1 * (uint64_t)n_comp;
+    if (blocks > 0x7fffffffULL) return fail(QPD_ERR_UNSUPPORTED, "too many tiles for one launch");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const dim3 grid((unsigned)blocks), block(kTrTile, kTrRows);
+    if (elem_bytes == 1)
+        hipLaunchKernelGGL(reverse_axes_kernel<uint8_t>, grid, block, 0, st, static_cast<const uint8_t*>(src),
+                           static_cast<uint8_t*>(dst), n0, n1, n2, t0, t2);
+    else if (elem_bytes == 4)
+        hipLaunchKernelGGL(reverse_axes_kernel<uint32_t>, grid, block, 0, st, static_cast<const uint32_t*>(src),
+                           static_cast<uint32_t*>(dst), n0, n1, n2, t0, t2);
+    else if (elem_bytes == 8)
+        hipLaunchKernelGGL(reverse_axes_kernel<uint64_t>, grid, block, 0, st, static_cast<const uint64_t*>(src),
+                           static_cast<uint64_t*>(dst), n0, n1, n2, t0, t2);
+    else
+        return fail(QPD_ERR_ARG, "elem_bytes must be 1, 4 or 8");
+    QPD_HIP(hipGetLastError());
     return QPD_OK;
 }
 

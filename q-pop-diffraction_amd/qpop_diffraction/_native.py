@@ -21,7 +21,8 @@ EXPORTS = (
     "qpd_plan_create", "qpd_plan_destroy", "qpd_workspace_bytes", "qpd_launches_per_volume",
     "qpd_plan_schedules", "qpd_plan_schedule_name", "qpd_plan_set_schedule", "qpd_plan_get_schedule",
     "qpd_fast_lengths", "qpd_intensity", "qpd_intensity_pass", "qpd_pass_bytes_per_cell", "qpd_fft3d",
-    "qpd_fft_length_supported", "qpd_ingest_fields", "qpd_project", "qpd_last_error", "qpd_version",
+    "qpd_fft_length_supported", "qpd_ingest_fields", "qpd_project", "qpd_reverse_axes", "qpd_last_error",
+    "qpd_version",
 )
 
 ROLES = {"source": 0, "contiguous": 1, "strided": 2, "intensity": 3, "strided1": 4, "split": 5,
@@ -104,6 +105,8 @@ def load_library():
     lib.qpd_ingest_fields.restype = i32
     lib.qpd_project.argtypes = [ctypes.POINTER(i64), vp, vp, vp, vp, vp]
     lib.qpd_project.restype = i32
+    lib.qpd_reverse_axes.argtypes = [ctypes.POINTER(i64), i32, i32, vp, vp, vp]
+    lib.qpd_reverse_axes.restype = i32
     lib.qpd_fft_length_supported.argtypes = [i64]
     lib.qpd_fft_length_supported.restype = i32
     lib.qpd_last_error.argtypes = []
@@ -246,6 +249,27 @@ def fast_lengths(role):
     buf = (ctypes.c_int64 * max(1, n))()
     lib.qpd_fast_lengths(r, buf, n)
     return [int(buf[i]) for i in range(n)]
+
+
+def reverse_axes(t, dims, stream=None):
+    """A new device tensor holding t's fields with the axis order reversed.
+
+    t: contiguous CUDA tensor of n_comp x dims[0] x dims[1] x dims[2] elements (dims[0]
+    fastest; 1-, 4- or 8-byte elements, any leading shape); the result has t's shape and
+    dims[2] fastest.
+    """
+    import torch
+    if not t.is_cuda or not t.is_contiguous():
+        raise TypeError("reverse_axes needs a contiguous CUDA tensor")
+    d0, d1, d2 = (int(v) for v in dims)
+    n = d0 * d1 * d2
+    if t.numel() % n:
+        raise ValueError(f"{t.numel()} elements are not whole fields of {d0} x {d1} x {d2}")
+    out = torch.empty_like(t)
+    check(load_library().qpd_reverse_axes((ctypes.c_int64 * 3)(d0, d1, d2), t.numel() // n, t.element_size(),
+                                          ctypes.c_void_p(t.data_ptr()), ctypes.c_void_p(out.data_ptr()),
+                                          stream if stream is not None else current_stream()))
+    return out
 
 
 def project(A, dims, which=(0, 1, 2), stream=None):

@@ -14,6 +14,7 @@ MI355X, which holds the grids the reference's multi-GPU program was written for.
 """
 
 import argparse
+import os
 import sys
 import time
 from pathlib import Path
@@ -22,6 +23,7 @@ import numpy as np
 
 from .atom_list import AtomList
 from .constants import L0, OS0
+from ._native import fast_lengths, reverse_axes
 from .diffraction import Engine, Volume, default_precision, q_axes, reciprocal_centre
 from .h5lite import open_file
 from .io_utils import (crop_slices, read_field, read_material_field, read_vector, write_diffraction_axes,
@@ -193,6 +195,18 @@ def _prepare_on_device(args, cfg, params, fe_ids, atoms, full, dz, sl, lims):
     return params, atoms, fields, step, cfg["output_file"]
 
 
+def run_z_fastest(nx, nz):
+    """Whether the CLI runs its x-fastest fields z fastest: when no compile-time source kernel
+    has the x length and one has the z length (the source pass then transforms z rows; the
+    generic kernel on long x rows forms each cell's source once per field).  QPD_CLI_LAYOUT=x
+    or z forces the layout."""
+    forced = os.environ.get("QPD_CLI_LAYOUT", "")
+    if forced in ("x", "z"):
+        return forced == "z"
+    kernels = set(fast_lengths("source")) | set(fast_lengths("source_wave"))
+    return nx not in kernels and nz in kernels
+
+
 def main(argv=None) -> int:
     """Run one diffraction calculation; 0 on success, 1 when parameter.atom.in is unusable."""
     args = _parse_args(argv)
@@ -210,7 +224,16 @@ def main(argv=None) -> int:
     engine = Engine(params, atoms)
     dims, axis = (params.nx, params.ny, params.nz), (0, 1, 2)
     vol = Volume(dims, axis, f["P"], f["U"], f["phase"], f["fraction"], f["region"], args.precision)
+    z_first = run_z_fastest(params.nx, params.nz)
+    if z_first:
+        # the fields reversed to z fastest on the device, and the intensity back to x fastest
+        rev = lambda t: None if t is None else reverse_axes(t, dims)
+        vol = Volume(dims[::-1], axis[::-1], rev(vol.P), rev(vol.U), rev(vol.phase), rev(vol.fraction),
+                     rev(vol.region), args.precision)
+        print(f"  x rows of {params.nx} cells: fields run z fastest")
     I = engine.intensity(vol)
+    if z_first:
+        I = reverse_axes(I, vol.dims)
     print(f"  GPU path {time.perf_counter() - t0:.3f} s for {params.n} cells")
     qC = engine.tables[4]
     # the intensity streams from HBM into the file slab by slab
