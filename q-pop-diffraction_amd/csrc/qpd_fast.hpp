@@ -757,8 +757,9 @@ __device__ __forceinline__ void wave_line_transform(cx<T>* line, cx<T> (&v)[SZ::
 // Three waves per SIMD: the register budget (168) that holds a row's eight cells of raw
 // inputs and the four fields' stage-0 values with at most a couple of spilled values
 // (measured faster than two waves without spills or four with more spills).  Lengths with
-// a radix-11, 13 or 17 stage, whose DFTs need more registers, and FP64 run two waves per
-// SIMD instead of spilling.
+// a radix-11, 13 or 17 stage, whose DFTs need more registers, run two waves per SIMD instead
+// of spilling; FP64 runs one, whose 512 registers (AGPRs included) hold a pair of groups'
+// e and u_d without spills (at two waves it spilled 132 to 340 bytes a lane).
 template <typename T, class SZ, int ROWS>
 __global__ __launch_bounds__(64 * kSrcWaves)
 __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 1 : (SZ::RMAX >= 11 ? 2 : 3)))) void fast_source_wave(PassArgs<T> a) {
@@ -883,7 +884,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 1 : (SZ::RMAX >= 11 ? 2 : 3)
                 __builtin_amdgcn_sched_barrier(0);
                 emit_eu(eA, uA, grp, cell0 + (uint32_t)j);
                 if (pair) emit_eu(eB, uB, grp + 1, cell0 + (uint32_t)j);
-                grp += pair ? 1 : 0;
+                grp += pair ? 1 : 0;  // the pair's second group is done as well
             } else {
                 cx<T> v[4][SZ::RMAX];
 #pragma unroll
