@@ -136,6 +136,51 @@ int qpd_reverse_axes(const int64_t dims[3], int n_comp, int elem_bytes, const vo
  * qpop_diffraction_mgpu.cu:154-157.  Allocates and uploads the plan tables. */
 int qpd_plan_create(qpd_plan** plan, const qpd_grid* grid, const qpd_crystal* crystal, int precision);
 
+/* Plan options of qpd_plan_create_opts.
+ *
+ * Pass order: pass 0 always transforms the contiguous dims[0] (and forms the source);
+ * passes 1 and 2 transform the two strided dims, in memory order (dims[1] then dims[2]) or
+ * swapped.  QPD_ORDER_AUTO keeps an order in which the cell's atoms form source groups, then
+ * takes the order of the smaller error estimate below.
+ *
+ * Arithmetic of a precision-32 plan (P, fractions and region are FP32 either way):
+ *   QPD_ARITH_FP32    complex FP32 transforms and spectra (FP64 phase reduction and
+ *                     FP64-generated tables);
+ *   QPD_ARITH_MIXED   complex FP64 transforms, FP32 spectra between passes (the memory traffic
+ *                     of QPD_ARITH_FP32);
+ *   QPD_ARITH_FP64    FP64 transforms and spectra, the FP32 inputs widened on the device first;
+ *   QPD_ARITH_AUTO    QPD_ARITH_MIXED when its error estimate is at most 6e-4, else
+ *                     QPD_ARITH_FP64.
+ * An FP32 transform of a line that holds a strong peak leaves errors of order 1e-7 of the
+ * peak amplitude on its other outputs, |dlog10 I| of about 1e-3 at voxels 1e-10 of the peak
+ * intensity, and where the lines of a volume are alike they repeat from line to line and the
+ * later transforms do not average them down; the transforms therefore run in FP64.  The
+ * rounding of FP32 spectra is relative to each stored value; the FP64 transforms after it
+ * spread it out: |dlog10 I| of about hypot(5.8e-3 / sqrt(N2), 3.5e-3 / sqrt(N1 N2)) with N2
+ * the length of the last pass and N1 that of pass 1 (a term vanishes when its length or
+ * plane is one cell).  A precision-64 plan is FP64 throughout (QPD_ARITH_AUTO or
+ * QPD_ARITH_FP64).  No counterpart in the reference (complex128 throughout,
+ * diffraction.py:97-111). */
+#define QPD_ORDER_AUTO 0
+#define QPD_ORDER_MEMORY 1
+#define QPD_ORDER_SWAPPED 2
+#define QPD_ARITH_AUTO 0
+#define QPD_ARITH_FP32 1
+#define QPD_ARITH_MIXED 2
+#define QPD_ARITH_FP64 3
+
+/* qpd_plan_create with explicit pass order and arithmetic (qpd_plan_create is this call with
+ * QPD_ORDER_AUTO and QPD_ARITH_AUTO). */
+int qpd_plan_create_opts(qpd_plan** plan, const qpd_grid* grid, const qpd_crystal* crystal, int precision, int order,
+                         int arith);
+
+/* What a plan chose: the memory dim each pass transforms (pass_dims[0] = 0), the precision
+ * of its transforms and of the spectra stored between passes (32 or 64), and the error
+ * estimate of that arithmetic (max |dlog10 I| over voxels >= 1e-10 of the peak; 0 when
+ * everything is FP64, -1 for QPD_ARITH_FP32, which has no bound).  NULL outputs are skipped. */
+int qpd_plan_info(const qpd_plan* plan, int pass_dims[3], int* compute_bits, int* storage_bits,
+                  double* dlog10_estimate);
+
 /* Release a plan (synchronises the device). */
 int qpd_plan_destroy(qpd_plan* plan);
 
@@ -177,7 +222,8 @@ int qpd_fast_lengths(int role, int64_t* lengths, int cap);
 int qpd_intensity(qpd_plan* plan, const qpd_fields* fields, double* I_dev, void* work_dev, void* stream);
 
 /* Enqueue only pass `pass` (0, 1 or 2) of qpd_intensity: 0 = source + transform along
- * dims[0], 1 = transform along dims[1], 2 = transform along dims[2] + intensity.  Has no
+ * dims[0], 1 and 2 = transforms along the strided dims in the plan's pass order, the last
+ * with the intensity (qpd_plan_info gives the order).  Has no
  * counterpart in the reference; it exists so that each kernel can be timed on its own.
  * Running passes 0, 1, 2 in order is exactly qpd_intensity. */
 int qpd_intensity_pass(qpd_plan* plan, const qpd_fields* fields, double* I_dev, void* work_dev, int pass,

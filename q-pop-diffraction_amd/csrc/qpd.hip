@@ -216,6 +216,25 @@ bool choose_launch(const AxisFFT& ax, int nin, bool contiguous, size_t csize, in
 }  // namespace
 
 // ---------------------------------------------------------------------------
+// Device tables of a plan in one element type (FP32 or FP64): twiddles, wavenumbers and
+// coefficient tables of each pass, chunk and group lists, the atom table and the
+// source-group mix tables.  An FP32 plan with an FP64 tail holds both sets.
+struct PlanTables {
+    void* dev = nullptr;              // one allocation holding every table below
+    const void* W[3] = {};
+    std::map<int, const void*> Wlen;  // twiddles exp(-2 pi i m / L) of the split step lengths L
+    const Stage* stages[3] = {};
+    const void* k[3] = {};
+    const void* coef[3] = {};
+    const int* chunk_src[3] = {};
+    const int* chunk_coef[3] = {};
+    const int* group_first[3] = {};
+    const void* atomtab = nullptr;
+    const void* mixc = nullptr;       // grouped plans: X_{Y,S}(k0) tables
+    const int* pair_first = nullptr;
+    const int* pair_y = nullptr;
+};
+
 struct qpd_plan {
     int precision;
     int64_t dims[3];
@@ -223,8 +242,20 @@ struct qpd_plan {
     int64_t ncell;
     int n_phase;
     int n_atom;
+    // pass d transforms memory dim pd[d]: pass 0 the contiguous dims[0], passes 1 and 2 the
+    // two strided dims, by default the longer one last (its transform spreads the rounding
+    // of the stored pass-1 spectra over the most points)
+    int pd[3] = {0, 1, 2};
+    // FP32 plans whose transforms run in FP64 on FP32 arrays (P, weights, the spectra between
+    // passes): every pass's kernels are the FP64 ones with FP32 I/O
+    int mixed = 0;
+    // FP32 plans computed and stored in FP64: P, fractions and region are widened into the
+    // workspace first (grids too small for FP32 storage between passes)
+    int wide = 0;
+    double est_dlog10 = 0.0;  // the error estimate the arithmetic was chosen from
     int nY, nZ;  // classes after pass 0 and pass 1
-    // atom classes: y-class = equal (R[ax1], R[ax2]), z-class = equal R[ax2]
+    // atom classes: y-class = equal (R[ax1], R[ax2]), z-class = equal R[ax2], with ax1, ax2
+    // the physical axes of passes 1 and 2
     std::vector<int> ycls, zcls;
     std::vector<std::pair<double, double>> ykeys;
     std::vector<double> zkeys;
@@ -234,27 +265,20 @@ struct qpd_plan {
     std::vector<std::complex<double>> phi;   // fE_a = phi_a fE_rep(group)
     int grouped = 0;                         // 1: pass 0 transforms groups, pass 1 mixes them
     int n1fields() const { return 3 * (grouped ? nS : nY); }
-    size_t csize;
-    AxisFFT fft[3];
+    size_t csz[3];                    // complex element bytes each pass computes in
+    size_t ssz = 8;                   // complex element bytes of the spectra stored between passes
+    AxisFFT fft[3];                   // transform of each pass
     PassPlan pass[3];                 // the schedule each pass runs (one of cand)
     std::vector<PassPlan> cand[3];    // schedules each pass can run, the default first
     int sel[3] = {0, 0, 0};
     int ngroups[3];
-    // device tables
-    void* dev = nullptr;          // one allocation holding every table below
-    const void* W[3];
-    std::map<int, const void*> Wlen;  // twiddles exp(-2 pi i m / L) of the split step lengths L
-    const Stage* stages[3];
-    const void* k[3];
-    const void* coef[3];
-    const int* chunk_src[3];
-    const int* chunk_coef[3];
-    const int* group_first[3];
-    const void* atomtab;
-    const void* mixc = nullptr;       // grouped plans: X_{Y,S}(k0) tables
-    const int* pair_first = nullptr;
-    const int* pair_y = nullptr;
+    PlanTables tab[2];                // [0] FP32, [1] FP64
+    const PlanTables& tables(int d) const { return tab[csz[d] == 16 ? 1 : 0]; }
     double qC[3];
+    size_t w1_bytes() const { return (size_t)n1fields() * (size_t)ncell * ssz; }
+    size_t w2_bytes() const { return (size_t)2 * nZ * (size_t)ncell * ssz; }
+    // the widened inputs of a wide plan: P (3 n), fractions (n_phase n), region (n), doubles
+    size_t wide_bytes() const { return wide ? (size_t)(4 + n_phase) * (size_t)ncell * sizeof(double) : 0; }
 };
 
 namespace {
@@ -285,7 +309,7 @@ struct TableBuilder {
 
 // Atom classes and source groups of a plan.
 void classify(qpd_plan* p, const qpd_grid* g, const qpd_crystal* c) {
-    const int ax1 = g->axis[1], ax2 = g->axis[2];
+    const int ax1 = g->axis[p->pd[1]], ax2 = g->axis[p->pd[2]];
     const int na = c->n_atom, np = c->n_phase;
     p->ycls.assign(na, 0);
     p->ykeys.clear();
@@ -343,7 +367,7 @@ void classify(qpd_plan* p, const qpd_grid* g, const qpd_crystal* c) {
 }
 
 template <typename T>
-int build_tables(qpd_plan* p, const qpd_grid* g, const qpd_crystal* c) {
+int build_tables(qpd_plan* p, const qpd_grid* g, const qpd_crystal* c, PlanTables& t) {
     TableBuilder tb;
     size_t offW[3], offK[3], offC[3], offSrc[3], offCoef[3], offGrp[3], offSt[3], offAtom;
     std::map<int, size_t> offWs;
@@ -358,8 +382,8 @@ int build_tables(qpd_plan* p, const qpd_grid* g, const qpd_crystal* c) {
     size_t offMix = 0, offPairF = 0, offPairY = 0;
 
     for (int d = 0; d < 3; ++d) {
-        const int64_t N = g->dims[d];
-        const double h = h_of[g->axis[d]];
+        const int64_t N = g->dims[p->pd[d]];
+        const double h = h_of[g->axis[p->pd[d]]];
         std::vector<double> wr(N), wi(N), kk(N);
         for (int64_t m = 0; m < N; ++m) {
             unit_root(m, N, wr[m], wi[m]);
@@ -493,24 +517,24 @@ int build_tables(qpd_plan* p, const qpd_grid* g, const qpd_crystal* c) {
         }
     offAtom = tb.add_real<T>(at);
 
-    QPD_HIP(hipMalloc(&p->dev, tb.bytes.size()));
-    QPD_HIP(hipMemcpy(p->dev, tb.bytes.data(), tb.bytes.size(), hipMemcpyHostToDevice));
-    unsigned char* base = static_cast<unsigned char*>(p->dev);
+    QPD_HIP(hipMalloc(&t.dev, tb.bytes.size()));
+    QPD_HIP(hipMemcpy(t.dev, tb.bytes.data(), tb.bytes.size(), hipMemcpyHostToDevice));
+    unsigned char* base = static_cast<unsigned char*>(t.dev);
     for (int d = 0; d < 3; ++d) {
-        p->W[d] = base + offW[d];
-        p->stages[d] = reinterpret_cast<const Stage*>(base + offSt[d]);
-        p->k[d] = base + offK[d];
-        p->coef[d] = base + offC[d];
-        p->chunk_src[d] = reinterpret_cast<const int*>(base + offSrc[d]);
-        p->chunk_coef[d] = reinterpret_cast<const int*>(base + offCoef[d]);
-        p->group_first[d] = reinterpret_cast<const int*>(base + offGrp[d]);
+        t.W[d] = base + offW[d];
+        t.stages[d] = reinterpret_cast<const Stage*>(base + offSt[d]);
+        t.k[d] = base + offK[d];
+        t.coef[d] = base + offC[d];
+        t.chunk_src[d] = reinterpret_cast<const int*>(base + offSrc[d]);
+        t.chunk_coef[d] = reinterpret_cast<const int*>(base + offCoef[d]);
+        t.group_first[d] = reinterpret_cast<const int*>(base + offGrp[d]);
     }
-    p->atomtab = base + offAtom;
-    for (auto& kv : offWs) p->Wlen[kv.first] = base + kv.second;
+    t.atomtab = base + offAtom;
+    for (auto& kv : offWs) t.Wlen[kv.first] = base + kv.second;
     if (p->grouped) {
-        p->mixc = base + offMix;
-        p->pair_first = reinterpret_cast<const int*>(base + offPairF);
-        p->pair_y = reinterpret_cast<const int*>(base + offPairY);
+        t.mixc = base + offMix;
+        t.pair_first = reinterpret_cast<const int*>(base + offPairF);
+        t.pair_y = reinterpret_cast<const int*>(base + offPairY);
     }
     return QPD_OK;
 }
@@ -585,6 +609,15 @@ void set_fast(Launch& L, const FastEntry* fe, size_t lds) {
 // do not spill).
 template <typename T, bool POW2>
 hipError_t launch_generic(const Launch& L, const PassArgs<T>& a, bool source, bool intensity, hipStream_t st) {
+    if constexpr (std::is_same<T, double>::value) {
+        // an FP32 plan computing in FP64: FP32 P, weights and spectra
+        if (L.io32 && L.contiguous) {
+            if (!source || a.nout != 3 || L.ept > 8) return hipErrorInvalidValue;
+            return L.ept <= 2   ? launch_one(pass_contig<T, 2, 3, true, POW2, true>, L, st, a)
+                   : L.ept <= 4 ? launch_one(pass_contig<T, 4, 3, true, POW2, true>, L, st, a)
+                                : launch_one(pass_contig<T, 8, 3, true, POW2, true>, L, st, a);
+        }
+    }
     if (L.contiguous) {
         if (source) {
             // two elements a thread where they suffice: the FP64 source kernel holds each
@@ -602,6 +635,27 @@ hipError_t launch_generic(const Launch& L, const PassArgs<T>& a, bool source, bo
     if (L.ept > 8) return hipErrorInvalidValue;
     // four elements a thread where they suffice: fewer registers (FP64 kernels spill)
     const bool e4 = L.ept <= 4;
+    if constexpr (std::is_same<T, double>::value) {
+        // an FP32 plan computing in FP64: complex FP32 inputs and outputs
+        if (L.io32) {
+            if (a.pair_first) {
+                if (a.ngroups == 1)
+                    return e4 ? launch_one(pass_strided_mix<T, 4, 1, POW2, true>, L, st, a)
+                              : launch_one(pass_strided_mix<T, 8, 1, POW2, true>, L, st, a);
+                if (a.ngroups == 2)
+                    return e4 ? launch_one(pass_strided_mix<T, 4, 2, POW2, true>, L, st, a)
+                              : launch_one(pass_strided_mix<T, 8, 2, POW2, true>, L, st, a);
+                return hipErrorInvalidValue;
+            }
+            if (intensity)
+                return a.nout != 1 ? hipErrorInvalidValue
+                       : e4        ? launch_one(pass_strided<T, 4, 1, true, POW2, true>, L, st, a)
+                                   : launch_one(pass_strided<T, 8, 1, true, POW2, true>, L, st, a);
+            if (a.nout != 2) return hipErrorInvalidValue;
+            return e4 ? launch_one(pass_strided<T, 4, 2, false, POW2, true>, L, st, a)
+                      : launch_one(pass_strided<T, 8, 2, false, POW2, true>, L, st, a);
+        }
+    }
     if (a.pair_first) {  // pass 1 over source groups: one or two z-classes
         if (a.ngroups == 1)
             return e4 ? launch_one(pass_strided_mix<T, 4, 1, POW2>, L, st, a)
@@ -642,16 +696,19 @@ hipError_t launch_pass(const Launch& L, const PassArgs<T>& a, bool source, bool 
 // split N1 x N2 into registered step lengths (most even first; only where allow_split),
 // then the generic kernel (for strided passes at two tile widths).  A split is the default
 // only where the length has no kernel of its own.
-int pass_candidates(const AxisFFT& ax, int d, const int64_t dims[3], int nin, size_t csize, bool intensity_last,
-                    int atomtab_entries, bool allow_split, std::vector<PassPlan>& out, int mix_role = -1) {
+// Pass d runs along memory dim md (pass 0 along dims[0]); io32: an FP64 pass on FP32 arrays
+// (an FP32 plan computing in FP64).
+int pass_candidates(const AxisFFT& ax, int d, int md, const int64_t dims[3], int nin, size_t csize, bool io32,
+                    bool intensity_last, int atomtab_entries, bool allow_split, std::vector<PassPlan>& out,
+                    int mix_role = -1) {
     out.clear();
     const int hard = max_lds_bytes();
     const int soft = std::min(hard, 65536);
     const bool contig = d == 0;
-    const int64_t N = dims[d];
+    const int64_t N = dims[md];
     const int64_t lines = contig ? dims[1] * dims[2] : 0;
-    const int64_t n_other = d == 1 ? dims[2] : dims[1];
-    const int64_t row_stride = d == 1 ? dims[0] : dims[0] * dims[1];
+    const int64_t n_other = md == 1 ? dims[2] : dims[1];
+    const int64_t row_stride = md == 1 ? dims[0] : dims[0] * dims[1];
     // compile-time kernels: every role in FP32; the strided roles and the block-per-row
     // source pass in FP64
     if (fast_enabled()) {
@@ -674,7 +731,7 @@ int pass_candidates(const AxisFFT& ax, int d, const int64_t dims[3], int nin, si
                 const int rl = roles[pos];
                 if (rl < 0) continue;
                 for (const FastEntry& e : fast_registry()) {
-                    if (e.N != N || e.role != rl || (size_t)e.real_bytes * 2 != csize) continue;
+                    if (e.N != N || e.role != rl || (size_t)e.real_bytes * 2 != csize || e.io32 != io32) continue;
                     if (rl == kRoleSourceW && (pos == 0) != (kernel_scratch_bytes(e.kernel) <= 32)) continue;
                     const bool src = rl == kRoleSource || rl == kRoleSourceW;
                     const size_t lds = e.lds + (src ? csize / 2 * (size_t)atomtab_entries : 0);
@@ -734,6 +791,8 @@ int pass_candidates(const AxisFFT& ax, int d, const int64_t dims[3], int nin, si
         fits = choose_launch(ax, nin, contig, csize, lines, soft, hard, ept_max, L);
     }
     if (fits) {
+        L.io32 = io32 ? 1 : 0;
+        L.mix = mix_role >= 0 && d == 1 ? 1 : 0;
         L.blocks = contig ? (lines + L.nl - 1) / L.nl : (dims[0] + L.nl - 1) / L.nl * n_other;
         const bool pushed = L.blocks <= 0x7fffffffLL;
         if (pushed) out.push_back(gp);
@@ -744,6 +803,8 @@ int pass_candidates(const AxisFFT& ax, int d, const int64_t dims[3], int nin, si
             PassPlan gp2;
             Launch& L2 = gp2.L[0];
             if (choose_launch(ax, nin, contig, csize, lines, soft, hard, ept_max, L2, L.nl / 2)) {
+                L2.io32 = io32 ? 1 : 0;
+                L2.mix = mix_role >= 0 && d == 1 ? 1 : 0;
                 L2.blocks = contig ? (lines + L2.nl - 1) / L2.nl : (dims[0] + L2.nl - 1) / L2.nl * n_other;
                 // FP64 source passes default to the half tile: at about 165 VGPRs a 512-thread
                 // block of 64 KB runs alone on its CU (3.55 against 2.60 ms at 512x512x104)
@@ -762,7 +823,7 @@ int pass_candidates(const AxisFFT& ax, int d, const int64_t dims[3], int nin, si
 int plan_launches(const AxisFFT fft[3], const int64_t dims[3], const int nin[3], size_t csize, PassPlan out[3]) {
     for (int d = 0; d < 3; ++d) {
         std::vector<PassPlan> c;
-        int rc = pass_candidates(fft[d], d, dims, nin[d], csize, false, 0, false, c);
+        int rc = pass_candidates(fft[d], d, d, dims, nin[d], csize, false, false, 0, false, c);
         if (rc) return rc;
         out[d] = c[0];
     }
@@ -782,85 +843,133 @@ void set_strided(PassArgs<T>& a, int d, const int64_t dims[3]) {
     }
 }
 
+// Enqueue pass d of a volume in element type T (the pass's own precision).  The workspace
+// holds pass 0's outputs (w1, pass-0 precision) followed by pass 1's (w2, pass-1 precision).
 template <typename T>
+int run_pass(qpd_plan* p, const qpd_fields* f, double* I, void* work, int d, hipStream_t st) {
+    const PlanTables& tb = p->tables(d);
+    unsigned char* w1 = static_cast<unsigned char*>(work);
+    unsigned char* w2 = w1 + p->w1_bytes();
+    const int md = p->pd[d];
+    const PassPlan& pp = p->pass[d];
+    const Launch& L = pp.L[pp.nlaunch - 1];  // the launch that accumulates
+    PassArgs<T> a;
+    fill_common(a, p->fft[d], tb.stages[d], L, p->dims);
+    a.W = static_cast<const cx<T>*>(tb.W[d]);
+    a.coef = static_cast<const cx<T>*>(tb.coef[d]);
+    a.k = static_cast<const T*>(tb.k[d]);
+    a.chunk_src = tb.chunk_src[d];
+    a.chunk_coef = tb.chunk_coef[d];
+    a.group_first = tb.group_first[d];
+    a.ngroups = p->ngroups[d];
+    a.nin = 4 - d;
+    a.nout = 3 - d;
+    if (d == 0) {
+        a.out = reinterpret_cast<cx<T>*>(w1);
+        a.P = static_cast<const T*>(f->P);
+        a.U = f->U;
+        a.phase = f->phase;
+        a.fraction = static_cast<const T*>(f->fraction);
+        a.region = static_cast<const T*>(f->region);
+        a.atomtab = static_cast<const T*>(tb.atomtab);
+        a.qC0 = p->qC[0];
+        a.qC1 = p->qC[1];
+        a.qC2 = p->qC[2];
+        a.n_phase = p->n_phase;
+        a.n_atom_tab = p->n_atom * p->n_phase * kAtomTab;
+        // the fields pass 0 writes: [E - i k0 U_ax0, U_ax1, U_ax2] with ax1, ax2 the physical
+        // axes that passes 1 and 2 transform
+        a.ax0 = p->axis[0];
+        a.ax1 = p->axis[p->pd[1]];
+        a.ax2 = p->axis[p->pd[2]];
+    } else {
+        set_strided(a, md, p->dims);
+        // the spectra of a plan computing in FP64 on FP32 arrays are passed through pointers of
+        // the compute type; its kernels (io32) reinterpret them
+        a.in = reinterpret_cast<const cx<T>*>(d == 1 ? w1 : w2);
+        if (d == 1)
+            a.out = reinterpret_cast<cx<T>*>(w2);
+        else
+            a.I = I;
+        if (d == 1 && p->grouped) {
+            a.nsrc = p->nS;
+            a.pair_first = tb.pair_first;
+            a.pair_y = tb.pair_y;
+            a.mixc = static_cast<const cx<T>*>(tb.mixc);
+        }
+    }
+    if (pp.nlaunch == 2) {
+        // step 1: N2-point sub-transforms of every input field, twiddled, in place (FP32 plans
+        // without a tail only: the first step rewrites its input fields in their own precision)
+        if (p->mixed) return fail(QPD_ERR_UNSUPPORTED, "split pass in a plan computing in FP64 on FP32 spectra");
+        const uint32_t N1 = (uint32_t)pp.N1, N2 = (uint32_t)pp.N2;
+        PassArgs<T> s1 = a;
+        s1.N = (int)N2;
+        s1.nl = pp.L[0].nl;
+        set_strided(s1, md, p->dims);
+        s1.W = static_cast<const cx<T>*>(tb.Wlen.at(pp.N2));
+        s1.Wsplit = static_cast<const cx<T>*>(tb.W[d]);
+        s1.nq = N1;
+        s1.divQ = make_fastdiv(N1);
+        s1.in_qmul = s1.out_qmul = 1;
+        s1.in_step = s1.out_step = N1;
+        s1.out = const_cast<cx<T>*>(a.in);
+        s1.nfields = (d == 1 ? p->n1fields() : 2 * p->nZ);
+        QPD_HIP(launch_pass<T>(pp.L[0], s1, false, false, st));
+        // step 2: N1-point transforms over rows N1 q + y1 into rows q + N2 k1, accumulated
+        a.N = (int)N1;
+        a.W = static_cast<const cx<T>*>(tb.Wlen.at(pp.N1));
+        a.nq = N2;
+        a.divQ = make_fastdiv(N2);
+        a.in_qmul = N1;
+        a.in_step = 1;
+        a.out_qmul = 1;
+        a.out_step = N2;
+        a.c_qmul = 1;
+        a.c_step = N2;
+    }
+    QPD_HIP(launch_pass<T>(L, a, d == 0, d == 2, st));
+    return QPD_OK;
+}
+
+// dst[i] = src[i] in double, i < n (grid-stride).
+__global__ __launch_bounds__(256) void widen_kernel(const float* __restrict__ src, double* __restrict__ dst, size_t n) {
+    const size_t step = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) dst[i] = (double)src[i];
+}
+
+hipError_t widen(const void* src, double* dst, size_t n, hipStream_t st) {
+    const size_t blocks = std::min<size_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(widen_kernel, dim3((unsigned)blocks), dim3(256), 0, st, static_cast<const float*>(src), dst, n);
+    return hipGetLastError();
+}
+
 int run_intensity(qpd_plan* p, const qpd_fields* f, double* I, void* work, hipStream_t st, int only = -1) {
-    const int64_t n = p->ncell;
-    cx<T>* w1 = static_cast<cx<T>*>(work);
-    cx<T>* w2 = w1 + (size_t)p->n1fields() * n;
     for (int d = 0; d < 3; ++d) {
         if (only >= 0 && d != only) continue;
-        const PassPlan& pp = p->pass[d];
-        const Launch& L = pp.L[pp.nlaunch - 1];  // the launch that accumulates
-        PassArgs<T> a;
-        fill_common(a, p->fft[d], p->stages[d], L, p->dims);
-        a.W = static_cast<const cx<T>*>(p->W[d]);
-        a.coef = static_cast<const cx<T>*>(p->coef[d]);
-        a.k = static_cast<const T*>(p->k[d]);
-        a.chunk_src = p->chunk_src[d];
-        a.chunk_coef = p->chunk_coef[d];
-        a.group_first = p->group_first[d];
-        a.ngroups = p->ngroups[d];
-        a.nin = 4 - d;
-        a.nout = 3 - d;
-        if (d == 0) {
-            a.out = w1;
-            a.P = static_cast<const T*>(f->P);
-            a.U = f->U;
-            a.phase = f->phase;
-            a.fraction = static_cast<const T*>(f->fraction);
-            a.region = static_cast<const T*>(f->region);
-            a.atomtab = static_cast<const T*>(p->atomtab);
-            a.qC0 = p->qC[0];
-            a.qC1 = p->qC[1];
-            a.qC2 = p->qC[2];
-            a.n_phase = p->n_phase;
-            a.n_atom_tab = p->n_atom * p->n_phase * kAtomTab;
-            a.ax0 = p->axis[0];
-            a.ax1 = p->axis[1];
-            a.ax2 = p->axis[2];
-        } else {
-            set_strided(a, d, p->dims);
-            a.in = d == 1 ? w1 : w2;
-            if (d == 1)
-                a.out = w2;
-            else
-                a.I = I;
-            if (d == 1 && p->grouped) {
-                a.nsrc = p->nS;
-                a.pair_first = p->pair_first;
-                a.pair_y = p->pair_y;
-                a.mixc = static_cast<const cx<T>*>(p->mixc);
+        int rc;
+        if (d == 0 && p->wide) {
+            // FP32 inputs of a plan computed in FP64: widened into the end of the workspace
+            const size_t n = (size_t)p->ncell;
+            double* Pd = reinterpret_cast<double*>(static_cast<unsigned char*>(work) + p->w1_bytes() + p->w2_bytes());
+            double* fr = Pd + 3 * n;
+            double* rg = fr + (size_t)p->n_phase * n;
+            qpd_fields wf = *f;
+            QPD_HIP(widen(f->P, Pd, 3 * n, st));
+            wf.P = Pd;
+            if (f->fraction) {
+                QPD_HIP(widen(f->fraction, fr, (size_t)p->n_phase * n, st));
+                wf.fraction = fr;
             }
+            if (f->region) {
+                QPD_HIP(widen(f->region, rg, n, st));
+                wf.region = rg;
+            }
+            rc = run_pass<double>(p, &wf, I, work, d, st);
+        } else {
+            rc = p->csz[d] == 8 ? run_pass<float>(p, f, I, work, d, st) : run_pass<double>(p, f, I, work, d, st);
         }
-        if (pp.nlaunch == 2) {
-            // step 1: N2-point sub-transforms of every input field, twiddled, in place
-            const uint32_t N1 = (uint32_t)pp.N1, N2 = (uint32_t)pp.N2;
-            PassArgs<T> s1 = a;
-            s1.N = (int)N2;
-            s1.nl = pp.L[0].nl;
-            set_strided(s1, d, p->dims);
-            s1.W = static_cast<const cx<T>*>(p->Wlen.at(pp.N2));
-            s1.Wsplit = static_cast<const cx<T>*>(p->W[d]);
-            s1.nq = N1;
-            s1.divQ = make_fastdiv(N1);
-            s1.in_qmul = s1.out_qmul = 1;
-            s1.in_step = s1.out_step = N1;
-            s1.out = const_cast<cx<T>*>(a.in);
-            s1.nfields = (d == 1 ? p->n1fields() : 2 * p->nZ);
-            QPD_HIP(launch_pass<T>(pp.L[0], s1, false, false, st));
-            // step 2: N1-point transforms over rows N1 q + y1 into rows q + N2 k1, accumulated
-            a.N = (int)N1;
-            a.W = static_cast<const cx<T>*>(p->Wlen.at(pp.N1));
-            a.nq = N2;
-            a.divQ = make_fastdiv(N2);
-            a.in_qmul = N1;
-            a.in_step = 1;
-            a.out_qmul = 1;
-            a.out_step = N2;
-            a.c_qmul = 1;
-            a.c_step = N2;
-        }
-        QPD_HIP(launch_pass<T>(L, a, d == 0, d == 2, st));
+        if (rc != QPD_OK) return rc;
     }
     return QPD_OK;
 }
@@ -959,11 +1068,12 @@ std::string describe_launch(const Launch& L) {
                                                "split-step", "strided", "intensity", "groups", "groups",
                                                "groups", "groups", "source-wave"};
     if (L.fe)
-        std::snprintf(buf, sizeof(buf), "fast %d (%s) G%d %s%s%s", L.fe->N, L.fe->radices, L.fe->G,
-                      role_name[L.fe->role], L.fe->no_prefetch ? " np" : "", L.fe->real_bytes == 8 ? " f64" : "");
+        std::snprintf(buf, sizeof(buf), "fast %d (%s) G%d %s%s%s%s", L.fe->N, L.fe->radices, L.fe->G,
+                      role_name[L.fe->role], L.fe->no_prefetch ? " np" : "", L.fe->real_bytes == 8 ? " f64" : "",
+                      L.fe->io32 ? " io32" : "");
     else
-        std::snprintf(buf, sizeof(buf), "generic %s nl%d nf%d nthr%d ept%d%s", L.contiguous ? "contiguous" : "strided",
-                      L.nl, L.nf, L.nthr, L.ept, L.pow2 ? " pow2" : "");
+        std::snprintf(buf, sizeof(buf), "generic %s%s nl%d nf%d nthr%d ept%d%s%s", L.contiguous ? "contiguous" : "strided",
+                      L.mix ? " groups" : "", L.nl, L.nf, L.nthr, L.ept, L.pow2 ? " pow2" : "", L.io32 ? " io32" : "");
     return buf;
 }
 
@@ -1042,10 +1152,21 @@ int qpd_fft_length_supported(int64_t n) {
 }
 
 int qpd_plan_create(qpd_plan** plan, const qpd_grid* grid, const qpd_crystal* c, int precision) {
+    return qpd_plan_create_opts(plan, grid, c, precision, QPD_ORDER_AUTO, QPD_ARITH_AUTO);
+}
+
+int qpd_plan_create_opts(qpd_plan** plan, const qpd_grid* grid, const qpd_crystal* c, int precision, int order,
+                         int arith) {
     g_error.clear();
     if (!plan || !grid || !c) return fail(QPD_ERR_ARG, "null argument");
     *plan = nullptr;
     if (precision != 32 && precision != 64) return fail(QPD_ERR_ARG, "precision must be 32 or 64");
+    if (order != QPD_ORDER_AUTO && order != QPD_ORDER_MEMORY && order != QPD_ORDER_SWAPPED)
+        return fail(QPD_ERR_ARG, "order must be QPD_ORDER_AUTO, QPD_ORDER_MEMORY or QPD_ORDER_SWAPPED");
+    if (arith < QPD_ARITH_AUTO || arith > QPD_ARITH_FP64)
+        return fail(QPD_ERR_ARG, "arith must be one of QPD_ARITH_AUTO, _FP32, _MIXED, _FP64");
+    if (precision == 64 && arith != QPD_ARITH_AUTO && arith != QPD_ARITH_FP64)
+        return fail(QPD_ERR_ARG, "a precision-64 plan computes in FP64 throughout");
     bool seen[3] = {false, false, false};
     for (int d = 0; d < 3; ++d) {
         if (grid->dims[d] < 1) return fail(QPD_ERR_ARG, "grid dims must be >= 1");
@@ -1069,35 +1190,91 @@ int qpd_plan_create(qpd_plan** plan, const qpd_grid* grid, const qpd_crystal* c,
         p->dims[d] = grid->dims[d];
         p->axis[d] = grid->axis[d];
         p->ncell *= grid->dims[d];
-        if (!make_axis(grid->dims[d], p->fft[d])) {
+    }
+    // source groups: when they are fewer than the y-classes, pass 0 transforms one
+    // representative per group and pass 1 mixes them into one or two z-classes (classes of
+    // equal position along the last pass's axis); QPD_CLASS_FORM=1 keeps the class form for
+    // comparison runs
+    const char* cf = std::getenv("QPD_CLASS_FORM");
+    const bool class_form = cf && cf[0] == '1';
+    auto groups_with = [&](int pd1) {
+        p->pd[1] = pd1;
+        p->pd[2] = 3 - pd1;
+        classify(p, grid, c);
+        return p->nZ <= 2 && p->nS < p->nY && !class_form;
+    };
+    // Arithmetic of FP32 plans.  An FP32 transform of a line that holds a strong peak leaves
+    // errors of order eps |A_peak| on its other outputs (the rounding inside the butterflies
+    // that carry the peak, whatever the radix order, twiddle precision or decimation), and a
+    // voxel at 1e-10 of the peak has |A| = 1e-5 |A_peak|: FP32 arithmetic gives |dlog10 I| of
+    // 1e-3 and more there (measured 0.9e-3 to 1.8e-3 on lines of 20 to 2000 points, NumPy
+    // emulation of the FP32 passes and the GPU results), and where the lines of a volume are
+    // alike these errors repeat from line to line and are not averaged down by the later
+    // transforms.  So the transforms run in FP64.  The spectra between passes stay FP32: the
+    // rounding of a stored value is relative to that value, so where lines repeat it follows
+    // the signal, and otherwise the FP64 transforms after it spread it out: pass 1's along the
+    // last axis, kLineErr / sqrt(N_last) (none when N_last = 1: the stored values are then the
+    // amplitudes), and pass 0's over the strided plane, kPlaneErr / sqrt(N1 N2) (none when the
+    // plane is one cell).  The constants are the worst cases measured per term (NumPy
+    // emulation of FP32 storage with FP64 transforms).  A plan whose estimate exceeds kBound,
+    // 0.6 of the 1e-3 target, stores its spectra in FP64 too.
+    constexpr double kLineErr = 5.8e-3, kPlaneErr = 3.5e-3, kBound = 6e-4;
+    auto spread = [](double k, double n) { return n > 1.0 ? k / std::sqrt(n) : 0.0; };
+    auto estimate = [&](int pd1) {
+        const double nl = (double)grid->dims[3 - pd1], n1 = (double)grid->dims[pd1];
+        return std::hypot(spread(kLineErr, nl), spread(kPlaneErr, n1 * nl));
+    };
+    // pass order: an order that keeps the source-group form, then the order of the smaller
+    // estimate (ties keep the memory order)
+    bool swap = order == QPD_ORDER_SWAPPED;
+    if (order == QPD_ORDER_AUTO) {
+        const bool g_mem = groups_with(1), g_swap = groups_with(2);
+        swap = g_mem != g_swap ? g_swap : estimate(2) < estimate(1);
+    }
+    p->pd[1] = swap ? 2 : 1;
+    p->pd[2] = swap ? 1 : 2;
+    for (int d = 0; d < 3; ++d) {
+        const int64_t N = grid->dims[p->pd[d]];
+        if (!make_axis(N, p->fft[d])) {
             delete p;
-            return fail(QPD_ERR_UNSUPPORTED, "transform length " + std::to_string(grid->dims[d]) +
+            return fail(QPD_ERR_UNSUPPORTED, "transform length " + std::to_string(N) +
                                                  " has a prime factor above 17 or is too long");
         }
     }
+    if (precision == 32) {
+        int a = arith;
+        const double est = estimate(p->pd[1]);
+        if (a == QPD_ARITH_AUTO) a = est <= kBound ? QPD_ARITH_MIXED : QPD_ARITH_FP64;
+        p->mixed = a == QPD_ARITH_MIXED ? 1 : 0;
+        p->wide = a == QPD_ARITH_FP64 ? 1 : 0;
+        // an explicit FP32 plan has no bound: its FP32 transforms leave about 1e-3 at voxels
+        // 1e-10 of the peak on lines that carry a strong one
+        p->est_dlog10 = p->wide ? 0.0 : p->mixed ? est : -1.0;
+    }
+    const bool f64 = precision == 64 || p->wide || p->mixed;
+    for (int d = 0; d < 3; ++d) p->csz[d] = f64 ? sizeof(cx<double>) : sizeof(cx<float>);
+    p->ssz = (precision == 64 || p->wide) ? sizeof(cx<double>) : sizeof(cx<float>);
     p->n_phase = c->n_phase;
     p->n_atom = c->n_atom;
-    p->csize = precision == 32 ? sizeof(cx<float>) : sizeof(cx<double>);
     for (int i = 0; i < 3; ++i) p->qC[i] = c->qC[i];
     const int nin[3] = {4, 3, 2};
     classify(p, grid, c);
-    // source groups: when they are fewer than the y-classes, pass 0 transforms one
-    // representative per group and pass 1 mixes them (one or two z-classes; compile-time
-    // FP32 kernels or the run-time-radix pass_strided_mix; QPD_CLASS_FORM=1 keeps the class
-    // form for comparison runs)
+    // pass 1 mixes the groups with the compile-time kernels or the run-time-radix
+    // pass_strided_mix
     const int mix_role = p->nZ == 1 ? kRoleMix1 : (p->nZ == 2 ? kRoleMix2 : -1);
-    const char* cf = std::getenv("QPD_CLASS_FORM");
-    p->grouped = (mix_role >= 0 && p->nS < p->nY && !(cf && cf[0] == '1')) ? 1 : 0;
+    p->grouped = (mix_role >= 0 && p->nS < p->nY && !class_form) ? 1 : 0;
+    const int tab_entries = c->n_atom * c->n_phase * kAtomTab;
     int rc = QPD_OK;
     for (int d = 0; d < 3 && rc == QPD_OK; ++d) {
-        rc = pass_candidates(p->fft[d], d, p->dims, nin[d], p->csize, true, c->n_atom * c->n_phase * kAtomTab, true,
+        const bool io32 = p->mixed != 0;
+        rc = pass_candidates(p->fft[d], d, p->pd[d], p->dims, nin[d], p->csz[d], io32, true, tab_entries, true,
                              p->cand[d], p->grouped ? mix_role : -1);
         if (d == 1 && p->grouped && rc != QPD_OK) {
             // no source-group kernel (direct or split) for this length: keep the class form
             g_error.clear();
             p->grouped = 0;
-            rc = pass_candidates(p->fft[d], d, p->dims, nin[d], p->csize, true, c->n_atom * c->n_phase * kAtomTab,
-                                 true, p->cand[d], -1);
+            rc = pass_candidates(p->fft[d], d, p->pd[d], p->dims, nin[d], p->csz[d], io32, true, tab_entries, true,
+                                 p->cand[d], -1);
             // pass 0 of the class form has no one-wave-per-row schedule
             std::vector<PassPlan>& c0 = p->cand[0];
             c0.erase(std::remove_if(c0.begin(), c0.end(),
@@ -1112,9 +1289,11 @@ int qpd_plan_create(qpd_plan** plan, const qpd_grid* grid, const qpd_crystal* c,
             p->pass[d] = p->cand[d][0];
         }
     }
-    if (rc == QPD_OK) rc = precision == 32 ? build_tables<float>(p, grid, c) : build_tables<double>(p, grid, c);
+    if (rc == QPD_OK && p->csz[0] == 8) rc = build_tables<float>(p, grid, c, p->tab[0]);
+    if (rc == QPD_OK && p->csz[0] == 16) rc = build_tables<double>(p, grid, c, p->tab[1]);
     if (rc != QPD_OK) {
-        if (p->dev) (void)hipFree(p->dev);
+        for (PlanTables& t : p->tab)
+            if (t.dev) (void)hipFree(t.dev);
         delete p;
         return rc;
     }
@@ -1122,19 +1301,28 @@ int qpd_plan_create(qpd_plan** plan, const qpd_grid* grid, const qpd_crystal* c,
     return QPD_OK;
 }
 
+int qpd_plan_info(const qpd_plan* p, int pass_dims[3], int* compute_bits, int* storage_bits, double* dlog10_estimate) {
+    if (!p) return fail(QPD_ERR_ARG, "null plan");
+    if (pass_dims)
+        for (int d = 0; d < 3; ++d) pass_dims[d] = p->pd[d];
+    if (compute_bits) *compute_bits = p->csz[0] == 16 ? 64 : 32;
+    if (storage_bits) *storage_bits = p->ssz == 16 ? 64 : 32;
+    if (dlog10_estimate) *dlog10_estimate = p->est_dlog10;
+    return QPD_OK;
+}
+
 int qpd_plan_destroy(qpd_plan* plan) {
     if (!plan) return QPD_OK;
-    if (plan->dev) {
-        (void)hipDeviceSynchronize();
-        (void)hipFree(plan->dev);
-    }
+    if (plan->tab[0].dev || plan->tab[1].dev) (void)hipDeviceSynchronize();
+    for (PlanTables& t : plan->tab)
+        if (t.dev) (void)hipFree(t.dev);
     delete plan;
     return QPD_OK;
 }
 
 size_t qpd_workspace_bytes(const qpd_plan* p) {
     if (!p) return 0;
-    return (size_t)(p->n1fields() + 2 * p->nZ) * (size_t)p->ncell * p->csize;
+    return p->w1_bytes() + p->w2_bytes() + p->wide_bytes();
 }
 
 
@@ -1290,21 +1478,22 @@ int qpd_intensity_pass(qpd_plan* p, const qpd_fields* f, double* I, void* work, 
     if (!f->P || !f->U || !f->phase) return fail(QPD_ERR_ARG, "P, U and phase are required");
     if (pass < 0 || pass > 2) return fail(QPD_ERR_ARG, "pass must be 0, 1 or 2");
     hipStream_t st = static_cast<hipStream_t>(stream);
-    return p->precision == 32 ? run_intensity<float>(p, f, I, work, st, pass)
-                              : run_intensity<double>(p, f, I, work, st, pass);
+    return run_intensity(p, f, I, work, st, pass);
 }
 
 double qpd_pass_bytes_per_cell(const qpd_plan* p, const qpd_fields* f, int pass) {
     if (!p || pass < 0 || pass > 2) return 0.0;
-    const double real = p->precision == 32 ? 4.0 : 8.0, c = 2.0 * real;
+    // a wide plan reads each FP32 input, writes it widened and reads that in pass 0
+    const double real = p->precision == 32 ? (p->wide ? 4.0 + 8.0 + 8.0 : 4.0) : 8.0;
+    const double c0 = (double)p->ssz, c1 = (double)p->ssz;
     if (pass == 0) {
         double in = 3 * real + 3 * 8.0 + 1.0;  // P, U, phase
         if (f && f->region) in += real;
         if (f && f->fraction) in += p->n_phase * real;
-        return in + (double)p->n1fields() * c;
+        return in + (double)p->n1fields() * c0;
     }
-    if (pass == 1) return (double)p->n1fields() * c + 2.0 * p->nZ * c;
-    return 2.0 * p->nZ * c + 8.0;
+    if (pass == 1) return (double)p->n1fields() * c0 + 2.0 * p->nZ * c1;
+    return 2.0 * p->nZ * c1 + 8.0;
 }
 
 int qpd_fft3d(const int64_t dims[3], int precision, void* data, int n_fields, void* stream) {
@@ -1324,7 +1513,7 @@ int qpd_intensity(qpd_plan* p, const qpd_fields* f, double* I, void* work, void*
     if (!p || !f || !I || !work) return fail(QPD_ERR_ARG, "null argument");
     if (!f->P || !f->U || !f->phase) return fail(QPD_ERR_ARG, "P, U and phase are required");
     hipStream_t st = static_cast<hipStream_t>(stream);
-    return p->precision == 32 ? run_intensity<float>(p, f, I, work, st) : run_intensity<double>(p, f, I, work, st);
+    return run_intensity(p, f, I, work, st);
 }
 
 }  // extern "C"

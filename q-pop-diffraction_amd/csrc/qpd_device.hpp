@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace qpd {
 
 template <typename T>
@@ -26,6 +28,15 @@ __device__ __forceinline__ cx<T> cscale(cx<T> a, T s) { return {a.x * s, a.y * s
 // -i * a
 template <typename T>
 __device__ __forceinline__ cx<T> negi(cx<T> a) { return {a.y, -a.x}; }
+// a complex value converted to element type T (exact when T is at least as wide)
+template <typename T, typename S>
+__device__ __forceinline__ cx<T> cx_as(cx<S> a) { return {(T)a.x, (T)a.y}; }
+
+// Element type of the arrays an FP64 kernel reads and writes: with IO32 it computes in FP64
+// and keeps its complex fields, P, the phase fractions and the region weight in FP32 (an FP32
+// plan whose transforms run in FP64).
+template <typename T, bool IO32>
+using io_t = std::conditional_t<IO32, float, T>;
 
 // Division by a run-time constant d: q = floor(n / d) for n * d < 2^32.
 // m = floor((2^32 - 1) / d) + 1, q = umulhi(n, m); d == 1 is flagged by m == 0.

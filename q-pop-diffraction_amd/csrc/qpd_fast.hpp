@@ -530,8 +530,10 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_contig(PassArgs<T> a) {
 // in natural order; then thread (f, row, j) transforms line (f, row) and keeps the
 // contribution of input f to its output: f = 0 and f = 1 (times -i k) both feed
 // output 0 and are summed through LDS once per group; f = 2, 3 feed outputs 1, 2.
-template <typename T, class SZ, int G>
+// IO32 (FP64 only): FP64 arithmetic on FP32 P, weights and outputs.
+template <typename T, class SZ, int G, bool IO32 = false>
 __global__ __launch_bounds__(4 * G * SZ::TPL) void fast_source(PassArgs<T> a) {
+    using TS = io_t<T, IO32>;
     constexpr int N = SZ::N, TPL = SZ::TPL, NRL = SZ::NRL, RL = SZ::RL;
     constexpr int NTHR = 4 * G * TPL;
     constexpr int CPT = (G * N + NTHR - 1) / NTHR;  // cells per thread in the source phase
@@ -566,7 +568,7 @@ __global__ __launch_bounds__(4 * G * SZ::TPL) void fast_source(PassArgs<T> a) {
         if constexpr (F32)
             cell_load_f(a, cell0 + e, valid, ci[m]);
         else
-            cell_load(a, cell0 + e, valid, ci[m]);
+            cell_load<T, IO32>(a, cell0 + e, valid, ci[m]);
     }
     lds_barrier();
 
@@ -589,7 +591,7 @@ __global__ __launch_bounds__(4 * G * SZ::TPL) void fast_source(PassArgs<T> a) {
                     if constexpr (F32)
                         cell_source_f(a, tabl, ci[m], cell0 + e, src, v0, v1, v2, v3);
                     else
-                        cell_source(a, tabl, ci[m], cell0 + e, src, v0, v1, v2, v3);
+                        cell_source<T, IO32>(a, tabl, ci[m], cell0 + e, src, v0, v1, v2, v3);
                     const int rr = e / N, x = e - rr * N;
                     X[lds_addr<false, SZ, 0>(0 * G + rr, x)] = v0;
                     X[lds_addr<false, SZ, 0>(1 * G + rr, x)] = v1;
@@ -620,12 +622,12 @@ __global__ __launch_bounds__(4 * G * SZ::TPL) void fast_source(PassArgs<T> a) {
         lds_barrier();
         if (row < nrows && j < NRL && f != 1) {
             const int q = f == 0 ? 0 : f - 1;
-            cx<T>* o = a.out + ((size_t)grp * 3 + q) * a.ncell + cell0 + (uint32_t)g * N + j;
+            cx<TS>* o = reinterpret_cast<cx<TS>*>(a.out) + ((size_t)grp * 3 + q) * a.ncell + cell0 + (uint32_t)g * N + j;
 #pragma unroll
             for (int r = 0; r < RL; ++r) {
                 cx<T> val = acc[r];
                 if (f == 0) val = val + ex[lds_addr<false, SZ, 0>(g, j + r * NRL)];
-                o[r * NRL] = val;
+                o[r * NRL] = cx_as<TS>(val);
             }
         }
         lds_barrier();
@@ -760,10 +762,12 @@ __device__ __forceinline__ void wave_line_transform(cx<T>* line, cx<T> (&v)[SZ::
 // a radix-11, 13 or 17 stage, whose DFTs need more registers, run two waves per SIMD instead
 // of spilling; FP64 runs one, whose 512 registers (AGPRs included) hold a pair of groups'
 // e and u_d without spills (at two waves it spilled 132 to 340 bytes a lane).
-template <typename T, class SZ, int ROWS>
+// IO32 (FP64 only): FP64 arithmetic on FP32 P, weights and outputs.
+template <typename T, class SZ, int ROWS, bool IO32 = false>
 __global__ __launch_bounds__(64 * kSrcWaves)
 __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 1 : (SZ::RMAX >= 11 ? 2 : 3)))) void fast_source_wave(PassArgs<T> a) {
     constexpr bool F32 = std::is_same<T, float>::value;
+    using TS = io_t<T, IO32>;
     constexpr int N = SZ::N, TPL = SZ::TPL, R0 = SZ::rad(0), NR0 = SZ::nr(0), RL = SZ::RL, NRL = SZ::NRL;
     constexpr int NTHR = 64 * kSrcWaves;
     constexpr int NTW = StageTw<SZ>::total();
@@ -804,32 +808,32 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 1 : (SZ::RMAX >= 11 ? 2 : 3)
     // fractional phases: transform group grp's four fields held in v (F[E] waits in
     // registers for F[U_d0]) and store A, B, C
     auto emit = [&](cx<T>(&v)[4][SZ::RMAX], int grp) {
-        cx<T>* oA = a.out + (size_t)grp * 3 * a.ncell + (size_t)cell0 + (size_t)j;
-        cx<T>* oB = oA + a.ncell;
-        cx<T>* oC = oB + a.ncell;
+        cx<TS>* oA = reinterpret_cast<cx<TS>*>(a.out) + (size_t)grp * 3 * a.ncell + (size_t)cell0 + (size_t)j;
+        cx<TS>* oB = oA + a.ncell;
+        cx<TS>* oC = oB + a.ncell;
         wave_line_transform<T, SZ>(line, v[0], j, inrow, tw);
         wave_line_transform<T, SZ>(line, v[1], j, inrow, tw);
         if (live && j < NRL) {
 #pragma unroll
-            for (int r = 0; r < RL; ++r) oA[r * NRL] = v[0][r] + cscale(negi(v[1][r]), kj[r * NRL]);
+            for (int r = 0; r < RL; ++r) oA[r * NRL] = cx_as<TS>(v[0][r] + cscale(negi(v[1][r]), kj[r * NRL]));
         }
         wave_line_transform<T, SZ>(line, v[2], j, inrow, tw);
         if (live && j < NRL) {
 #pragma unroll
-            for (int r = 0; r < RL; ++r) oB[r * NRL] = v[2][r];
+            for (int r = 0; r < RL; ++r) oB[r * NRL] = cx_as<TS>(v[2][r]);
         }
         wave_line_transform<T, SZ>(line, v[3], j, inrow, tw);
         if (live && j < NRL) {
 #pragma unroll
-            for (int r = 0; r < RL; ++r) oC[r * NRL] = v[3][r];
+            for (int r = 0; r < RL; ++r) oC[r * NRL] = cx_as<TS>(v[3][r]);
         }
     };
     // one-hot phases: a group's fields are e and e u_d; each field e u_d is formed just
     // before its transform, and F[e u_0] waits for F[e], which is transformed last, in place
     auto emit_eu = [&](cx<T>(&e)[SZ::RMAX], T(&u)[3][R0], int grp, uint32_t lane_cell) {
-        cx<T>* oA = a.out + (size_t)grp * 3 * a.ncell + (size_t)lane_cell;
-        cx<T>* oB = oA + a.ncell;
-        cx<T>* oC = oB + a.ncell;
+        cx<TS>* oA = reinterpret_cast<cx<TS>*>(a.out) + (size_t)grp * 3 * a.ncell + (size_t)lane_cell;
+        cx<TS>* oB = oA + a.ncell;
+        cx<TS>* oC = oB + a.ncell;
         cx<T> w[SZ::RMAX], f0[RL];
 #pragma unroll
         for (int r = 0; r < R0; ++r) w[r] = cscale(e[r], u[0][r]);
@@ -841,19 +845,19 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 1 : (SZ::RMAX >= 11 ? 2 : 3)
         wave_line_transform<T, SZ>(line, w, j, inrow, tw);
         if (live && j < NRL) {
 #pragma unroll
-            for (int r = 0; r < RL; ++r) oB[r * NRL] = w[r];
+            for (int r = 0; r < RL; ++r) oB[r * NRL] = cx_as<TS>(w[r]);
         }
 #pragma unroll
         for (int r = 0; r < R0; ++r) w[r] = cscale(e[r], u[2][r]);
         wave_line_transform<T, SZ>(line, w, j, inrow, tw);
         if (live && j < NRL) {
 #pragma unroll
-            for (int r = 0; r < RL; ++r) oC[r * NRL] = w[r];
+            for (int r = 0; r < RL; ++r) oC[r * NRL] = cx_as<TS>(w[r]);
         }
         wave_line_transform<T, SZ>(line, e, j, inrow, tw);
         if (live && j < NRL) {
 #pragma unroll
-            for (int r = 0; r < RL; ++r) oA[r * NRL] = e[r] + cscale(negi(f0[r]), kj[r * NRL]);
+            for (int r = 0; r < RL; ++r) oA[r * NRL] = cx_as<TS>(e[r] + cscale(negi(f0[r]), kj[r * NRL]));
         }
     };
     if constexpr (!F32) {
@@ -867,7 +871,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 1 : (SZ::RMAX >= 11 ? 2 : 3)
             constexpr int kAhead = R0 < 2 ? R0 : 2;
             CellInputs<T> ci[R0];
 #pragma unroll
-            for (int r = 0; r < kAhead; ++r) cell_load(a, cell0 + (uint32_t)(j + r * NR0), has0, ci[r]);
+            for (int r = 0; r < kAhead; ++r) cell_load<T, IO32>(a, cell0 + (uint32_t)(j + r * NR0), has0, ci[r]);
             if (!a.fraction) {
                 // a pair of groups from one read of the cells
                 const bool pair = grp + 1 < a.ngroups;
@@ -877,7 +881,8 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 1 : (SZ::RMAX >= 11 ? 2 : 3)
 #pragma unroll
                 for (int r = 0; r < R0; ++r) {
                     __builtin_amdgcn_sched_barrier(0);
-                    if (r + kAhead < R0) cell_load(a, cell0 + (uint32_t)(j + (r + kAhead) * NR0), has0, ci[r + kAhead]);
+                    if (r + kAhead < R0)
+                        cell_load<T, IO32>(a, cell0 + (uint32_t)(j + (r + kAhead) * NR0), has0, ci[r + kAhead]);
                     cell_source_eu(a, tabl, ci[r], src, eA[r], uA[0][r], uA[1][r], uA[2][r]);
                     if (pair) cell_source_eu(a, tabl, ci[r], srcB, eB[r], uB[0][r], uB[1][r], uB[2][r]);
                 }
@@ -890,8 +895,10 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 1 : (SZ::RMAX >= 11 ? 2 : 3)
 #pragma unroll
                 for (int r = 0; r < R0; ++r) {
                     __builtin_amdgcn_sched_barrier(0);
-                    if (r + kAhead < R0) cell_load(a, cell0 + (uint32_t)(j + (r + kAhead) * NR0), has0, ci[r + kAhead]);
-                    cell_source(a, tabl, ci[r], cell0 + (uint32_t)(j + r * NR0), src, v[0][r], v[1][r], v[2][r], v[3][r]);
+                    if (r + kAhead < R0)
+                        cell_load<T, IO32>(a, cell0 + (uint32_t)(j + (r + kAhead) * NR0), has0, ci[r + kAhead]);
+                    cell_source<T, IO32>(a, tabl, ci[r], cell0 + (uint32_t)(j + r * NR0), src, v[0][r], v[1][r], v[2][r],
+                                         v[3][r]);
                 }
                 __builtin_amdgcn_sched_barrier(0);
                 emit(v, grp);
@@ -966,8 +973,10 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 1 : (SZ::RMAX >= 11 ? 2 : 3)
 // Rows follow the split addressing of PassArgs (plain passes: nq = 1, q = 0, unit steps).
 // PF: the next input is prefetched into registers while the current one is transformed;
 // without it the kernel needs fewer registers and more blocks fit on a CU.
-template <typename T, class SZ, int G, int NIN, bool INTENSITY, bool SPLIT, bool PF = true>
+// IO32 (FP64 only): FP64 arithmetic on complex FP32 inputs and outputs.
+template <typename T, class SZ, int G, int NIN, bool INTENSITY, bool SPLIT, bool PF = true, bool IO32 = false>
 __global__ __launch_bounds__(G* SZ::TPL) void fast_strided(PassArgs<T> a) {
+    using TI = io_t<T, IO32>;
     // the NIN inputs of a chunk are transformed one after another (one line per thread)
     constexpr int N = SZ::N, R0 = SZ::rad(0), NR0 = SZ::nr(0), LP = SZ::strided_lp(G);
     constexpr int NOUT = NIN > 1 ? NIN - 1 : 1;
@@ -1000,7 +1009,7 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided(PassArgs<T> a) {
     const uint32_t jw = (uint32_t)__builtin_amdgcn_readfirstlane(tid / 64) * (64 / G);
     const uint32_t jl = (uint32_t)(tid % 64) / G;
     const size_t tile0 = (size_t)t0 * G + (size_t)outer * a.OS;
-    const uint32_t in_lane = ((col_ok ? i0 : a.n0 - 1) - t0 * G + jl * in_step * a.S) * (uint32_t)sizeof(cx<T>);
+    const uint32_t in_lane = ((col_ok ? i0 : a.n0 - 1) - t0 * G + jl * in_step * a.S) * (uint32_t)sizeof(cx<TI>);
     const uint32_t c_lane = jl * c_step;
     load_twiddles<T, N>(Wl, a.W, tid, G * SZ::TPL);
     lds_barrier();
@@ -1008,11 +1017,11 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided(PassArgs<T> a) {
     const int nitems = a.group_first[a.ngroups] * NIN;
     auto load_item = [&](int it, cx<T>(&dst)[1][SZ::RMAX]) {
         const int c = it / NIN, f = it - (it / NIN) * NIN;
-        const cx<T>* in = a.in + ((size_t)a.chunk_src[c] * a.nin + f) * a.ncell + tile0;
+        const cx<TI>* in = reinterpret_cast<const cx<TI>*>(a.in) + ((size_t)a.chunk_src[c] * a.nin + f) * a.ncell + tile0;
         if (j < NR0) {
 #pragma unroll
             for (int r = 0; r < R0; ++r)
-                dst[0][r] = load_at(in + (size_t)((in0 + (jw + r * NR0) * in_step) * a.S), in_lane);
+                dst[0][r] = cx_as<T>(load_at(in + (size_t)((in0 + (jw + r * NR0) * in_step) * a.S), in_lane));
         }
     };
     cx<T> nxt[1][SZ::RMAX];
@@ -1072,14 +1081,14 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided(PassArgs<T> a) {
                 a.I[(size_t)s0 + (size_t)a.n0 * ((size_t)s1 + (size_t)a.n1 * s2)] = fmax(re * re + im * im, 1e-50);
             }
         } else {
-            cx<T>* o = a.out + (size_t)grp * NOUT * a.ncell + tile0;
-            const uint32_t out_lane = (g + jl * out_step * a.S) * (uint32_t)sizeof(cx<T>);
+            cx<TI>* o = reinterpret_cast<cx<TI>*>(a.out) + (size_t)grp * NOUT * a.ncell + tile0;
+            const uint32_t out_lane = (g + jl * out_step * a.S) * (uint32_t)sizeof(cx<TI>);
 #pragma unroll
             for (int qq = 0; qq < NOUT; ++qq)
 #pragma unroll
                 for (int r = 0; r < SZ::RL; ++r)
                     store_at(o + (size_t)qq * a.ncell + (size_t)((out0 + (jw + r * SZ::NRL) * out_step) * a.S), out_lane,
-                             acc[qq][r]);
+                             cx_as<TI>(acc[qq][r]));
         }
     }
 }
@@ -1095,8 +1104,10 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided(PassArgs<T> a) {
 //   out_{Z,0} = sum_S sum_{Y in Z} c_Y(k1) X_{Y,S}(k0) [F1 A_S - i k1 F1 B_S],
 //   out_{Z,1} = sum_S sum_{Y in Z} c_Y(k1) X_{Y,S}(k0) F1 C_S,
 // the same outputs as the class form.  Each group's three inputs are transformed once.
-template <typename T, class SZ, int G, int NZ, bool SPLIT>
+// IO32 (FP64 only): FP64 arithmetic on complex FP32 inputs and outputs.
+template <typename T, class SZ, int G, int NZ, bool SPLIT, bool IO32 = false>
 __global__ __launch_bounds__(G* SZ::TPL) void fast_strided_mix(PassArgs<T> a) {
+    using TI = io_t<T, IO32>;
     constexpr int N = SZ::N, R0 = SZ::rad(0), NR0 = SZ::nr(0), LP = SZ::strided_lp(G), RL = SZ::RL, NRL = SZ::NRL;
     constexpr int TILE = SZ::strided_tile(G);
     constexpr bool ALT = (chunk_writes<SZ, false>() & 1) == 1;
@@ -1122,18 +1133,18 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided_mix(PassArgs<T> a) {
     const uint32_t jw = (uint32_t)__builtin_amdgcn_readfirstlane(tid / 64) * (64 / G);
     const uint32_t jl = (uint32_t)(tid % 64) / G;
     const size_t tile0 = (size_t)t0 * G + (size_t)outer * a.OS;
-    const uint32_t in_lane = ((col_ok ? i0 : a.n0 - 1) - t0 * G + jl * in_step * a.S) * (uint32_t)sizeof(cx<T>);
+    const uint32_t in_lane = ((col_ok ? i0 : a.n0 - 1) - t0 * G + jl * in_step * a.S) * (uint32_t)sizeof(cx<TI>);
     const uint32_t c_lane = jl * c_step;
     load_twiddles<T, N>(Wl, a.W, tid, G * SZ::TPL);
     lds_barrier();
     constexpr bool PF = true;
     const int nitems = 3 * a.nsrc;
     auto load_item = [&](int it, cx<T>(&dst)[1][SZ::RMAX]) {
-        const cx<T>* in = a.in + (size_t)it * a.ncell + tile0;
+        const cx<TI>* in = reinterpret_cast<const cx<TI>*>(a.in) + (size_t)it * a.ncell + tile0;
         if (j < NR0) {
 #pragma unroll
             for (int r = 0; r < R0; ++r)
-                dst[0][r] = load_at(in + (size_t)((in0 + (jw + r * NR0) * in_step) * a.S), in_lane);
+                dst[0][r] = cx_as<T>(load_at(in + (size_t)((in0 + (jw + r * NR0) * in_step) * a.S), in_lane));
         }
     };
     cx<T> acc[NZ][2][RL];
@@ -1192,15 +1203,15 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided_mix(PassArgs<T> a) {
         }
     }
     if (!col_ok || j >= NRL) return;
-    const uint32_t out_lane = (g + jl * out_step * a.S) * (uint32_t)sizeof(cx<T>);
+    const uint32_t out_lane = (g + jl * out_step * a.S) * (uint32_t)sizeof(cx<TI>);
 #pragma unroll
     for (int z = 0; z < NZ; ++z)
 #pragma unroll
         for (int oq = 0; oq < 2; ++oq) {
-            cx<T>* o = a.out + (size_t)(2 * z + oq) * a.ncell + tile0;
+            cx<TI>* o = reinterpret_cast<cx<TI>*>(a.out) + (size_t)(2 * z + oq) * a.ncell + tile0;
 #pragma unroll
             for (int r = 0; r < RL; ++r)
-                store_at(o + (size_t)((out0 + (jw + r * NRL) * out_step) * a.S), out_lane, acc[z][oq][r]);
+                store_at(o + (size_t)((out0 + (jw + r * NRL) * out_step) * a.S), out_lane, cx_as<TI>(acc[z][oq][r]));
         }
 }
 

@@ -64,23 +64,25 @@ void add_fast(std::vector<FastEntry>& v, const char* radices) {
 // complex FP64.  A thread holds the same points as in FP32 at twice the registers, so the
 // strided tiles take half the columns (rows of the same bytes), and every block has at most
 // 256 threads: a wave may then use 512 registers (AGPRs included) instead of spilling.
-template <class SZ, int ROLE, int G, bool PF = true>
+// IO32: the same kernels computing in FP64 on FP32 arrays (P, weights, the spectra between
+// passes), for FP32 plans whose transforms run in FP64.
+template <class SZ, int ROLE, int G, bool PF = true, bool IO32 = false>
 hipError_t fast_launch64(const Launch& L, const PassArgs<double>& a, hipStream_t st) {
     if constexpr (ROLE == kRoleSource)
-        return launch_one(fast_source<double, SZ, G>, L, st, a);
+        return launch_one(fast_source<double, SZ, G, IO32>, L, st, a);
     else if constexpr (ROLE == kRoleSourceW)
-        return launch_one(fast_source_wave<double, SZ, G / kSrcWaves>, L, st, a);
+        return launch_one(fast_source_wave<double, SZ, G / kSrcWaves, IO32>, L, st, a);
     else if constexpr (ROLE == kRoleStrided)
-        return launch_one(fast_strided<double, SZ, G, 3, false, false>, L, st, a);
+        return launch_one(fast_strided<double, SZ, G, 3, false, false, true, IO32>, L, st, a);
     else if constexpr (ROLE == kRoleIntensity)
-        return launch_one(fast_strided<double, SZ, G, 2, true, false, PF>, L, st, a);
+        return launch_one(fast_strided<double, SZ, G, 2, true, false, PF, IO32>, L, st, a);
     else if constexpr (ROLE == kRoleMix1)
-        return launch_one(fast_strided_mix<double, SZ, G, 1, false>, L, st, a);
+        return launch_one(fast_strided_mix<double, SZ, G, 1, false, IO32>, L, st, a);
     else
-        return launch_one(fast_strided_mix<double, SZ, G, 2, false>, L, st, a);
+        return launch_one(fast_strided_mix<double, SZ, G, 2, false, IO32>, L, st, a);
 }
 
-template <class SZ, int ROLE, int G, bool PF = true>
+template <class SZ, int ROLE, int G, bool PF = true, bool IO32 = false>
 void add_fast64(std::vector<FastEntry>& v, const char* radices) {
     constexpr bool src = ROLE == kRoleSource;
     if constexpr (G >= (src ? 1 : 4) && (src ? 4 : 1) * G * SZ::TPL <= 256) {
@@ -88,20 +90,27 @@ void add_fast64(std::vector<FastEntry>& v, const char* radices) {
         constexpr size_t lds = (2 * tile + SZ::N) * sizeof(cx<double>);
         if constexpr (lds <= 160 * 1024)
             v.push_back({SZ::N, ROLE, G, (src ? 4 : 1) * G * SZ::TPL, SZ::TPL, lds, radices, nullptr, nullptr, !PF,
-                         8, &fast_launch64<SZ, ROLE, G, PF>});
+                         8, &fast_launch64<SZ, ROLE, G, PF, IO32>, IO32});
     }
 }
 
+template <class SZ, int ROLE, bool IO32>
+void add_fast64_tiles(std::vector<FastEntry>& v, const char* radices) {
+    constexpr int G = fast_tile(ROLE, SZ::TPL) / (ROLE == kRoleSource ? 1 : 2);
+    add_fast64<SZ, ROLE, G, true, IO32>(v, radices);
+    if constexpr (ROLE == kRoleIntensity) {
+        add_fast64<SZ, ROLE, G, false, IO32>(v, radices);
+        add_fast64<SZ, ROLE, G / 2, false, IO32>(v, radices);
+    } else if constexpr (ROLE != kRoleSource) {
+        add_fast64<SZ, ROLE, G / 2, true, IO32>(v, radices);
+    }
+}
+
+// FP64 plans, and FP32 plans computing in FP64 (FP32 arrays, IO32)
 template <class SZ, int ROLE>
 void add_fast64_variants(std::vector<FastEntry>& v, const char* radices) {
-    constexpr int G = fast_tile(ROLE, SZ::TPL) / (ROLE == kRoleSource ? 1 : 2);
-    add_fast64<SZ, ROLE, G>(v, radices);
-    if constexpr (ROLE == kRoleIntensity) {
-        add_fast64<SZ, ROLE, G, false>(v, radices);
-        add_fast64<SZ, ROLE, G / 2, false>(v, radices);
-    } else if constexpr (ROLE != kRoleSource) {
-        add_fast64<SZ, ROLE, G / 2>(v, radices);
-    }
+    add_fast64_tiles<SZ, ROLE, false>(v, radices);
+    add_fast64_tiles<SZ, ROLE, true>(v, radices);
 }
 
 // The one-wave-per-row source pass (lengths whose lines fit one wave and whose first
@@ -118,6 +127,9 @@ void add_source_wave(std::vector<FastEntry>& v, const char* radices) {
         v.push_back({SZ::N, kRoleSourceW, kSrcWaves * rows, 64 * kSrcWaves, SZ::TPL, cells * 16, radices, nullptr,
                      (const void*)fast_source_wave<double, SZ, rows>, false, 8,
                      &fast_launch64<SZ, kRoleSourceW, kSrcWaves * rows>});
+        v.push_back({SZ::N, kRoleSourceW, kSrcWaves * rows, 64 * kSrcWaves, SZ::TPL, cells * 16, radices, nullptr,
+                     (const void*)fast_source_wave<double, SZ, rows, true>, false, 8,
+                     &fast_launch64<SZ, kRoleSourceW, kSrcWaves * rows, true, true>, true});
     }
 }
 

@@ -114,24 +114,26 @@ struct CellInputs {
     int ph;
 };
 
-template <typename T>
+template <typename T, bool IO32 = false>
 __device__ __forceinline__ void cell_load(const PassArgs<T>& a, uint32_t off, bool valid, CellInputs<T>& c) {
+    using TR = io_t<T, IO32>;
     c.ph = -1;
     c.rg = (T)1;
     c.P0 = c.P1 = c.P2 = (T)0;
     c.U0 = c.U1 = c.U2 = 0.0;
     if (!valid) return;
+    const TR* P = reinterpret_cast<const TR*>(a.P);
     c.ph = a.phase[off];
-    if (a.region) c.rg = a.region[off];
-    c.P0 = a.P[off];
-    c.P1 = a.P[a.ncell + off];
-    c.P2 = a.P[(size_t)2 * a.ncell + off];
+    if (a.region) c.rg = (T)reinterpret_cast<const TR*>(a.region)[off];
+    c.P0 = (T)P[off];
+    c.P1 = (T)P[a.ncell + off];
+    c.P2 = (T)P[(size_t)2 * a.ncell + off];
     c.U0 = a.U[off];
     c.U1 = a.U[a.ncell + off];
     c.U2 = a.U[(size_t)2 * a.ncell + off];
 }
 
-template <typename T>
+template <typename T, bool IO32 = false>
 __device__ __forceinline__ void cell_source(const PassArgs<T>& a, const T* atomtab, const CellInputs<T>& ci,
                                             uint32_t off, int atom, cx<T>& E, cx<T>& V0, cx<T>& V1, cx<T>& V2) {
     E = V0 = V1 = V2 = {(T)0, (T)0};
@@ -140,7 +142,7 @@ __device__ __forceinline__ void cell_source(const PassArgs<T>& a, const T* atomt
     const int p_hi = a.fraction ? a.n_phase : ci.ph + 1;
     for (int p = p_lo; p < p_hi; ++p) {
         T w = ci.rg;
-        if (a.fraction) w *= a.fraction[(size_t)p * a.ncell + off];
+        if (a.fraction) w *= (T)reinterpret_cast<const io_t<T, IO32>*>(a.fraction)[(size_t)p * a.ncell + off];
         if (w == (T)0) continue;
         const T* tab = atomtab + (atom * a.n_phase + p) * kAtomTab;
         const double ux = ci.U0 + (double)(tab[2] * ci.P0 + tab[3] * ci.P1 + tab[4] * ci.P2);
@@ -209,8 +211,10 @@ __device__ __forceinline__ cx<T> chunk_coef(const PassArgs<T>& a, int ct, int id
 // axis has only radix-2/4/8 stages (fft_tile).
 // LDS tile: point idx of line (f, row) at tile[idx * (nf*nl) + f*nl + row].
 // Element e = tid + m*nthr of a field's tile is (row, idx) = divmod(e, N).
-template <typename T, int EPT, int NOUT, bool SOURCE, bool POW2 = false>
+// IO32: FP64 arithmetic on FP32 arrays (inputs, P and weights, outputs).
+template <typename T, int EPT, int NOUT, bool SOURCE, bool POW2 = false, bool IO32 = false>
 __global__ __launch_bounds__(512) void pass_contig(PassArgs<T> a) {
+    using TS = io_t<T, IO32>;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* tile = reinterpret_cast<cx<T>*>(smem_raw);
     const int tid = threadIdx.x, nthr = blockDim.x;
@@ -228,7 +232,7 @@ __global__ __launch_bounds__(512) void pass_contig(PassArgs<T> a) {
         for (int m = 0; m < EPT; ++m) {
             const int e = tid + m * nthr;
             const int row = (int)fdiv((uint32_t)e, a.divN);
-            cell_load(a, cell0 + (uint32_t)e, e < tileN && row0 + row < nrows, ci[m]);
+            cell_load<T, IO32>(a, cell0 + (uint32_t)e, e < tileN && row0 + row < nrows, ci[m]);
         }
     }
 
@@ -244,7 +248,8 @@ __global__ __launch_bounds__(512) void pass_contig(PassArgs<T> a) {
             const int ct = a.chunk_coef[c];
             for (int j0 = 0; j0 < a.nin; j0 += a.nf) {
                 const int nfc = min(a.nf, a.nin - j0);
-                const cx<T>* inbase = SOURCE ? nullptr : a.in + ((size_t)src * a.nin + j0) * a.ncell + cell0;
+                const cx<TS>* inbase =
+                    SOURCE ? nullptr : reinterpret_cast<const cx<TS>*>(a.in) + ((size_t)src * a.nin + j0) * a.ncell + cell0;
 #pragma unroll
                 for (int m = 0; m < EPT; ++m) {
                     const int e = tid + m * nthr;
@@ -254,7 +259,7 @@ __global__ __launch_bounds__(512) void pass_contig(PassArgs<T> a) {
                     cx<T>* dst = tile + idx * stride + row;
                     if constexpr (SOURCE) {
                         cx<T> v0, v1, v2, v3;
-                        cell_source(a, a.atomtab, ci[m], cell0 + (uint32_t)e, src, v0, v1, v2, v3);
+                        cell_source<T, IO32>(a, a.atomtab, ci[m], cell0 + (uint32_t)e, src, v0, v1, v2, v3);
                         for (int f = 0; f < nfc; ++f) {
                             const int jj = j0 + f;
                             dst[f * nl] = jj == 0 ? v0 : (jj == 1 ? v1 : (jj == 2 ? v2 : v3));
@@ -262,7 +267,7 @@ __global__ __launch_bounds__(512) void pass_contig(PassArgs<T> a) {
                     } else {
                         const bool valid = row0 + row < nrows;
                         for (int f = 0; f < nfc; ++f)
-                            dst[f * nl] = valid ? inbase[(size_t)f * a.ncell + e] : cx<T>{(T)0, (T)0};
+                            dst[f * nl] = valid ? cx_as<T>(inbase[(size_t)f * a.ncell + e]) : cx<T>{(T)0, (T)0};
                     }
                 }
                 __syncthreads();
@@ -281,7 +286,7 @@ __global__ __launch_bounds__(512) void pass_contig(PassArgs<T> a) {
                 __syncthreads();
             }
         }
-        cx<T>* obase = a.out + (size_t)g * NOUT * a.ncell + cell0;
+        cx<TS>* obase = reinterpret_cast<cx<TS>*>(a.out) + (size_t)g * NOUT * a.ncell + cell0;
 #pragma unroll
         for (int m = 0; m < EPT; ++m) {
             const int e = tid + m * nthr;
@@ -289,7 +294,7 @@ __global__ __launch_bounds__(512) void pass_contig(PassArgs<T> a) {
             const int row = (int)fdiv((uint32_t)e, a.divN);
             if (row0 + row >= nrows) continue;
 #pragma unroll
-            for (int q = 0; q < NOUT; ++q) obase[(size_t)q * a.ncell + e] = acc[m][q];
+            for (int q = 0; q < NOUT; ++q) obase[(size_t)q * a.ncell + e] = cx_as<TS>(acc[m][q]);
         }
     }
 }
@@ -299,8 +304,10 @@ __global__ __launch_bounds__(512) void pass_contig(PassArgs<T> a) {
 // LDS tile: point idx of line (f, tx) at tile[idx * (nf*TX) + f*TX + tx].  A thread
 // keeps one column tx = tid % TX and the points idx = tid / TX + m * (nthr / TX).
 // INTENSITY: the single output is the amplitude; write fftshifted max(|A|^2, 1e-50).
-template <typename T, int EPT, int NOUT, bool INTENSITY, bool POW2 = false>
+// IO32: FP64 arithmetic on complex FP32 inputs and outputs.
+template <typename T, int EPT, int NOUT, bool INTENSITY, bool POW2 = false, bool IO32 = false>
 __global__ __launch_bounds__(512) void pass_strided(PassArgs<T> a) {
+    using TI = io_t<T, IO32>;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* tile = reinterpret_cast<cx<T>*>(smem_raw);
     const int tid = threadIdx.x, nthr = blockDim.x;
@@ -328,7 +335,7 @@ __global__ __launch_bounds__(512) void pass_strided(PassArgs<T> a) {
             const int ct = a.chunk_coef[c];
             for (int j0 = 0; j0 < a.nin; j0 += a.nf) {
                 const int nfc = min(a.nf, a.nin - j0);
-                const cx<T>* inbase = a.in + ((size_t)src * a.nin + j0) * a.ncell;
+                const cx<TI>* inbase = reinterpret_cast<const cx<TI>*>(a.in) + ((size_t)src * a.nin + j0) * a.ncell;
 #pragma unroll
                 for (int m = 0; m < EPT; ++m) {
                     const int idx = r0 + m * rstep;
@@ -336,7 +343,7 @@ __global__ __launch_bounds__(512) void pass_strided(PassArgs<T> a) {
                     const uint32_t off = colbase + (uint32_t)idx * a.S;
                     cx<T>* dst = tile + idx * stride + tx;
                     for (int f = 0; f < nfc; ++f)
-                        dst[f * TX] = col_ok ? inbase[(size_t)f * a.ncell + off] : cx<T>{(T)0, (T)0};
+                        dst[f * TX] = col_ok ? cx_as<T>(inbase[(size_t)f * a.ncell + off]) : cx<T>{(T)0, (T)0};
                 }
                 __syncthreads();
                 fft_tile<T, POW2>(tile, a.nst, a.stages, a.W, stride, nfc * TX, lines_div(a, nfc), tid, nthr);
@@ -370,14 +377,14 @@ __global__ __launch_bounds__(512) void pass_strided(PassArgs<T> a) {
                 a.I[(size_t)s0 + (size_t)a.n0 * ((size_t)s1 + (size_t)a.n1 * s2)] = fmax(re * re + im * im, 1e-50);
             }
         } else {
-            cx<T>* obase = a.out + (size_t)g * NOUT * a.ncell;
+            cx<TI>* obase = reinterpret_cast<cx<TI>*>(a.out) + (size_t)g * NOUT * a.ncell;
 #pragma unroll
             for (int m = 0; m < EPT; ++m) {
                 const int idx = r0 + m * rstep;
                 if (idx >= N) continue;
                 const uint32_t off = colbase + (uint32_t)idx * a.S;
 #pragma unroll
-                for (int q = 0; q < NOUT; ++q) obase[(size_t)q * a.ncell + off] = acc[m][q];
+                for (int q = 0; q < NOUT; ++q) obase[(size_t)q * a.ncell + off] = cx_as<TI>(acc[m][q]);
             }
         }
     }
@@ -390,8 +397,10 @@ __global__ __launch_bounds__(512) void pass_strided(PassArgs<T> a) {
 //   out_{z,0} = sum_s sum_{Y in z} c_Y(k1) X_{Y,s}(k0) [F1 A_s - i k1 F1 B_s],
 //   out_{z,1} = sum_s sum_{Y in z} c_Y(k1) X_{Y,s}(k0) F1 C_s,
 // the outputs of the class form's pass 1.
-template <typename T, int EPT, int NZ, bool POW2 = false>
+// IO32: FP64 arithmetic on complex FP32 inputs and outputs.
+template <typename T, int EPT, int NZ, bool POW2 = false, bool IO32 = false>
 __global__ __launch_bounds__(512) void pass_strided_mix(PassArgs<T> a) {
+    using TI = io_t<T, IO32>;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* tile = reinterpret_cast<cx<T>*>(smem_raw);
     const int tid = threadIdx.x, nthr = blockDim.x;
@@ -418,7 +427,7 @@ __global__ __launch_bounds__(512) void pass_strided_mix(PassArgs<T> a) {
         for (int m = 0; m < EPT; ++m) tA[m] = tC[m] = {(T)0, (T)0};
         for (int j0 = 0; j0 < 3; j0 += a.nf) {
             const int nfc = min(a.nf, 3 - j0);
-            const cx<T>* inbase = a.in + ((size_t)s * 3 + j0) * a.ncell;
+            const cx<TI>* inbase = reinterpret_cast<const cx<TI>*>(a.in) + ((size_t)s * 3 + j0) * a.ncell;
 #pragma unroll
             for (int m = 0; m < EPT; ++m) {
                 const int idx = r0 + m * rstep;
@@ -426,7 +435,7 @@ __global__ __launch_bounds__(512) void pass_strided_mix(PassArgs<T> a) {
                 const uint32_t off = colbase + (uint32_t)idx * a.S;
                 cx<T>* dst = tile + idx * stride + tx;
                 for (int f = 0; f < nfc; ++f)
-                    dst[f * TX] = col_ok ? inbase[(size_t)f * a.ncell + off] : cx<T>{(T)0, (T)0};
+                    dst[f * TX] = col_ok ? cx_as<T>(inbase[(size_t)f * a.ncell + off]) : cx<T>{(T)0, (T)0};
             }
             __syncthreads();
             fft_tile<T, POW2>(tile, a.nst, a.stages, a.W, stride, nfc * TX, lines_div(a, nfc), tid, nthr);
@@ -475,7 +484,8 @@ __global__ __launch_bounds__(512) void pass_strided_mix(PassArgs<T> a) {
 #pragma unroll
         for (int z = 0; z < NZ; ++z)
 #pragma unroll
-            for (int oq = 0; oq < 2; ++oq) a.out[(size_t)(2 * z + oq) * a.ncell + off] = acc[m][z][oq];
+            for (int oq = 0; oq < 2; ++oq)
+                reinterpret_cast<cx<TI>*>(a.out)[(size_t)(2 * z + oq) * a.ncell + off] = cx_as<TI>(acc[m][z][oq]);
     }
 }
 

@@ -45,6 +45,8 @@ struct Launch {
     int nthr = 0;
     int ept = 0;                    // elements per thread, 4 / 8 (generic)
     int pow2 = 0;                   // generic FP64: every stage radix 2, 4 or 8 (fft_tile POW2)
+    int io32 = 0;                   // generic FP64 kernel on FP32 arrays (FP32 plan computing in FP64)
+    int mix = 0;                    // generic strided pass 1 over source groups (pass_strided_mix)
     size_t lds = 0;
     int64_t blocks = 0;
 };
@@ -74,6 +76,7 @@ struct FastEntry {
     bool no_prefetch;    // strided roles: the variant without the register prefetch
     int real_bytes = 4;  // 4: an FP32 kernel (launch), 8: an FP64 one (launch64)
     FastLaunchFn64 launch64 = nullptr;
+    bool io32 = false;   // FP64 kernel on FP32 arrays (P, weights, spectra): FP32 plans computing in FP64
 };
 
 // Raise a kernel's dynamic LDS limit once per (kernel, device) and size.

@@ -18,7 +18,8 @@ LIB_PATH = Path(os.environ.get("QPD_LIB", Path(__file__).resolve().parent / "lib
 
 # Entry points declared in include/qpd.h
 EXPORTS = (
-    "qpd_plan_create", "qpd_plan_destroy", "qpd_workspace_bytes", "qpd_launches_per_volume",
+    "qpd_plan_create", "qpd_plan_create_opts", "qpd_plan_info", "qpd_plan_destroy", "qpd_workspace_bytes",
+    "qpd_launches_per_volume",
     "qpd_plan_schedules", "qpd_plan_schedule_name", "qpd_plan_set_schedule", "qpd_plan_get_schedule",
     "qpd_fast_lengths", "qpd_intensity", "qpd_intensity_pass", "qpd_pass_bytes_per_cell", "qpd_fft3d",
     "qpd_fft_length_supported", "qpd_ingest_fields", "qpd_project", "qpd_reverse_axes", "qpd_last_error",
@@ -77,6 +78,12 @@ def load_library():
     vp, i32, i64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64
     lib.qpd_plan_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(Grid), ctypes.POINTER(Crystal), i32]
     lib.qpd_plan_create.restype = i32
+    lib.qpd_plan_create_opts.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(Grid), ctypes.POINTER(Crystal), i32, i32,
+                                         i32]
+    lib.qpd_plan_create_opts.restype = i32
+    lib.qpd_plan_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                  ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)]
+    lib.qpd_plan_info.restype = i32
     lib.qpd_plan_destroy.argtypes = [vp]
     lib.qpd_plan_destroy.restype = i32
     lib.qpd_workspace_bytes.argtypes = [vp]
@@ -127,14 +134,22 @@ def _dptr(a):
     return ctypes.cast(a.ctypes.data, ctypes.POINTER(ctypes.c_double))
 
 
+ORDERS = {"auto": 0, "memory": 1, "swapped": 2}
+ARITHS = {"auto": 0, "fp32": 1, "mixed": 2, "fp64": 3}
+
+
 class Plan:
     """A libqpd plan: one grid layout, one crystal, one precision.
 
     dims are the memory extents (dims[0] fastest) and axis[d] the physical axis
     (0 = x, 1 = y, 2 = z) of memory dim d; h the x, y, z spacings in nm.
+    order ("auto", "memory", "swapped") is the order of the two strided transforms,
+    arith ("auto", "fp32", "mixed", "fp64") the arithmetic of a precision-32 plan: FP32
+    transforms and spectra, FP64 transforms with FP32 spectra between passes, or FP64
+    throughout (inputs widened); "auto" takes "mixed" where its error estimate allows.
     """
 
-    def __init__(self, dims, axis, h, R, f, su, G, qC, V, precision=32):
+    def __init__(self, dims, axis, h, R, f, su, G, qC, V, precision=32, order="auto", arith="auto"):
         lib = load_library()
         self.dims = tuple(int(d) for d in dims)
         self.axis = tuple(int(a) for a in axis)
@@ -154,17 +169,69 @@ class Plan:
         grid = Grid((ctypes.c_int64 * 3)(*self.dims), (ctypes.c_int * 3)(*self.axis), c_double3(*h))
         cr = Crystal(n_atom, n_phase, 3, _dptr(self._R), _dptr(self._fre), _dptr(self._fim), _dptr(self._su),
                      c_double3(*G), c_double3(*qC), float(V))
+        if order not in ORDERS or arith not in ARITHS:
+            raise ValueError(f"order must be one of {list(ORDERS)} and arith one of {list(ARITHS)}")
         handle = ctypes.c_void_p()
-        check(lib.qpd_plan_create(ctypes.byref(handle), ctypes.byref(grid), ctypes.byref(cr), self.precision))
+        check(lib.qpd_plan_create_opts(ctypes.byref(handle), ctypes.byref(grid), ctypes.byref(cr), self.precision,
+                                       ORDERS[order], ARITHS[arith]))
         self._h = handle
         self.workspace_bytes = int(lib.qpd_workspace_bytes(handle))
         self.launches = int(lib.qpd_launches_per_volume(handle))
+        pd = (ctypes.c_int * 3)()
+        cbits, sbits = ctypes.c_int(), ctypes.c_int()
+        est = ctypes.c_double()
+        check(lib.qpd_plan_info(handle, pd, ctypes.byref(cbits), ctypes.byref(sbits), ctypes.byref(est)))
+        self.pass_dims = tuple(int(v) for v in pd)  # memory dim transformed by each pass
+        self.compute_bits = int(cbits.value)        # precision of the transforms
+        self.storage_bits = int(sbits.value)        # precision of the spectra between passes
+        self.dlog10_estimate = float(est.value)     # the error estimate of that arithmetic
+
+    def _check_inputs(self, P, U, phase, I, work, fraction, region, which=None):
+        """Shapes, dtypes, device and contiguity of the buffers a call reads or writes (every
+        pass, or pass ``which`` alone), before any device access."""
+        import torch
+        real = torch.float32 if self.precision == 32 else torch.float64
+        n = self.n
+
+        def need(t, name, dtype, numel, optional=False):
+            if t is None:
+                if optional:
+                    return
+                raise ValueError(f"{name} is required")
+            if not isinstance(t, torch.Tensor) or not t.is_cuda:
+                raise ValueError(f"{name} must be a CUDA tensor")
+            if t.dtype != dtype:
+                raise ValueError(f"{name} must be {dtype}, got {t.dtype}")
+            if not t.is_contiguous():
+                raise ValueError(f"{name} must be contiguous")
+            if numel is not None and t.numel() != numel:
+                raise ValueError(f"{name} must hold {numel} values, got {t.numel()}")
+
+        if which in (None, 0):
+            need(P, "P", real, 3 * n)
+            need(U, "U", torch.float64, 3 * n)
+            need(phase, "phase", torch.int8, n)
+            need(fraction, "fraction", real, self.n_phase * n, optional=True)
+            need(region, "region", real, n, optional=True)
+        else:
+            P = U = phase = fraction = region = None
+        if which in (None, 2):
+            need(I, "I", torch.float64, n)
+        else:
+            I = None
+        if not isinstance(work, torch.Tensor) or not work.is_cuda or not work.is_contiguous():
+            raise ValueError("work must be a contiguous CUDA tensor")
+        if work.numel() * work.element_size() < self.workspace_bytes:
+            raise ValueError(f"work holds {work.numel() * work.element_size()} bytes, the plan needs "
+                             f"{self.workspace_bytes}")
+        devs = {t.device for t in (P, U, phase, fraction, region, I, work) if t is not None}
+        if len(devs) != 1:
+            raise ValueError(f"buffers are on several devices: {sorted(str(d) for d in devs)}")
 
     def intensity(self, P, U, phase, I, work, fraction=None, region=None, stream=None):
         """Enqueue one volume: device tensors in, fftshifted |A|^2 (float64) into I."""
-        fields = Fields(P.data_ptr(), U.data_ptr(), phase.data_ptr(),
-                        fraction.data_ptr() if fraction is not None else None,
-                        region.data_ptr() if region is not None else None)
+        self._check_inputs(P, U, phase, I, work, fraction, region)
+        fields = self._fields(P, U, phase, fraction, region)
         check(load_library().qpd_intensity(self._h, ctypes.byref(fields), I.data_ptr(), work.data_ptr(),
                                            stream if stream is not None else current_stream()))
 
@@ -175,6 +242,7 @@ class Plan:
 
     def run_pass(self, which, P, U, phase, I, work, fraction=None, region=None, stream=None):
         """Enqueue one of the three passes only (for timing each kernel)."""
+        self._check_inputs(P, U, phase, I, work, fraction, region, which=int(which))
         fields = self._fields(P, U, phase, fraction, region)
         check(load_library().qpd_intensity_pass(self._h, ctypes.byref(fields), I.data_ptr(), work.data_ptr(),
                                                 int(which), stream if stream is not None else current_stream()))

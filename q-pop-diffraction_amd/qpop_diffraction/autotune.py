@@ -66,10 +66,12 @@ def device_name():
 
 
 def plan_key(plan, fields_kind=""):
-    """Device, precision, layout and the atom positions (which set the classes of each pass)."""
+    """Device, precision, layout, pass order and arithmetic, and the atom positions (which set
+    the classes of each pass)."""
     h = hashlib.sha1(np.round(plan._R, 12).tobytes()).hexdigest()[:12]
     return (f"{device_name()}|fp{plan.precision}|dims{'x'.join(map(str, plan.dims))}|"
-            f"axis{''.join(map(str, plan.axis))}|atoms{plan.n_atom}x{plan.n_phase}:{h}{fields_kind}")
+            f"axis{''.join(map(str, plan.axis))}|passes{''.join(map(str, plan.pass_dims))}|"
+            f"bits{plan.compute_bits}-{plan.storage_bits}|atoms{plan.n_atom}x{plan.n_phase}:{h}{fields_kind}")
 
 
 def time_pass(plan, which, P, U, phase, out, work, fraction=None, region=None, reps=3, warmup=1):

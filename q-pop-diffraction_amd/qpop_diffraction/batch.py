@@ -43,10 +43,13 @@ class Reflection:
 
 
 class PlanCache:
-    """libqpd plans keyed by (memory dims, axis, spacing, reflection, precision)."""
+    """libqpd plans keyed by (memory dims, axis, spacing, reflection, precision).
 
-    def __init__(self, precision=32):
+    order and arith are the plan options of ``Plan`` (pass order, arithmetic of FP32 plans)."""
+
+    def __init__(self, precision=32, order="auto", arith="auto"):
         self.precision = precision
+        self.order, self.arith = order, arith
         self._plans = {}
         self.built = 0
 
@@ -58,7 +61,8 @@ class PlanCache:
         plan = self._plans.get(key)
         if plan is None:
             R, f, su, G, qC, V = refl.crystal(p)
-            plan = Plan(snap.dims, snap.axis, h, R, f, su, G, qC, V, self.precision)
+            plan = Plan(snap.dims, snap.axis, h, R, f, su, G, qC, V, self.precision, order=self.order,
+                        arith=self.arith if self.precision == 32 else "auto")
             self._plans[key] = plan
             self.built += 1
         return plan

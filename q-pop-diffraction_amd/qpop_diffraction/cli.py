@@ -196,15 +196,45 @@ def _prepare_on_device(args, cfg, params, fe_ids, atoms, full, dz, sl, lims):
 
 
 def run_z_fastest(nx, nz):
-    """Whether the CLI runs its x-fastest fields z fastest: when no compile-time source kernel
-    has the x length and one has the z length (the source pass then transforms z rows; the
-    generic kernel on long x rows forms each cell's source once per field).  QPD_CLI_LAYOUT=x
-    or z forces the layout."""
+    """Whether the CLI runs its x-fastest fields z fastest: for a run one cell thick along z
+    (a 2-D run: pass 0 then has nothing to transform, so the FP32 rounding of its spectra,
+    which only the later transforms spread out, is that of the source alone, over the whole
+    x-y plane), and when no compile-time source kernel has the x length and one has the z
+    length (the source pass then transforms z rows; the generic kernel on long x rows forms
+    each cell's source once per field).  QPD_CLI_LAYOUT=x or z forces the layout."""
     forced = os.environ.get("QPD_CLI_LAYOUT", "")
     if forced in ("x", "z"):
         return forced == "z"
+    if nz == 1 and nx > 1:
+        return True
     kernels = set(fast_lengths("source")) | set(fast_lengths("source_wave"))
     return nx not in kernels and nz in kernels
+
+
+def intensity_of_fields(engine, params, fields, precision, log=print):
+    """The fftshifted intensity (device float64, x fastest) of a run's fields in file order.
+
+    fields: dict with P, U, phase, fraction, region (host arrays or device tensors, x fastest);
+    it is emptied, so that the only references to the x-fastest copies are this function's.
+    When ``run_z_fastest`` chooses the z-fastest layout the fields are reversed on the device,
+    the x-fastest copies released, and the intensity reversed back (peak HBM: the fields twice,
+    briefly, then the z-fastest fields, the plan's workspace and two intensities).
+    """
+    dims, axis = (params.nx, params.ny, params.nz), (0, 1, 2)
+    vol = Volume(dims, axis, fields.pop("P"), fields.pop("U"), fields.pop("phase"), fields.pop("fraction", None),
+                 fields.pop("region", None), precision)
+    fields.clear()
+    if not run_z_fastest(params.nx, params.nz):
+        return engine.intensity(vol)
+    # the fields reversed to z fastest on the device, and the intensity back to x fastest
+    rev = lambda t: None if t is None else reverse_axes(t, dims)
+    zvol = Volume(dims[::-1], axis[::-1], rev(vol.P), rev(vol.U), rev(vol.phase), rev(vol.fraction), rev(vol.region),
+                  precision)
+    del vol
+    log(f"  fields run z fastest ({params.nx} x {params.ny} x {params.nz} cells)")
+    Iz = engine.intensity(zvol)
+    del zvol
+    return reverse_axes(Iz, dims[::-1])
 
 
 def main(argv=None) -> int:
@@ -222,18 +252,7 @@ def main(argv=None) -> int:
     print(f"\nComputing diffraction (MI355X, complex FP{args.precision})")
     t0 = time.perf_counter()
     engine = Engine(params, atoms)
-    dims, axis = (params.nx, params.ny, params.nz), (0, 1, 2)
-    vol = Volume(dims, axis, f["P"], f["U"], f["phase"], f["fraction"], f["region"], args.precision)
-    z_first = run_z_fastest(params.nx, params.nz)
-    if z_first:
-        # the fields reversed to z fastest on the device, and the intensity back to x fastest
-        rev = lambda t: None if t is None else reverse_axes(t, dims)
-        vol = Volume(dims[::-1], axis[::-1], rev(vol.P), rev(vol.U), rev(vol.phase), rev(vol.fraction),
-                     rev(vol.region), args.precision)
-        print(f"  x rows of {params.nx} cells: fields run z fastest")
-    I = engine.intensity(vol)
-    if z_first:
-        I = reverse_axes(I, vol.dims)
+    I = intensity_of_fields(engine, params, f, args.precision)
     print(f"  GPU path {time.perf_counter() - t0:.3f} s for {params.n} cells")
     qC = engine.tables[4]
     # the intensity streams from HBM into the file slab by slab

@@ -115,20 +115,26 @@ def weights_from_phases(oPhase, region, nPhase, n):
 
 
 class Engine:
-    """Plans and workspaces for one crystal on one GPU, cached by layout and precision."""
+    """Plans and workspaces for one crystal on one GPU, cached by layout and precision.
 
-    def __init__(self, params, atoms):
+    Workspaces are cached per stream: calls enqueued on different streams never share one.
+    order and arith are the plan options of ``Plan`` (pass order, arithmetic of FP32 plans).
+    """
+
+    def __init__(self, params, atoms, order="auto", arith="auto"):
         self.params = params
         self.atoms = atoms
         self.tables = crystal_tables(params, atoms)
+        self.order, self.arith = order, arith
         self._plans = {}
         self._work = {}
 
     @classmethod
-    def from_tables(cls, params, tables):
+    def from_tables(cls, params, tables, order="auto", arith="auto"):
         """An engine for crystal tables (R, f, su, G, qC, V) that were prepared elsewhere."""
         eng = cls.__new__(cls)
         eng.params, eng.atoms, eng.tables, eng._plans, eng._work = params, None, tuple(tables), {}, {}
+        eng.order, eng.arith = order, arith
         return eng
 
     def plan(self, dims, axis, precision):
@@ -136,14 +142,17 @@ class Engine:
         if key not in self._plans:
             R, f, su, G, qC, V = self.tables
             h = (self.params.dx, self.params.dy, self.params.dz)
-            self._plans[key] = Plan(dims, axis, h, R, f, su, G, qC, V, precision)
+            self._plans[key] = Plan(dims, axis, h, R, f, su, G, qC, V, precision, order=self.order,
+                                    arith=self.arith if precision == 32 else "auto")
         return self._plans[key]
 
-    def workspace(self, plan):
+    def workspace(self, plan, stream=None):
+        """A workspace of the plan's size for calls on ``stream`` (None: torch's current stream)."""
         import torch
-        key = plan.workspace_bytes
+        sid = stream.value if stream is not None else torch.cuda.current_stream().cuda_stream
+        key = (plan.workspace_bytes, sid)
         if key not in self._work:
-            self._work[key] = torch.empty(max(1, key), dtype=torch.uint8, device="cuda")
+            self._work[key] = torch.empty(max(1, plan.workspace_bytes), dtype=torch.uint8, device="cuda")
         return self._work[key]
 
     def tune(self, vol: Volume, cache=None, out=None):
@@ -161,7 +170,7 @@ class Engine:
         plan = self.plan(vol.dims, vol.axis, vol.precision)
         if out is None:
             out = torch.empty(vol.n, dtype=torch.float64, device="cuda")
-        plan.intensity(vol.P, vol.U, vol.phase, out, self.workspace(plan), vol.fraction, vol.region, stream)
+        plan.intensity(vol.P, vol.U, vol.phase, out, self.workspace(plan, stream), vol.fraction, vol.region, stream)
         return out
 
 

@@ -26,8 +26,9 @@ from ._native import Plan
 
 
 def _plan(dims, axis, params, tables, precision):
+    # the passes in memory order (z, y, x): the slabs are cut along the last pass's axis
     R, f, su, G, qC, V = tables
-    return Plan(dims, axis, (params.dx, params.dy, params.dz), R, f, su, G, qC, V, precision)
+    return Plan(dims, axis, (params.dx, params.dy, params.dz), R, f, su, G, qC, V, precision, order="memory")
 
 
 def crop_indices(n, m):

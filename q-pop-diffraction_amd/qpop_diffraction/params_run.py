@@ -20,7 +20,8 @@ from types import SimpleNamespace
 
 import numpy as np
 
-from .diffraction import Engine, Volume, default_precision
+from .cli import intensity_of_fields
+from .diffraction import Engine, default_precision
 from .h5lite import open_file, read_dataset
 from .ingest import ingest, read_components, read_material_columns
 from .io_utils import read_field, read_mgpu_params, write_diffraction_streamed
@@ -71,8 +72,9 @@ def run(params_file, precision=None, log=print):
     f = prm["fAtom_re"] + 1j * prm["fAtom_im"]
     tables = (prm["R"], f, prm["su"], prm["G"], prm["qC"], float(prm["V"][0]))
     engine = Engine.from_tables(params, tables)
-    vol = Volume((params.nx, params.ny, params.nz), (0, 1, 2), P, U, phase, fraction, region, precision)
-    I = engine.intensity(vol)
+    # the layout rule of the command line, so that both routes compute the same volume
+    fields = {"P": P, "U": U, "phase": phase, "fraction": fraction, "region": region}
+    I = intensity_of_fields(engine, params, fields, precision, log=log)
     out = prm["output_file"]
     base = out[:-3] if out.endswith(".h5") else out
     with open_file(out, "r") as fh:

@@ -110,3 +110,18 @@ def dlog10(I, ref, floor_rel=1e-10):
 
 def rel_to_peak(I, ref):
     return float(np.max(np.abs(np.asarray(I) - np.asarray(ref))) / np.max(ref))
+
+
+# The FP32 bounds: max |I - I_ref| <= 1e-5 of the peak, max |dlog10 I| <= 1e-3 over voxels
+# >= 1e-10 of the peak, and the peak on the same voxel.
+FP32_PEAK_TOL = 1e-5
+FP32_DLOG_TOL = 1e-3
+
+
+def check_fp32(I, ref, what=""):
+    """Hold an FP32 intensity to the FP32 bounds against an FP64 one (same layout)."""
+    rel, dl = rel_to_peak(I, ref), dlog10(I, ref)
+    print(f"  fp32 {what}: max|dI|/peak {rel:.2e}, max|dlog10 I| {dl:.2e}", flush=True)
+    assert rel <= FP32_PEAK_TOL, (what, rel)
+    assert dl <= FP32_DLOG_TOL, (what, dl)
+    assert np.argmax(I) == np.argmax(ref), what

@@ -13,7 +13,7 @@
 import numpy as np
 import pytest
 
-from helpers import dlog10, rel_to_peak
+from helpers import check_fp32
 from qpop_diffraction import workloads
 from qpop_diffraction.batch import Batch, PlanCache, Reflection, plain_job, resident_job
 from qpop_diffraction.serve import Server, poisson_requests
@@ -88,9 +88,7 @@ def test_batch_job_vs_oracle(snapshots, reflections, r):
     I = b.result(0, 0).cpu().numpy().reshape(nz, ny, nx).transpose(2, 1, 0)
     oPhase, P, U, region = workloads.host_inputs(snap)
     ref = diffraction_fp64(snap.params, reflections[r].atoms, oPhase, P, U, region)[0]
-    print(f"  max|dI|/peak {rel_to_peak(I, ref):.2e}, max|dlog10 I| {dlog10(I, ref):.2e}")
-    assert rel_to_peak(I, ref) <= 1e-5
-    assert np.argmax(I) == np.argmax(ref)
+    check_fp32(I, ref, f"batched job hkl {reflections[r].hkl}")
 
 
 def test_plain_job_matches_batch(snapshots, reflections):
@@ -149,7 +147,7 @@ def test_served_same_alone_and_in_mix():
 def test_served_under_tight_budget():
     """A budget that holds the largest job but not every snapshot the requests read plus the
     smallest job's buffers: whatever order the slots finish in, the store must evict."""
-    reqs = poisson_requests(16, rate=0.0, grids=SERVE_GRIDS, n_snapshots=6, n_reflections=4, seed=11)
+    reqs = poisson_requests(24, rate=0.0, grids=SERVE_GRIDS, n_snapshots=12, n_reflections=4, seed=11)
     refl = [Reflection(h, workloads.pto_atoms(h)) for h in workloads.REFLECTIONS]
     plans = PlanCache(32)
     snap_bytes = lambda grid: workloads.snapshot_spec(*grid).nbytes
@@ -165,7 +163,7 @@ def test_served_under_tight_budget():
     assert all_snaps + min(buffer_bytes(g) for g in grids) > budget, "the case would not force an eviction"
     print(f"serve: tight budget {budget / 1e6:.1f} MB, snapshots {all_snaps / 1e6:.1f} MB", flush=True)
     tight, done, srv = serve(reqs, n_slots=3, budget=budget, realtime=False)
-    assert sorted(tight) == list(range(16))
+    assert sorted(tight) == list(range(24))
     assert srv.peak_bytes <= budget
     assert srv.store.evictions > 0
     roomy, _, _ = serve(reqs, n_slots=3, budget=200e9, realtime=False)

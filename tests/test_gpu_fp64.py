@@ -23,13 +23,13 @@ def refl():
     return Reflection((1, 0, 3), workloads.pto_atoms((1, 0, 3)))
 
 
-def run_fp64(refl, shape, schedules=None, pick=None):
+def run_fp64(refl, shape, schedules=None, pick=None, order="memory"):
     """FP64 intensity of a twin snapshot; pick: {pass: suffix} selects the first schedule
-    whose name ends with it."""
+    whose name ends with it; order: the plan's pass order (memory order: pass d along dim d)."""
     import torch
     nx, ny, nz = shape
     snap = workloads.twin_snapshot(("fp64",) + tuple(shape), nx, ny, nz, seed=5, precision=64)
-    plans = PlanCache(64)
+    plans = PlanCache(64, order=order)
     plan = plans.get(snap, 0, refl)
     for d, idx in (schedules or {}).items():
         plan.set_schedule(d, idx)
@@ -42,15 +42,16 @@ def run_fp64(refl, shape, schedules=None, pick=None):
     return I, snap, plan
 
 
+@pytest.mark.parametrize("order", ["auto", "memory"])
 @pytest.mark.parametrize("shape", [(48, 40, 24), (104, 64, 30), (40, 24, 104)])
 @pytest.mark.parametrize("form", ["groups", "class"])
-def test_fp64_every_schedule(refl, shape, form, monkeypatch):
+def test_fp64_every_schedule(refl, shape, form, order, monkeypatch):
     from qpd_oracle import diffraction_fp64
     if form == "class":
         monkeypatch.setenv("QPD_CLASS_FORM", "1")
     else:
         monkeypatch.delenv("QPD_CLASS_FORM", raising=False)
-    I0, snap, plan = run_fp64(refl, shape)
+    I0, snap, plan = run_fp64(refl, shape, order=order)
     ref = diffraction_fp64(snap.params, refl.atoms, *workloads.host_inputs(snap))[0]
     atol = 1e-12 * ref.max()
     np.testing.assert_allclose(I0, ref, rtol=1e-9, atol=atol)
@@ -58,7 +59,7 @@ def test_fp64_every_schedule(refl, shape, form, monkeypatch):
     print(f"{shape} {form}: workspace {plan.workspace_bytes} B, schedules {names}", flush=True)
     for d in range(3):
         for idx in range(1, len(names[d])):
-            I, _, _ = run_fp64(refl, shape, {d: idx})
+            I, _, _ = run_fp64(refl, shape, {d: idx}, order=order)
             np.testing.assert_allclose(I, ref, rtol=1e-9, atol=atol, err_msg=f"pass {d}: {names[d][idx]}")
 
 
@@ -97,7 +98,7 @@ def test_fp64_every_length(refl, role, dim, form, monkeypatch):
         # pass 1 of the source-group form needs a length with a source-group kernel on dim 1 (20)
         shape = [6, 20, 25] if role == "source_wave" else [6, 5, 4]
         shape[dim] = N
-        plan_names = PlanCache(64).get(
+        plan_names = PlanCache(64, order="memory").get(
             workloads.twin_snapshot(("names",) + tuple(shape), *shape, seed=5, precision=64), 0, refl)
         names = plan_names.schedules(dim)
         wanted = " source-wave f64" if role == "source_wave" else " f64"

@@ -2,9 +2,11 @@
 
 * ``large.fp64_crop`` (x slabs through passes 0 and 1, the crop's ky kept, pass 2 on the
   reduced grid) equals the crop of a full FP64 plan of the same volume to 1e-12 of the peak;
-* the polar-vortex generator's FP32 volume agrees with that FP64 crop to the FP32 bound
-  (1e-5 of the peak, same peak voxel);
-* a long strided axis runs split (the schedule of pass 2 starts with "split").
+* the polar-vortex generator's FP32 volume agrees with that FP64 crop to the FP32 bounds
+  (1e-5 of the peak, |dlog10 I| <= 1e-3, same peak voxel);
+* a long last axis runs in one launch: a split stores its first step in FP32, whose
+  rounding its 25- to 128-point second step would spread over too few voxels (pass 2 of
+  an FP32 plan of the automatic choice keeps no split schedule).
 """
 
 import numpy as np
@@ -42,7 +44,7 @@ def test_vortex_fp32_against_fp64_crop():
     vol = Volume(v.dims, v.axis, v.P, v.U, v.phase, precision=32)
     plan = eng.plan(vol.dims, vol.axis, 32)
     names = plan.schedules(2)
-    assert names[plan.schedule(2)].startswith("split"), names
+    assert not any(nm.startswith("split") for nm in names), names
     I32 = crop_of(eng.intensity(vol), v.dims, crop=(128, 128))
     ref = fp64_crop(v, v.params, eng.tables, crop=(128, 128), slab=250)
     peak = ref.max()
@@ -51,4 +53,5 @@ def test_vortex_fp32_against_fp64_crop():
     dl = (torch.log10(I32[mask]) - torch.log10(ref[mask])).abs().max().item()
     print(f"  max|dI|/peak {rel:.2e}, max|dlog10 I| {dl:.2e}")
     assert rel <= 1e-5
+    assert dl <= 1e-3
     assert int(I32.argmax()) == int(ref.argmax())

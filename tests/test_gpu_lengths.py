@@ -3,24 +3,26 @@
 * qpd_fft3d on each registered length of the plain contiguous and strided roles
   (FP32, to 2e-6 of the largest output, like test_gpu_fft.py);
 * the volume path with each registered length on each memory dim it has a kernel for
-  (source pass on dim 0, 3 -> 2 pass on dim 1, intensity pass on dim 2), FP32 against
-  the FP64 oracle: max |dI| <= 1e-5 of the peak and the same peak voxel;
+  (source pass on dim 0, 3 -> 2 pass on dim 1, intensity pass on dim 2), with the passes
+  in memory order and every pass in FP32 so that the FP32 kernel of that length runs,
+  against the FP64 oracle: max |dI| <= 1e-5 of the peak and the same peak voxel; the same
+  with passes 1 and 2 in FP64 (the pass-1 kernels that read FP32 spectra);
 * long strided lengths with no kernel of their own run split into two launches
   (N = N1 x N2 up to 4000, the length of the 4000 x 4000 x 68 volume), same bound;
-* on one plan, every schedule of every pass gives the same bound.
+* on one plan, every schedule of every pass gives the same bound;
+* every volume above also runs through the default plan (pass order and arithmetic chosen
+  by the plan), held to the FP32 bounds: 1e-5 of the peak, |dlog10 I| <= 1e-3, same peak.
 """
 
 import numpy as np
 import pytest
 
-from helpers import rel_to_peak
+from helpers import FP32_PEAK_TOL, check_fp32, rel_to_peak
 from qpop_diffraction import workloads
 from qpop_diffraction._native import fast_lengths
 from qpop_diffraction.batch import PlanCache, Reflection, resident_job
 
 pytestmark = pytest.mark.gpu
-
-FP32_PEAK_TOL = 1e-5
 
 
 @pytest.fixture(scope="module")
@@ -59,15 +61,19 @@ def test_fft3d_every_strided_length():
     print(f"strided lengths: worst {worst:.2e}")
 
 
-def volume_check(refl, nx, ny, nz, seed=0, expect=None, schedules=None, pick=None):
-    """FP32 volume of a twin snapshot against the oracle; returns max |dI| / peak.
+def volume_check(refl, nx, ny, nz, seed=0, expect=None, schedules=None, pick=None, order="memory", arith="fp32"):
+    """FP32 volume of a twin snapshot against the oracle, with the kernels under test selected;
+    returns max |dI| / peak.  The same snapshot through the default plan is held to the FP32
+    bounds.
 
-    expect: {pass: prefix of the default schedule's name}; pick: {pass: suffix} selects the
-    first schedule whose name ends with it (which must exist)."""
+    order, arith: plan options (by default the memory order and FP32 in every pass, so that
+    pass d runs along memory dim d in FP32); expect: {pass: prefix of the default schedule's
+    name}; pick: {pass: suffix} selects the first schedule whose name ends with it (which must
+    exist)."""
     import torch
     from qpd_oracle import diffraction_fp64
     snap = workloads.twin_snapshot(("len", nx, ny, nz), nx, ny, nz, seed=seed)
-    plans = PlanCache(32)
+    plans = PlanCache(32, order=order, arith=arith)
     plan = plans.get(snap, 0, refl)
     if expect is not None:
         for d, prefix in expect.items():
@@ -87,29 +93,54 @@ def volume_check(refl, nx, ny, nz, seed=0, expect=None, schedules=None, pick=Non
     I = out.cpu().numpy().reshape(nz, ny, nx).transpose(2, 1, 0)
     ref = diffraction_fp64(snap.params, refl.atoms, *workloads.host_inputs(snap))[0]
     assert np.argmax(I) == np.argmax(ref), (nx, ny, nz)
+    dplans = PlanCache(32)
+    dplan = dplans.get(snap, 0, refl)
+    dwork = torch.empty(dplan.workspace_bytes, dtype=torch.uint8, device="cuda")
+    resident_job(dplans, snap, 0, refl, out, dwork)
+    Id = out.cpu().numpy().reshape(nz, ny, nx).transpose(2, 1, 0)
+    check_fp32(Id, ref, f"default plan {(nx, ny, nz)} passes {dplan.pass_dims} "
+                        f"bits {dplan.compute_bits}/{dplan.storage_bits}")
     return rel_to_peak(I, ref)
 
 
-@pytest.mark.parametrize("role,dim,form", [("source_wave", 0, "groups"), ("source", 0, "groups"),
-                                           ("source", 0, "class"), ("strided", 1, "class"),
-                                           ("groups", 1, "groups"), ("intensity", 2, "groups")])
-def test_volume_every_length(refl, role, dim, form, monkeypatch):
+@pytest.mark.parametrize("role,dim,form,arith", [("source_wave", 0, "groups", "fp32"), ("source", 0, "groups", "fp32"),
+                                                 ("source", 0, "class", "fp32"), ("strided", 1, "class", "fp32"),
+                                                 ("groups", 1, "groups", "fp32"), ("intensity", 2, "groups", "fp32"),
+                                                 ("source_wave", 0, "groups", "mixed"),
+                                                 ("source", 0, "class", "mixed"),
+                                                 ("strided", 1, "class", "mixed"),
+                                                 ("groups", 1, "groups", "mixed"),
+                                                 ("intensity", 2, "groups", "mixed")])
+def test_volume_every_length(refl, role, dim, form, arith, monkeypatch):
     """Pass 1 runs the source-group kernel for PbTiO3 unless QPD_CLASS_FORM=1; pass 0 of
     the source-group form defaults to the one-wave-per-row kernel where the length has one,
-    and the block-per-row source kernel is picked explicitly for the "source" case."""
+    and the block-per-row source kernel is picked explicitly for the "source" case.  With
+    arith "mixed" the pass runs the FP64 kernel of the length on FP32 arrays (FP32 inputs and
+    spectra); a length whose FP64 tile does not fit runs the generic kernel."""
     if form == "class":
         monkeypatch.setenv("QPD_CLASS_FORM", "1")
     else:
         monkeypatch.delenv("QPD_CLASS_FORM", raising=False)
     worst = 0.0
     lengths = fast_lengths("strided_groups2" if role == "groups" else role)
+    suffix = None
+    if arith == "mixed":
+        suffix = " source-wave f64 io32" if role == "source_wave" else " f64 io32"
+    ran_fast = 0
     for N in lengths:
         # pass 1 of the source-group form needs a compile-time length on dim 1 (20); 500 rows
         # leave a partial last block for most rows-per-block of the source kernels
         shape = [6, 20, 25] if role.startswith("source") and form == "groups" else [6, 5, 4]
         shape[dim] = N
         print(f"{role} N={N} grid {shape}", flush=True)
-        if role == "source_wave":
+        if suffix:
+            # a length whose FP64 tile does not fit (more than 256 threads) runs the generic kernel
+            names = PlanCache(32, order="memory", arith=arith).get(
+                workloads.twin_snapshot(("names",) + tuple(shape), *shape, seed=0), 0, refl).schedules(dim)
+            has = any(nm.endswith(suffix) for nm in names)
+            ran_fast += has
+            err = volume_check(refl, *shape, pick={dim: suffix} if has else None, arith=arith)
+        elif role == "source_wave":
             err = volume_check(refl, *shape, expect={dim: "fast"}, pick={dim: " source-wave"})
         elif role == "source" and form == "groups":
             err = volume_check(refl, *shape, expect={dim: "fast"}, pick={dim: " source"})
@@ -117,7 +148,10 @@ def test_volume_every_length(refl, role, dim, form, monkeypatch):
             err = volume_check(refl, *shape, expect={dim: "fast"})
         worst = max(worst, err)
         assert err <= FP32_PEAK_TOL, (N, err)
-    print(f"{role}: worst max|dI|/peak {worst:.2e}")
+    print(f"{role} {arith}: worst max|dI|/peak {worst:.2e}")
+    if suffix:
+        print(f"{role}: {ran_fast} of {len(lengths)} lengths on an FP64 kernel reading FP32 spectra")
+        assert ran_fast >= len(lengths) // 2
 
 
 @pytest.mark.parametrize("N", [640, 1000, 2000, 4000])
@@ -145,7 +179,7 @@ def test_long_contiguous_rows(refl, N):
 def test_every_schedule(refl, shape):
     import torch
     snap = workloads.twin_snapshot(("sched",) + shape, *shape, seed=1)
-    plan = PlanCache(32).get(snap, 0, refl)
+    plan = PlanCache(32, order="memory", arith="fp32").get(snap, 0, refl)
     counts = [len(plan.schedules(d)) for d in range(3)]
     print(f"{shape}: schedules per pass {counts}", flush=True)
     for d in range(3):
@@ -192,7 +226,7 @@ def test_source_groups_match_class_form(refl, shape, axis, monkeypatch):
     I = outs["groups"].reshape(nz, ny, nx) if axis == (0, 1, 2) else outs["groups"].reshape(nx, ny, nz).transpose(2, 1, 0)
     snap0 = workloads.twin_snapshot(("grp",) + shape, *shape, seed=9)
     ref = diffraction_fp64(snap0.params, refl.atoms, *workloads.host_inputs(snap0))[0].transpose(2, 1, 0)
-    assert rel_to_peak(I, ref) <= FP32_PEAK_TOL and np.argmax(I) == np.argmax(ref)
+    check_fp32(I, ref, f"{shape} axis {axis}")
 
 
 @pytest.mark.parametrize("case", ["two_phase", "fraction_region"])
@@ -200,7 +234,7 @@ def test_source_schedules_with_weights(case):
     """Every pass-0 schedule of a source-group plan (the one-wave-per-row kernel, the
     block-per-row kernel, the generic one) on a two-phase volume: with non-scattering cells
     (one-hot phase index, -1 in places), and with fractional phase weights and a region
-    weight (the fraction path).  Each against the oracle: 1e-5 of the peak, same peak voxel."""
+    weight (the fraction path).  Each against the oracle at the FP32 bounds."""
     import torch
     from qpd_oracle import diffraction_fp64
     from qpop_diffraction import Engine, Volume
@@ -235,9 +269,7 @@ def test_source_schedules_with_weights(case):
     for idx, name in enumerate(names):
         plan.set_schedule(0, idx)
         I = eng.intensity(vol).cpu().numpy().reshape(nz, ny, nx)
-        err = rel_to_peak(I, ref)
-        print(f"{case} pass 0 schedule {name}: max|dI|/peak {err:.2e}", flush=True)
-        assert err <= FP32_PEAK_TOL and np.argmax(I) == np.argmax(ref), (name, err)
+        check_fp32(I, ref, f"{case} pass 0 schedule {name}")
 
 
 @pytest.mark.parametrize("shape", [(64, 48, 1), (40, 1, 24), (1, 32, 20), (1, 1, 16)])

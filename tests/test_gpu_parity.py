@@ -1,13 +1,9 @@
 """The HIP path (qpop-diffraction CLI and diffraction()) against the oracle and the golden vectors.
 
 FP64 plans are held to the reference's own parity bound (rtol 1e-9, atol 1e-12 of
-the peak, reference tests/test_mgpu.py:51-53).  FP32 plans are held to the bulk
-parity bound: max |I - I_ref| <= 1e-5 of the peak and the peak on the same voxel.
-Their max |dlog10 I| over voxels >= 1e-10 of the peak is bounded by 3e-3 on these
-small grids (2.5e4 - 1.3e5 cells): rounding the spectra to FP32 between passes
-leaves an absolute error of about 1e-8 of the peak amplitude, which the remaining
-transforms do not average down on short axes; the 1e-3 bound of the full
-512 x 512 x 104 volume is tested in test_gpu_fullsize.py.
+the peak, reference tests/test_mgpu.py:51-53).  FP32 plans are held to the FP32
+bounds: max |I - I_ref| <= 1e-5 of the peak, max |dlog10 I| <= 1e-3 over voxels
+>= 1e-10 of the peak, and the peak on the same voxel.
 """
 
 import numpy as np
@@ -15,20 +11,11 @@ import pytest
 
 import golden_cases
 from conftest import copy_example, load_golden
-from helpers import dlog10, golden_check, oracle_inputs, oracle_run, rel_to_peak, run_cli
+from helpers import check_fp32, dlog10, golden_check, oracle_inputs, oracle_run, rel_to_peak, run_cli
 from qpop_diffraction.h5lite import open_file
 
 pytestmark = pytest.mark.gpu
 
-FP32_PEAK_TOL = 1e-5
-FP32_DLOG_TOL_SMALL = 3e-3
-
-
-def check_fp32(I, ref):
-    print(f"  fp32: max|dI|/peak = {rel_to_peak(I, ref):.3e}, max|dlog10 I| = {dlog10(I, ref):.3e}")
-    assert rel_to_peak(I, ref) <= FP32_PEAK_TOL, rel_to_peak(I, ref)
-    assert dlog10(I, ref) <= FP32_DLOG_TOL_SMALL, dlog10(I, ref)
-    assert np.argmax(I) == np.argmax(ref)
 
 
 @pytest.mark.parametrize("name", sorted(golden_cases.CASES))
