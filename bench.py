@@ -8,10 +8,13 @@ rms 0.005 nm), PbTiO3-like 5-atom cell, reflection 002, synthetic seeded data
 generated on the GPU.  One step = one volume: source, 3-D transforms and the
 fftshifted intensity (three kernel launches), inputs resident in HBM.
 
-Prints one JSON line: voxels/s (FP32 path) as ``value``, ms per volume, the
-FP64 path's ms per volume, |dlog10 I| of FP32 against FP64, per-pass kernel
-times and GB/s, the roofline entry of the dominant kernel, and the CPU
-baseline (the FP64 NumPy oracle on a 256 x 256 x 104 crop, one core).
+Prints one JSON line: voxels/s of the FP32 plan (FP32 inputs and spectra, FP64
+transforms: the default arithmetic) as ``value``, ms per volume, the same volume
+with FP32 transforms and with FP64 throughout beside it, |dlog10 I| of each
+against FP64, per-pass kernel times and GB/s, the roofline entry of the dominant
+kernel, the batched, served and 4000 x 4000 x 68 legs (every batched and served
+job against its FP64 result), and the CPU baseline (the FP64 NumPy oracle on the
+same 512 x 512 x 104 volume, one core).
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
@@ -35,7 +38,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--grid", type=int, nargs=3, default=(512, 512, 104), metavar=("NX", "NY", "NZ"))
-    ap.add_argument("--cpu-crop", type=int, default=256, help="lateral size of the CPU-baseline crop (0: skip)")
+    ap.add_argument("--cpu-crop", type=int, default=512, help="lateral size of the CPU-baseline crop (0: skip)")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 48 snapshots x 4 reflections leg")
     ap.add_argument("--batch-streams", type=int, default=4)
@@ -44,8 +47,8 @@ def parse():
     ap.add_argument("--serve-rate", type=float, default=20.0, help="mean arrivals per second")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch schedules")
-    ap.add_argument("--large", action="store_true",
-                    help="also run the 4000 x 4000 x 68 polar-vortex volume (about 190 GB of HBM) with its FP64 crop")
+    ap.add_argument("--no-large", action="store_true",
+                    help="skip the 4000 x 4000 x 68 polar-vortex volume (about 140 GB of HBM) and its FP64 crop")
     ap.add_argument("--large-grid", type=int, nargs=3, default=(4000, 4000, 68), metavar=("NX", "NY", "NZ"))
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first copy BYTES device to device three times (a kernel of known traffic for counters)")
@@ -133,25 +136,32 @@ def max_dlog10(I, ref):
 
 
 def batch_leg(args):
-    """48 twin-stack snapshots x 4 reflections: one graph replay per batch, against one job at a time."""
+    """48 twin-stack snapshots x 4 reflections: one graph replay per batch, against one job at a
+    time; the default arithmetic (FP64 transforms on FP32 spectra) and FP32 transforms, each job
+    of both against its FP64 volume."""
     import torch
     from qpop_diffraction import workloads
+    from qpop_diffraction.autotune import ScheduleCache
     from qpop_diffraction.batch import Batch, PlanCache, Reflection, plain_job, resident_job
     grids = workloads.batch_grids(48, seed=3)
     log(f"batch: generating 48 snapshots {grids[:4]} ...")
     snaps = [workloads.twin_snapshot(("twin", i), L, L, d, seed=100 + i) for i, (L, d) in enumerate(grids)]
     refls = [Reflection(h, workloads.pto_atoms(h)) for h in workloads.REFLECTIONS]
-    from qpop_diffraction.autotune import ScheduleCache
-    t0 = time.perf_counter()
-    b = Batch(snaps, refls, n_streams=args.batch_streams, graph=True)
-    t_plan = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    b.tune(ScheduleCache(None))
-    t_tune = time.perf_counter() - t0
-    log(f"batch: tuned {len(b.tuning)} plans in {t_tune:.1f} s")
-    b.capture()
+
+    def build(arith):
+        t0 = time.perf_counter()
+        b = Batch(snaps, refls, n_streams=args.batch_streams, graph=True, plans=PlanCache(32, arith=arith))
+        t_plan = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        b.tune(ScheduleCache(None))
+        t_tune = time.perf_counter() - t0
+        b.capture()
+        log(f"batch ({arith}): {len(b.jobs)} jobs, {len(b.plans)} plans in {t_plan:.2f} s, tuned in {t_tune:.1f} s, "
+            f"capture {b.stats.capture_s:.2f} s")
+        return b, t_plan, t_tune
+
+    b, t_plan, t_tune = build("auto")
     vox = sum(j.plan.n for j in b.jobs)
-    log(f"batch: {len(b.jobs)} jobs, {len(b.plans)} plans built in {t_plan:.2f} s, capture {b.stats.capture_s:.2f} s")
     reps = 5
     sec = time_steps(b.run, reps, 1)
     # the same jobs one at a time: resident fields, cached plans, one stream, waiting for each
@@ -162,37 +172,46 @@ def batch_leg(args):
             resident_job(b.plans, snaps[j.snapshot], j.reflection, refls[j.reflection], j.out, work)
             torch.cuda.synchronize()
     sec_serial = time_steps(serial, 2, 1)
+    del work
     # the plain form: what one drop-in call does per job (plan, buffers, intensity to host)
     plain_job(snaps[0], refls[0])
     t0 = time.perf_counter()
     for j in b.jobs:
         plain_job(snaps[j.snapshot], refls[j.reflection])
     sec_plain = time.perf_counter() - t0
-    # accuracy: FP32 jobs against FP64 ones on the largest and smallest snapshots, every reflection
-    order = sorted(range(len(snaps)), key=lambda i: grids[i][0] ** 2 * grids[i][1])
-    worst = 0.0
-    plans64 = PlanCache(64)
+    # the same batch with FP32 transforms
+    b32, _, _ = build("fp32")
+    sec32 = time_steps(b32.run, reps, 1)
+    # accuracy: every job of both batches against its FP64 volume
     b.run()
+    b32.run()
     torch.cuda.synchronize()
-    for i in (order[0], order[-1]):
-        L, d = grids[i]
+    worst, worst32, n_over = 0.0, 0.0, 0
+    plans64 = PlanCache(64)
+    for i, (L, d) in enumerate(grids):
         s64 = workloads.twin_snapshot(("twin", i), L, L, d, seed=100 + i, precision=64)
+        out64 = torch.empty(L * L * d, dtype=torch.float64, device="cuda")
         w64 = None
         for r in range(len(refls)):
-            out64 = torch.empty(L * L * d, dtype=torch.float64, device="cuda")
             plan = plans64.get(s64, r, refls[r])
             if w64 is None or w64.numel() < plan.workspace_bytes:
                 w64 = torch.empty(plan.workspace_bytes, dtype=torch.uint8, device="cuda")
             resident_job(plans64, s64, r, refls[r], out64, w64)
-            worst = max(worst, max_dlog10(b.result(i, r), out64)[0])
-        del s64, w64
+            dl = max_dlog10(b.result(i, r), out64)[0]
+            worst = max(worst, dl)
+            n_over += dl > 1e-3
+            worst32 = max(worst32, max_dlog10(b32.result(i, r), out64)[0])
+        del s64, w64, out64
+        plans64 = PlanCache(64)
     n_jobs = len(b.jobs)
+    arith = sorted({f"{j.plan.compute_bits}/{j.plan.storage_bits}" for j in b.jobs})
     res = {
         "workload": "48 two-variant 90-degree twin-stack snapshots (lateral 128-512 step 64, depth 24-104 from "
-                    f"{list(workloads.DEPTHS)}) x reflections 002, 103, 202, 113 = {n_jobs} volumes, FP32",
+                    f"{list(workloads.DEPTHS)}) x reflections 002, 103, 202, 113 = {n_jobs} volumes, FP32 inputs",
         "grids": [list(g) for g in grids],
         "volumes": n_jobs, "voxels": vox, "streams": b.n_streams, "plans": len(b.plans),
         "plan_build_s": t_plan, "autotune_s": t_tune,
+        "arithmetic": f"transforms/spectra bits per job: {arith}",
         "graph_replay_ms": sec * 1e3, "volumes_per_s": n_jobs / sec, "voxels_per_s": vox / sec,
         "serial_resident_volumes_per_s": n_jobs / sec_serial,
         "plain_volumes_per_s": n_jobs / sec_plain,
@@ -200,12 +219,15 @@ def batch_leg(args):
         "plain_form": "one job at a time as one drop-in diffraction() call does it on resident fields: plan "
                       "built and destroyed, workspace and output allocated, float64 intensity copied to host, "
                       "synchronous",
-        "max_abs_dlog10_I_vs_fp64": worst,
-        "dlog10_sample": "every reflection of the smallest and the largest snapshot",
+        "max_abs_dlog10_I_vs_fp64": worst, "jobs_over_1e-3": int(n_over),
+        "dlog10_sample": f"every job ({n_jobs}) against its FP64 volume",
+        "fp32_transforms": {"volumes_per_s": n_jobs / sec32, "graph_replay_ms": sec32 * 1e3,
+                            "max_abs_dlog10_I_vs_fp64": worst32},
     }
-    log(f"batch: {res['volumes_per_s']:.1f} vol/s graph, {res['serial_resident_volumes_per_s']:.1f} serial, "
-        f"{res['plain_volumes_per_s']:.1f} plain; dlog10 {worst:.2e}")
-    del b, snaps
+    log(f"batch: {res['volumes_per_s']:.1f} vol/s graph ({n_jobs / sec32:.1f} with FP32 transforms), "
+        f"{res['serial_resident_volumes_per_s']:.1f} serial, {res['plain_volumes_per_s']:.1f} plain; dlog10 {worst:.2e} "
+        f"({worst32:.2e} FP32 transforms)")
+    del b, b32, snaps
     torch.cuda.empty_cache()
     return res
 
@@ -248,23 +270,29 @@ def serve_leg(args):
         plain_job(store.acquire(r.key), refls[r.reflection])
         store.release(r.key)
     sec_plain = time.perf_counter() - t0
-    # accuracy: one FP64 job per grid
-    worst = 0.0
+    # accuracy: every served job (as the server delivers it to host memory) against its FP64
+    # volume, in a run of its own (the comparison runs in the result callback)
     plans64 = PlanCache(64)
-    for g in grids:
-        s32 = store.acquire((g, 0))
-        s64 = workloads.hex_snapshot((g, 0), *g, seed=1000, precision=64)
-        n = int(np.prod(g))
-        o32 = torch.empty(n, dtype=torch.float64, device="cuda")
-        o64 = torch.empty(n, dtype=torch.float64, device="cuda")
-        p32 = srv.plans.get(s32, 1, refls[1])
-        p64 = plans64.get(s64, 1, refls[1])
-        w = torch.empty(max(p32.workspace_bytes, p64.workspace_bytes), dtype=torch.uint8, device="cuda")
-        resident_job(srv.plans, s32, 1, refls[1], o32, w)
-        resident_job(plans64, s64, 1, refls[1], o64, w)
-        worst = max(worst, max_dlog10(o32, o64)[0])
-        store.release((g, 0))
-        del s64, w
+    snaps64, acc = {}, {"worst": 0.0, "n": 0, "over": 0}
+
+    def check(req, host):
+        key = req.key
+        if key not in snaps64:
+            snaps64[key] = workloads.hex_snapshot(key, *req.grid, seed=1000 + int(req.snapshot), precision=64)
+        s64 = snaps64[key]
+        plan = plans64.get(s64, req.reflection, refls[req.reflection])
+        o64 = torch.empty(plan.n, dtype=torch.float64, device="cuda")
+        w = torch.empty(plan.workspace_bytes, dtype=torch.uint8, device="cuda")
+        resident_job(plans64, s64, req.reflection, refls[req.reflection], o64, w)
+        dl = max_dlog10(host.to("cuda").reshape(-1), o64)[0]
+        acc["worst"] = max(acc["worst"], dl)
+        acc["n"] += 1
+        acc["over"] += dl > 1e-3
+
+    srv_check = Server(store, refls, n_slots=args.serve_slots, hbm_budget=200e9, on_result=check, plans=srv.plans)
+    srv_check.run(reqs, realtime=False)
+    worst = acc["worst"]
+    del snaps64
     res = {
         "workload": f"Poisson stream, mean {args.serve_rate} req/s, {len(reqs)} jobs; grid uniform from "
                     "128x128x32, 256x256x64, 512x512x104; snapshot id uniform from 16 three-variant hexagonal-twin "
@@ -275,7 +303,8 @@ def serve_leg(args):
         "plain_volumes_per_s": len(reqs) / sec_plain,
         "speedup_vs_plain": sec_plain / cap["elapsed_s"],
         "peak_hbm_bytes": srv.peak_bytes, "deferred_admissions": srv.deferred,
-        "max_abs_dlog10_I_vs_fp64": worst, "dlog10_sample": "snapshot 0 of each grid, reflection 103",
+        "max_abs_dlog10_I_vs_fp64": worst, "jobs_over_1e-3": int(acc["over"]),
+        "dlog10_sample": f"every served job ({acc['n']}) as delivered to host memory, against its FP64 volume",
     }
     log(f"serve: capacity {cap['volumes_per_s']:.1f} vol/s vs plain {res['plain_volumes_per_s']:.1f}")
     return res
@@ -500,10 +529,23 @@ def run():
         tuned = engine.tune(vol, cache=ScheduleCache(None), out=out)
     log("schedules: " + "; ".join(f"pass {p['pass']}: {p['chosen']}" for p in tuned["passes"]))
 
-    log(f"timing FP32: {args.warmup} warmup + {args.steps} steps")
+    log(f"timing FP32 inputs (default arithmetic: {plan.compute_bits}-bit transforms, {plan.storage_bits}-bit "
+        f"spectra): {args.warmup} warmup + {args.steps} steps")
     sec = time_steps(lambda: engine.intensity(vol, out), args.steps, args.warmup)
     I32 = out.clone()
-    log(f"FP32 {sec * 1e3:.3f} ms/volume")
+    log(f"FP32 inputs {sec * 1e3:.3f} ms/volume")
+    # the same volume with FP32 transforms (explicit arithmetic, no accuracy bound)
+    eng_f = Engine(w.params, w.atoms, arith="fp32")
+    out_f = torch.empty(n, dtype=torch.float64, device="cuda")
+    if not args.no_tune:
+        eng_f.tune(vol, cache=ScheduleCache(None), out=out_f)
+    sec_f = time_steps(lambda: eng_f.intensity(vol, out_f), args.steps, args.warmup)
+    plan_f = eng_f.plan(vol.dims, vol.axis, 32)
+    passes_f = pass_times(plan_f, vol, out_f, eng_f.workspace(plan_f), max(5, args.steps))
+    eng_f.intensity(vol, out_f)
+    I32f = out_f.clone()
+    del out_f, eng_f
+    log(f"FP32 transforms {sec_f * 1e3:.3f} ms/volume, passes {passes_f}")
     passes = pass_times(plan, vol, out, work, max(5, args.steps))
     bpc = [plan.pass_bytes_per_cell(p, vol.P, vol.U, vol.phase) for p in range(3)]
     gbs = [bpc[p] * n / (passes[p] * 1e-3) / 1e9 for p in range(3)]
@@ -525,8 +567,13 @@ def run():
         "data": "synthetic (seeded, generated on the GPU)",
         "config": {"workload": f"two-phase Voronoi volume {nx}x{ny}x{nz}, 40 grains, 5-atom PbTiO3-like cell, "
                                f"hkl 002, one volume per step", "grid": [nx, ny, nz], "precision": "fp32",
-                   "launches_per_volume": plan.launches},
+                   "arithmetic": f"FP32 inputs and spectra between passes, {plan.compute_bits}-bit transforms "
+                                 f"(the default of an FP32 plan)",
+                   "pass_dims": list(plan.pass_dims), "launches_per_volume": plan.launches},
         "ms_per_volume_fp32": sec * 1e3,
+        "fp32_transforms": {"ms_per_volume": sec_f * 1e3, "voxels_per_s": n / sec_f, "passes_ms": passes_f,
+                            "pass_dims": list(plan_f.pass_dims),
+                            "note": "explicit FP32 arithmetic (arith='fp32'): no accuracy bound"},
         "kernels": [{"pass": p, "ms": passes[p], "bytes_per_cell": bpc[p], "GB_s": gbs[p],
                      "schedule": tuned["passes"][p]["chosen"]} for p in range(3)],
         "roofline": {"bound": "hbm", "kernel": f"pass {dom}: {tuned['passes'][dom]['chosen']}",
@@ -554,9 +601,12 @@ def run():
         log("timing FP64")
         sec64 = time_steps(lambda: engine.intensity(vol64, out64), max(3, args.steps // 4), 1)
         dl, relpk = max_dlog10(I32, out64)
+        dlf, relpkf = max_dlog10(I32f, out64)
         result["ms_per_volume_fp64"] = sec64 * 1e3
         result["accuracy_fp32_vs_fp64"] = {"max_abs_dlog10_I": dl, "max_abs_dI_over_peak": relpk,
-                                           "floor": "voxels >= 1e-10 of the peak"}
+                                           "floor": "voxels >= 1e-10 of the peak",
+                                           "estimate": plan.dlog10_estimate}
+        result["fp32_transforms"]["accuracy_vs_fp64"] = {"max_abs_dlog10_I": dlf, "max_abs_dI_over_peak": relpkf}
         del vol64, out64, w64
         engine._work.clear()
         torch.cuda.empty_cache()
@@ -568,7 +618,7 @@ def run():
         result["batched"] = batch_leg(args)
     if not args.no_serve:
         result["served"] = serve_leg(args)
-    if args.large:
+    if not args.no_large:
         torch.cuda.empty_cache()
         result["large"] = large_leg(args)
     if args.cpu_crop > 0:

@@ -55,3 +55,69 @@ def test_vortex_fp32_against_fp64_crop():
     assert rel <= 1e-5
     assert dl <= 1e-3
     assert int(I32.argmax()) == int(ref.argmax())
+
+
+def test_small_vortex_on_substrate_against_oracle():
+    """The polar-vortex generator with its 16-cell non-scattering substrate (z-fastest layout,
+    u_z ramp of the substrate's misfit) at a size the oracle handles, FP32 default plan against
+    the FP64 oracle at the FP32 bounds, and the FP64 plan at the reference's bound."""
+    import torch
+    from helpers import check_fp32
+    from qpd_oracle import diffraction_fp64
+    from qpop_diffraction import Engine, Volume, workloads
+    nx, ny, nz = 80, 64, 40
+    print(f"vortex {nx}x{ny}x{nz} on a 16-cell substrate vs oracle", flush=True)
+    out = {}
+    for prec in (32, 64):
+        v = workloads.vortex(nx, ny, nz, substrate=16, slab=40, precision=prec)
+        eng = Engine(v.params, v.atoms)
+        vol = Volume(v.dims, v.axis, v.P, v.U, v.phase, precision=prec)
+        out[prec] = eng.intensity(vol).cpu().numpy().reshape(nx, ny, nz)
+    # the oracle's inputs: (x, y, z) C order is the generator's z-fastest layout itself
+    n = nx * ny * nz
+    P = v.P.double().cpu().numpy().ravel()
+    U = v.U.cpu().numpy().ravel()
+    ph = v.phase.cpu().numpy()
+    oPhase = (ph == 0).astype(np.float64)
+    region = (ph >= 0).astype(np.float64)
+    v32 = workloads.vortex(nx, ny, nz, substrate=16, slab=40, precision=32)
+    P32 = v32.P.double().cpu().numpy().ravel()
+    ref = diffraction_fp64(v.params, v.atoms, oPhase, P, U, region)[0]
+    ref32 = diffraction_fp64(v.params, v.atoms, oPhase, P32, U, region)[0]
+    np.testing.assert_allclose(out[64], ref, rtol=1e-9, atol=1e-12 * ref.max())
+    check_fp32(out[32], ref32, "vortex on substrate")
+    assert n == out[32].size
+
+
+def test_full_size_vortex_fp32_against_fp64_crop():
+    """The 4000 x 4000 x 68 polar-vortex volume whole on one GPU with the default FP32 plan,
+    its 512 x 512 x 68 crop around G against the FP64 crop at the FP32 bounds."""
+    import torch
+    from qpop_diffraction import Engine, Volume, workloads
+    from qpop_diffraction.large import crop_of, fp64_crop
+    torch.cuda.empty_cache()
+    free, total = torch.cuda.mem_get_info()
+    if free < 180e9:
+        pytest.skip(f"needs about 140 GB of free HBM, {free / 1e9:.0f} GB free")
+    print("vortex 4000x4000x68 fp32 vs fp64 crop", flush=True)
+    v = workloads.vortex(4000, 4000, 68, precision=32)
+    eng = Engine(v.params, v.atoms)
+    vol = Volume(v.dims, v.axis, v.P, v.U, v.phase, precision=32)
+    I = eng.intensity(vol)
+    I32 = crop_of(I, v.dims, crop=(512, 512))
+    del I
+    eng._work.clear()
+    torch.cuda.empty_cache()
+    ref = fp64_crop(v, v.params, eng.tables, crop=(512, 512), slab=250)
+    del v, vol
+    torch.cuda.empty_cache()
+    peak = ref.max()
+    rel = ((I32 - ref).abs().max() / peak).item()
+    mask = ref >= 1e-10 * peak
+    dl = (torch.log10(I32[mask]) - torch.log10(ref[mask])).abs().max().item()
+    print(f"  max|dI|/peak {rel:.2e}, max|dlog10 I| {dl:.2e} over {int(mask.sum())} voxels", flush=True)
+    assert rel <= 1e-5
+    assert dl <= 1e-3
+    assert int(I32.argmax()) == int(ref.argmax())
+    del I32, ref
+    torch.cuda.empty_cache()
