@@ -756,18 +756,47 @@ __device__ __forceinline__ void wave_line_transform(cx<T>* line, cx<T> (&v)[SZ::
     }
 }
 
+// The source inputs of an FP64 kernel on FP32 arrays seen as an FP32 pass's arguments (the
+// FP32 source helpers read P, U, the phases, the weights, qC, the axes and n_phase only).
+template <typename T>
+__device__ __forceinline__ PassArgs<float> source_view(const PassArgs<T>& a) {
+    if constexpr (std::is_same<T, float>::value) {
+        return a;
+    } else {
+        PassArgs<float> f{};
+        f.P = reinterpret_cast<const float*>(a.P);
+        f.U = a.U;
+        f.phase = a.phase;
+        f.fraction = reinterpret_cast<const float*>(a.fraction);
+        f.region = reinterpret_cast<const float*>(a.region);
+        f.ncell = a.ncell;
+        f.qC0 = a.qC0;
+        f.qC1 = a.qC1;
+        f.qC2 = a.qC2;
+        f.ax0 = a.ax0;
+        f.ax1 = a.ax1;
+        f.ax2 = a.ax2;
+        f.n_phase = a.n_phase;
+        return f;
+    }
+}
+
 // Three waves per SIMD: the register budget (168) that holds a row's eight cells of raw
 // inputs and the four fields' stage-0 values with at most a couple of spilled values
 // (measured faster than two waves without spills or four with more spills).  Lengths with
 // a radix-11, 13 or 17 stage, whose DFTs need more registers, run two waves per SIMD instead
 // of spilling; FP64 runs one, whose 512 registers (AGPRs included) hold a pair of groups'
 // e and u_d without spills (at two waves it spilled 132 to 340 bytes a lane).
-// IO32 (FP64 only): FP64 arithmetic on FP32 P, weights and outputs.
+// IO32 (FP64 only): FP64 transforms on FP32 P, weights and outputs; the source is formed as in
+// the FP32 kernel (FP64 phase reduction, FP32 values, held in FP32 registers) and each field
+// is widened just before its transform, which keeps two waves per SIMD.
 template <typename T, class SZ, int ROWS, bool IO32 = false>
-__global__ __launch_bounds__(64 * kSrcWaves)
-__attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 1 : (SZ::RMAX >= 11 ? 2 : 3)))) void fast_source_wave(PassArgs<T> a) {
+__global__ __launch_bounds__(64 * kSrcWaves) __attribute__((amdgpu_waves_per_eu(
+    sizeof(T) == 8 ? (IO32 ? 2 : 1) : (SZ::RMAX >= 11 ? 2 : 3)))) void fast_source_wave(PassArgs<T> a) {
     constexpr bool F32 = std::is_same<T, float>::value;
+    constexpr bool SRC32 = F32 || IO32;  // the source formed in FP32 (FP64 phase reduction)
     using TS = io_t<T, IO32>;
+    using TT = std::conditional_t<SRC32, float, T>;  // atom-table entries
     constexpr int N = SZ::N, TPL = SZ::TPL, R0 = SZ::rad(0), NR0 = SZ::nr(0), RL = SZ::RL, NRL = SZ::NRL;
     constexpr int NTHR = 64 * kSrcWaves;
     constexpr int NTW = StageTw<SZ>::total();
@@ -775,7 +804,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 1 : (SZ::RMAX >= 11 ? 2 : 3)
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* lines = reinterpret_cast<cx<T>*>(smem_raw);
     cx<T>* tw = lines + kSrcWaves * ROWS * SZ::PITCH;
-    T* tabl = reinterpret_cast<T*>(tw + NTW);
+    TT* tabl = reinterpret_cast<TT*>(tw + NTW);
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
     const int g = lane / TPL, j = lane - g * TPL;
@@ -787,17 +816,17 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 1 : (SZ::RMAX >= 11 ? 2 : 3)
     fill_stage_tw<T, SZ, 1>(tw, a.W, tid, NTHR);
     fill_stage_tw<T, SZ, 2>(tw, a.W, tid, NTHR);
     fill_stage_tw<T, SZ, 3>(tw, a.W, tid, NTHR);
-    // FP32: the atom table with the su rows in memory-dim order (entry 2 + 3 m + k <-
-    // 2 + 3 ax_m + k); FP64 keeps the table's own order (cell_source picks the axes)
+    // FP32 source: the atom table in FP32 with the su rows in memory-dim order (entry
+    // 2 + 3 m + k <- 2 + 3 ax_m + k); FP64 keeps the table's own order (cell_source picks the axes)
     for (int m = tid; m < a.n_atom_tab; m += NTHR) {
         const int e = m % kAtomTab;
         int src = m;
-        if (F32 && e >= 2 && e < 11) {
+        if (SRC32 && e >= 2 && e < 11) {
             const int dm = (e - 2) / 3;
             const int ax = dm == 0 ? a.ax0 : (dm == 1 ? a.ax1 : a.ax2);
             src = m - e + 2 + 3 * ax + (e - 2) % 3;
         }
-        tabl[m] = a.atomtab[src];
+        tabl[m] = (TT)a.atomtab[src];
     }
     lds_barrier();
     cx<T>* line = lines + (wave * ROWS + (inrow ? g : 0)) * SZ::PITCH;
@@ -829,38 +858,41 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 1 : (SZ::RMAX >= 11 ? 2 : 3)
         }
     };
     // one-hot phases: a group's fields are e and e u_d; each field e u_d is formed just
-    // before its transform, and F[e u_0] waits for F[e], which is transformed last, in place
-    auto emit_eu = [&](cx<T>(&e)[SZ::RMAX], T(&u)[3][R0], int grp, uint32_t lane_cell) {
+    // before its transform (in T from the held e and u, FP32 for an FP32 source), and
+    // F[e u_0] waits for F[e], which is transformed last
+    auto emit_eu = [&](auto& e, auto& u, int grp, uint32_t lane_cell) {
         cx<TS>* oA = reinterpret_cast<cx<TS>*>(a.out) + (size_t)grp * 3 * a.ncell + (size_t)lane_cell;
         cx<TS>* oB = oA + a.ncell;
         cx<TS>* oC = oB + a.ncell;
         cx<T> w[SZ::RMAX], f0[RL];
 #pragma unroll
-        for (int r = 0; r < R0; ++r) w[r] = cscale(e[r], u[0][r]);
+        for (int r = 0; r < R0; ++r) w[r] = cscale(cx_as<T>(e[r]), (T)u[0][r]);
         wave_line_transform<T, SZ>(line, w, j, inrow, tw);
 #pragma unroll
         for (int r = 0; r < RL; ++r) f0[r] = w[r];
 #pragma unroll
-        for (int r = 0; r < R0; ++r) w[r] = cscale(e[r], u[1][r]);
+        for (int r = 0; r < R0; ++r) w[r] = cscale(cx_as<T>(e[r]), (T)u[1][r]);
         wave_line_transform<T, SZ>(line, w, j, inrow, tw);
         if (live && j < NRL) {
 #pragma unroll
             for (int r = 0; r < RL; ++r) oB[r * NRL] = cx_as<TS>(w[r]);
         }
 #pragma unroll
-        for (int r = 0; r < R0; ++r) w[r] = cscale(e[r], u[2][r]);
+        for (int r = 0; r < R0; ++r) w[r] = cscale(cx_as<T>(e[r]), (T)u[2][r]);
         wave_line_transform<T, SZ>(line, w, j, inrow, tw);
         if (live && j < NRL) {
 #pragma unroll
             for (int r = 0; r < RL; ++r) oC[r * NRL] = cx_as<TS>(w[r]);
         }
-        wave_line_transform<T, SZ>(line, e, j, inrow, tw);
+#pragma unroll
+        for (int r = 0; r < R0; ++r) w[r] = cx_as<T>(e[r]);
+        wave_line_transform<T, SZ>(line, w, j, inrow, tw);
         if (live && j < NRL) {
 #pragma unroll
-            for (int r = 0; r < RL; ++r) oA[r * NRL] = cx_as<TS>(e[r] + cscale(negi(f0[r]), kj[r * NRL]));
+            for (int r = 0; r < RL; ++r) oA[r * NRL] = cx_as<TS>(w[r] + cscale(negi(f0[r]), kj[r * NRL]));
         }
     };
-    if constexpr (!F32) {
+    if constexpr (!SRC32) {
         // FP64: the phase of each atom formed whole in FP64; one-hot phases keep e and u_d
         // (5 values a cell) of a pair of groups per read of the cells, fractional ones the
         // four fields of one group
@@ -913,6 +945,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 1 : (SZ::RMAX >= 11 ? 2 : 3)
         // from one read of the cells and only e and u_d (5 values a cell) are kept.  Each
         // field e u_d is formed just before its transform; F[e u_0] waits for F[e], which is
         // transformed last, in place.
+        const PassArgs<float> af = source_view(a);
         for (int grp = 0; grp < a.ngroups; grp += 2) {
             const bool pair = grp + 1 < a.ngroups;
             const int srcA = a.chunk_src[a.group_first[grp]];
@@ -922,9 +955,9 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 1 : (SZ::RMAX >= 11 ? 2 : 3)
             // made from it, is held across the transforms
             uint32_t lane_cell = cell0 + (uint32_t)j;
             asm volatile("" : "+v"(lane_cell));
-            const CellPtrsF cp = cell_ptrs_f(a, (size_t)lane_cell);
-            cx<T> eA[SZ::RMAX], eB[SZ::RMAX];
-            T uA[3][R0], uB[3][R0];
+            const CellPtrsF cp = cell_ptrs_f(af, (size_t)lane_cell);
+            cx<float> eA[SZ::RMAX], eB[SZ::RMAX];
+            float uA[3][R0], uB[3][R0];
             // at most kAhead cells' loads in flight: cell r + kAhead is requested once cell r
             // has been formed
             constexpr int kAhead = R0 < 4 ? R0 : 4;
@@ -935,10 +968,10 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 1 : (SZ::RMAX >= 11 ? 2 : 3)
             for (int r = 0; r < R0; ++r) {
                 __builtin_amdgcn_sched_barrier(0);
                 CellInputsF ci;
-                cell_prepare_f(a, raw[r], ci);
+                cell_prepare_f(af, raw[r], ci);
                 if (r + kAhead < R0) raw_load_f(cp, (r + kAhead) * NR0, has0, raw[r + kAhead]);
-                cell_source_1h(a, tabl, ci, srcA, eA[r], uA[0][r], uA[1][r], uA[2][r]);
-                if (pair) cell_source_1h(a, tabl, ci, srcB, eB[r], uB[0][r], uB[1][r], uB[2][r]);
+                cell_source_1h(af, tabl, ci, srcA, eA[r], uA[0][r], uA[1][r], uA[2][r]);
+                if (pair) cell_source_1h(af, tabl, ci, srcB, eB[r], uB[0][r], uB[1][r], uB[2][r]);
             }
             __builtin_amdgcn_sched_barrier(0);
             emit_eu(eA, uA, grp, lane_cell);
@@ -946,7 +979,8 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 1 : (SZ::RMAX >= 11 ? 2 : 3)
         }
     } else {
         // fractional phases: one group per read of the cells
-        const CellPtrsF cp = cell_ptrs_f(a, (size_t)cell0 + (size_t)j);
+        const PassArgs<float> af = source_view(a);
+        const CellPtrsF cp = cell_ptrs_f(af, (size_t)cell0 + (size_t)j);
         for (int grp = 0; grp < a.ngroups; ++grp) {
             const int src = a.chunk_src[a.group_first[grp]];
             cx<T> v[4][SZ::RMAX];
@@ -957,9 +991,13 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 1 : (SZ::RMAX >= 11 ? 2 : 3)
             for (int r = 0; r < R0; ++r) {
                 __builtin_amdgcn_sched_barrier(0);
                 CellInputsF ci;
-                cell_prepare_f(a, raw[r], ci);
-                cell_source_m(a, tabl, ci, cell0 + (uint32_t)(j + r * NR0), src, v[0][r], v[1][r], v[2][r],
-                              v[3][r]);
+                cell_prepare_f(af, raw[r], ci);
+                cx<float> s0, s1, s2, s3;
+                cell_source_m(af, tabl, ci, cell0 + (uint32_t)(j + r * NR0), src, s0, s1, s2, s3);
+                v[0][r] = cx_as<T>(s0);
+                v[1][r] = cx_as<T>(s1);
+                v[2][r] = cx_as<T>(s2);
+                v[3][r] = cx_as<T>(s3);
             }
             __builtin_amdgcn_sched_barrier(0);
             emit(v, grp);
