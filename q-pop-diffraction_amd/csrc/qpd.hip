@@ -1068,9 +1068,9 @@ std::string describe_launch(const Launch& L) {
                                                "split-step", "strided", "intensity", "groups", "groups",
                                                "groups", "groups", "source-wave"};
     if (L.fe)
-        std::snprintf(buf, sizeof(buf), "fast %d (%s) G%d %s%s%s%s", L.fe->N, L.fe->radices, L.fe->G,
+        std::snprintf(buf, sizeof(buf), "fast %d (%s) G%d %s%s%s%s%s", L.fe->N, L.fe->radices, L.fe->G,
                       role_name[L.fe->role], L.fe->no_prefetch ? " np" : "", L.fe->real_bytes == 8 ? " f64" : "",
-                      L.fe->io32 ? " io32" : "");
+                      L.fe->io32 ? " io32" : "", L.fe->one_buffer ? " 1buf" : "");
     else
         std::snprintf(buf, sizeof(buf), "generic %s%s nl%d nf%d nthr%d ept%d%s%s", L.contiguous ? "contiguous" : "strided",
                       L.mix ? " groups" : "", L.nl, L.nf, L.nthr, L.ept, L.pow2 ? " pow2" : "", L.io32 ? " io32" : "");

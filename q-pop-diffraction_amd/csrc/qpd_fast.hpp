@@ -355,14 +355,18 @@ __device__ __forceinline__ void stage_read(const cx<T>* tile, cx<T> (&v)[F][SZ::
 // X[j + r NRL] in v.  No barrier is pending on return.  The next call may write
 // the buffer this call read last only after a barrier, so callers alternate the
 // roles of X and Y between calls when the number of writes (chunk_writes) is odd.
+// ONEBUF (LOAD0 false): X is Y, and a barrier also precedes every write, after the
+// reads of the same buffer before it (this call's or the previous call's); half the
+// LDS at twice the barriers.
 template <class SZ, bool LOAD0>
 constexpr int chunk_writes() {
     return SZ::NST - 1 + (LOAD0 ? 1 : 0);
 }
 
-template <typename T, class SZ, int F, bool STRIDED, int LP, bool LOAD0>
+template <typename T, class SZ, int F, bool STRIDED, int LP, bool LOAD0, bool ONEBUF = false>
 __device__ __forceinline__ void transform(cx<T>* X, cx<T>* Y, cx<T> (&v)[F][SZ::RMAX], int j, int line0, int dline,
                                           const cx<T>* __restrict__ W) {
+    static_assert(!(ONEBUF && LOAD0), "one buffer holds the stage-0 inputs only when they are in registers");
     cx<T>* rd = X;  // buffer holding the current stage's inputs
     cx<T>* wr = LOAD0 ? Y : X;
     const bool a0 = j < SZ::nr(0);
@@ -371,6 +375,7 @@ __device__ __forceinline__ void transform(cx<T>* X, cx<T>* Y, cx<T> (&v)[F][SZ::
     }
     if (a0) stage_compute<T, SZ, 0, F>(v, j, W);
     if constexpr (SZ::NST > 1) {
+        if constexpr (ONEBUF) lds_barrier();
         if (a0) stage_write<T, SZ, 0, F, STRIDED, LP>(wr, v, j, line0, dline);
         lds_barrier();
         rd = wr;
@@ -381,6 +386,10 @@ __device__ __forceinline__ void transform(cx<T>* X, cx<T>* Y, cx<T> (&v)[F][SZ::
             stage_compute<T, SZ, 1, F>(v, j, W);
         }
         if constexpr (SZ::NST > 2) {
+            if constexpr (ONEBUF) {
+                lds_barrier();
+                wr = X;
+            }
             if (a1) stage_write<T, SZ, 1, F, STRIDED, LP>(wr, v, j, line0, dline);
             lds_barrier();
             rd = wr;
@@ -391,6 +400,10 @@ __device__ __forceinline__ void transform(cx<T>* X, cx<T>* Y, cx<T> (&v)[F][SZ::
                 stage_compute<T, SZ, 2, F>(v, j, W);
             }
             if constexpr (SZ::NST > 3) {
+                if constexpr (ONEBUF) {
+                    lds_barrier();
+                    wr = X;
+                }
                 if (a2) stage_write<T, SZ, 2, F, STRIDED, LP>(wr, v, j, line0, dline);
                 lds_barrier();
                 rd = wr;
@@ -1012,17 +1025,19 @@ __global__ __launch_bounds__(64 * kSrcWaves) __attribute__((amdgpu_waves_per_eu(
 // PF: the next input is prefetched into registers while the current one is transformed;
 // without it the kernel needs fewer registers and more blocks fit on a CU.
 // IO32 (FP64 only): FP64 arithmetic on complex FP32 inputs and outputs.
-template <typename T, class SZ, int G, int NIN, bool INTENSITY, bool SPLIT, bool PF = true, bool IO32 = false>
+// ONEBUF: one LDS stage buffer (transform's ONEBUF), for more blocks per CU.
+template <typename T, class SZ, int G, int NIN, bool INTENSITY, bool SPLIT, bool PF = true, bool IO32 = false,
+          bool ONEBUF = false>
 __global__ __launch_bounds__(G* SZ::TPL) void fast_strided(PassArgs<T> a) {
     using TI = io_t<T, IO32>;
     // the NIN inputs of a chunk are transformed one after another (one line per thread)
     constexpr int N = SZ::N, R0 = SZ::rad(0), NR0 = SZ::nr(0), LP = SZ::strided_lp(G);
     constexpr int NOUT = NIN > 1 ? NIN - 1 : 1;
     constexpr int TILE = SZ::strided_tile(G);
-    constexpr bool ALT = (chunk_writes<SZ, false>() & 1) == 1;
+    constexpr bool ALT = !ONEBUF && (chunk_writes<SZ, false>() & 1) == 1;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* bufA = reinterpret_cast<cx<T>*>(smem_raw);
-    cx<T>* bufB = bufA + TILE;
+    cx<T>* bufB = ONEBUF ? bufA : bufA + TILE;
     cx<T>* Wl = bufB + TILE;
     const int tid = threadIdx.x;
     const int g = tid % G, j = tid / G;
@@ -1085,7 +1100,7 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided(PassArgs<T> a) {
                 } else {
                     load_item(it, v);
                 }
-                transform<T, SZ, 1, true, LP, false>(par ? bufB : bufA, par ? bufA : bufB, v, j, g, 0, Wl);
+                transform<T, SZ, 1, true, LP, false, ONEBUF>(par ? bufB : bufA, par ? bufA : bufB, v, j, g, 0, Wl);
                 if (ALT) par ^= 1;
                 if (j < SZ::NRL) {
                     const uint32_t coff = c_lane * (uint32_t)sizeof(cx<T>);
@@ -1143,15 +1158,16 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided(PassArgs<T> a) {
 //   out_{Z,1} = sum_S sum_{Y in Z} c_Y(k1) X_{Y,S}(k0) F1 C_S,
 // the same outputs as the class form.  Each group's three inputs are transformed once.
 // IO32 (FP64 only): FP64 arithmetic on complex FP32 inputs and outputs.
-template <typename T, class SZ, int G, int NZ, bool SPLIT, bool IO32 = false>
+// ONEBUF: one LDS stage buffer (transform's ONEBUF), for more blocks per CU.
+template <typename T, class SZ, int G, int NZ, bool SPLIT, bool IO32 = false, bool ONEBUF = false>
 __global__ __launch_bounds__(G* SZ::TPL) void fast_strided_mix(PassArgs<T> a) {
     using TI = io_t<T, IO32>;
     constexpr int N = SZ::N, R0 = SZ::rad(0), NR0 = SZ::nr(0), LP = SZ::strided_lp(G), RL = SZ::RL, NRL = SZ::NRL;
     constexpr int TILE = SZ::strided_tile(G);
-    constexpr bool ALT = (chunk_writes<SZ, false>() & 1) == 1;
+    constexpr bool ALT = !ONEBUF && (chunk_writes<SZ, false>() & 1) == 1;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* bufA = reinterpret_cast<cx<T>*>(smem_raw);
-    cx<T>* bufB = bufA + TILE;
+    cx<T>* bufB = ONEBUF ? bufA : bufA + TILE;
     cx<T>* Wl = bufB + TILE;
     const int tid = threadIdx.x;
     const int g = tid % G, j = tid / G;
@@ -1206,7 +1222,7 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided_mix(PassArgs<T> a) {
             } else {
                 load_item(it, v);
             }
-            transform<T, SZ, 1, true, LP, false>(par ? bufB : bufA, par ? bufA : bufB, v, j, g, 0, Wl);
+            transform<T, SZ, 1, true, LP, false, ONEBUF>(par ? bufB : bufA, par ? bufA : bufB, v, j, g, 0, Wl);
             if (ALT) par ^= 1;
             if (j < NRL) {
                 const uint32_t koff = c_lane * (uint32_t)sizeof(T);

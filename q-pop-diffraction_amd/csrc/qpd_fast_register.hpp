@@ -66,43 +66,55 @@ void add_fast(std::vector<FastEntry>& v, const char* radices) {
 // 256 threads: a wave may then use 512 registers (AGPRs included) instead of spilling.
 // IO32: the same kernels computing in FP64 on FP32 arrays (P, weights, the spectra between
 // passes), for FP32 plans whose transforms run in FP64.
-template <class SZ, int ROLE, int G, bool PF = true, bool IO32 = false>
+template <class SZ, int ROLE, int G, bool PF = true, bool IO32 = false, bool ONEBUF = false>
 hipError_t fast_launch64(const Launch& L, const PassArgs<double>& a, hipStream_t st) {
     if constexpr (ROLE == kRoleSource)
         return launch_one(fast_source<double, SZ, G, IO32>, L, st, a);
     else if constexpr (ROLE == kRoleSourceW)
         return launch_one(fast_source_wave<double, SZ, G / kSrcWaves, IO32>, L, st, a);
     else if constexpr (ROLE == kRoleStrided)
-        return launch_one(fast_strided<double, SZ, G, 3, false, false, true, IO32>, L, st, a);
+        return launch_one(fast_strided<double, SZ, G, 3, false, false, true, IO32, ONEBUF>, L, st, a);
     else if constexpr (ROLE == kRoleIntensity)
-        return launch_one(fast_strided<double, SZ, G, 2, true, false, PF, IO32>, L, st, a);
+        return launch_one(fast_strided<double, SZ, G, 2, true, false, PF, IO32, ONEBUF>, L, st, a);
     else if constexpr (ROLE == kRoleMix1)
-        return launch_one(fast_strided_mix<double, SZ, G, 1, false, IO32>, L, st, a);
+        return launch_one(fast_strided_mix<double, SZ, G, 1, false, IO32, ONEBUF>, L, st, a);
     else
-        return launch_one(fast_strided_mix<double, SZ, G, 2, false, IO32>, L, st, a);
+        return launch_one(fast_strided_mix<double, SZ, G, 2, false, IO32, ONEBUF>, L, st, a);
 }
 
-template <class SZ, int ROLE, int G, bool PF = true, bool IO32 = false>
+// ONEBUF (strided roles): one LDS stage buffer instead of two
+template <class SZ, int ROLE, int G, bool PF = true, bool IO32 = false, bool ONEBUF = false>
 void add_fast64(std::vector<FastEntry>& v, const char* radices) {
     constexpr bool src = ROLE == kRoleSource;
-    if constexpr (G >= (src ? 1 : 4) && (src ? 4 : 1) * G * SZ::TPL <= 256) {
+    if constexpr (G >= (src ? 1 : 4) && (src ? 4 : 1) * G * SZ::TPL <= 256 && !(src && ONEBUF)) {
         constexpr size_t tile = src ? (size_t)4 * G * SZ::PITCH : (size_t)SZ::strided_tile(G);
-        constexpr size_t lds = (2 * tile + SZ::N) * sizeof(cx<double>);
+        constexpr size_t lds = ((ONEBUF ? 1 : 2) * tile + SZ::N) * sizeof(cx<double>);
         if constexpr (lds <= 160 * 1024)
             v.push_back({SZ::N, ROLE, G, (src ? 4 : 1) * G * SZ::TPL, SZ::TPL, lds, radices, nullptr, nullptr, !PF,
-                         8, &fast_launch64<SZ, ROLE, G, PF, IO32>, IO32});
+                         8, &fast_launch64<SZ, ROLE, G, PF, IO32, ONEBUF>, IO32, ONEBUF});
     }
 }
 
+// FP32 plans computing in FP64 (IO32): the intensity pass lists its tiles with one LDS
+// stage buffer first (the two-buffer tile of a 512-point pass, 82 KB, leaves one block of
+// four waves on a CU, one buffer, 45 KB, two: 0.72 -> 0.46 ms at 512x512x104); the plain
+// strided pass lists it after the two-buffer tile.  The pass over source groups holds
+// about 390 registers a thread, one wave per SIMD whatever its LDS, and keeps two buffers.
 template <class SZ, int ROLE, bool IO32>
 void add_fast64_tiles(std::vector<FastEntry>& v, const char* radices) {
     constexpr int G = fast_tile(ROLE, SZ::TPL) / (ROLE == kRoleSource ? 1 : 2);
-    add_fast64<SZ, ROLE, G, true, IO32>(v, radices);
     if constexpr (ROLE == kRoleIntensity) {
+        if constexpr (IO32) {
+            add_fast64<SZ, ROLE, G, false, IO32, true>(v, radices);
+            add_fast64<SZ, ROLE, G, true, IO32, true>(v, radices);
+        }
+        add_fast64<SZ, ROLE, G, true, IO32>(v, radices);
         add_fast64<SZ, ROLE, G, false, IO32>(v, radices);
         add_fast64<SZ, ROLE, G / 2, false, IO32>(v, radices);
-    } else if constexpr (ROLE != kRoleSource) {
-        add_fast64<SZ, ROLE, G / 2, true, IO32>(v, radices);
+    } else {
+        add_fast64<SZ, ROLE, G, true, IO32>(v, radices);
+        if constexpr (IO32 && ROLE == kRoleStrided) add_fast64<SZ, ROLE, G, true, IO32, true>(v, radices);
+        if constexpr (ROLE != kRoleSource) add_fast64<SZ, ROLE, G / 2, true, IO32>(v, radices);
     }
 }
 

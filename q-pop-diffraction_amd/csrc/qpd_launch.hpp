@@ -77,6 +77,7 @@ struct FastEntry {
     int real_bytes = 4;  // 4: an FP32 kernel (launch), 8: an FP64 one (launch64)
     FastLaunchFn64 launch64 = nullptr;
     bool io32 = false;   // FP64 kernel on FP32 arrays (P, weights, spectra): FP32 plans computing in FP64
+    bool one_buffer = false;  // strided FP64 kernel with one LDS stage buffer
 };
 
 // Raise a kernel's dynamic LDS limit once per (kernel, device) and size.

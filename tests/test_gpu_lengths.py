@@ -110,13 +110,16 @@ def volume_check(refl, nx, ny, nz, seed=0, expect=None, schedules=None, pick=Non
                                                  ("source", 0, "class", "mixed"),
                                                  ("strided", 1, "class", "mixed"),
                                                  ("groups", 1, "groups", "mixed"),
-                                                 ("intensity", 2, "groups", "mixed")])
+                                                 ("intensity", 2, "groups", "mixed"),
+                                                 ("strided", 1, "class", "mixed-1buf"),
+                                                 ("intensity", 2, "groups", "mixed-1buf")])
 def test_volume_every_length(refl, role, dim, form, arith, monkeypatch):
     """Pass 1 runs the source-group kernel for PbTiO3 unless QPD_CLASS_FORM=1; pass 0 of
     the source-group form defaults to the one-wave-per-row kernel where the length has one,
     and the block-per-row source kernel is picked explicitly for the "source" case.  With
     arith "mixed" the pass runs the FP64 kernel of the length on FP32 arrays (FP32 inputs and
-    spectra); a length whose FP64 tile does not fit runs the generic kernel."""
+    spectra), "mixed-1buf" its variant with one LDS stage buffer; a length whose FP64 tile
+    does not fit runs the generic kernel."""
     if form == "class":
         monkeypatch.setenv("QPD_CLASS_FORM", "1")
     else:
@@ -124,7 +127,9 @@ def test_volume_every_length(refl, role, dim, form, arith, monkeypatch):
     worst = 0.0
     lengths = fast_lengths("strided_groups2" if role == "groups" else role)
     suffix = None
-    if arith == "mixed":
+    if arith == "mixed-1buf":
+        arith, suffix = "mixed", " f64 io32 1buf"
+    elif arith == "mixed":
         suffix = " source-wave f64 io32" if role == "source_wave" else " f64 io32"
     ran_fast = 0
     for N in lengths:
