@@ -47,17 +47,6 @@ struct Sizes {
     static_assert(R0_ * (R1_ > 1 ? R1_ : 1) * (R2_ > 1 ? R2_ : 1) * (R3_ > 1 ? R3_ : 1) == N_, "radices must multiply to N");
 };
 
-// Workgroups are dispatched to the 8 XCDs round robin (block b to XCD b % 8), and each
-// XCD has its own L2.  Strided passes read rows of G columns; neighbouring column tiles
-// share 128-byte lines, so the logical block b / 8 + (b % 8) * (nblocks / 8) puts runs of
-// consecutive tiles on one XCD, where the second read of a line hits L2.  The last
-// nblocks % 8 blocks keep their index (a bijection on [0, nblocks)).
-__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nblocks) {
-    constexpr uint32_t kXcd = 8;
-    const uint32_t per = nblocks / kXcd;
-    return b < per * kXcd ? (b % kXcd) * per + b / kXcd : b;
-}
-
 // Per-cell inputs of the FP32 source with the phase split exp(-i qC.u_a) =
 // exp(-i qC.U) exp(-i g_a.P): e0 = exp(-i qC.U) is formed once per cell with the
 // FP64-reduced sincos; per atom only g_a.P (a few radians) remains.

@@ -91,6 +91,17 @@ struct PassArgs {
     int ax0, ax1, ax2;        // physical axis of memory dims 0, 1, 2
 };
 
+// Workgroups are dispatched to the 8 XCDs round robin (block b to XCD b % 8), and each
+// XCD has its own L2.  Strided passes read rows of G columns; neighbouring column tiles
+// share 128-byte lines, so the logical block b / 8 + (b % 8) * (nblocks / 8) puts runs of
+// consecutive tiles on one XCD, where the second read of a line hits L2.  The last
+// nblocks % 8 blocks keep their index (a bijection on [0, nblocks)).
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nblocks) {
+    constexpr uint32_t kXcd = 8;
+    const uint32_t per = nblocks / kXcd;
+    return b < per * kXcd ? (b % kXcd) * per + b / kXcd : b;
+}
+
 // v[i] for a run-time i in 0..2 without indexing a private array.
 __device__ __forceinline__ double pick3(double v0, double v1, double v2, int i) {
     return i == 0 ? v0 : (i == 1 ? v1 : v2);
@@ -314,8 +325,10 @@ __global__ __launch_bounds__(512) void pass_strided(PassArgs<T> a) {
     const int N = a.N;
     const int TX = 1 << a.log2TX;
     const int stride = a.nf * TX;
-    const uint32_t t0 = blockIdx.x % a.ntiles0;
-    const uint32_t outer = blockIdx.x / a.ntiles0;
+    // narrow tiles share HBM lines with their neighbours: keep neighbours on one XCD
+    const uint32_t blk = TX * sizeof(cx<TI>) < 128 ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    const uint32_t t0 = blk % a.ntiles0;
+    const uint32_t outer = blk / a.ntiles0;
     const int tx = tid & (TX - 1);
     const int r0 = tid >> a.log2TX;
     const int rstep = nthr >> a.log2TX;
@@ -407,8 +420,10 @@ __global__ __launch_bounds__(512) void pass_strided_mix(PassArgs<T> a) {
     const int N = a.N;
     const int TX = 1 << a.log2TX;
     const int stride = a.nf * TX;
-    const uint32_t t0 = blockIdx.x % a.ntiles0;
-    const uint32_t outer = blockIdx.x / a.ntiles0;
+    // narrow tiles share HBM lines with their neighbours: keep neighbours on one XCD
+    const uint32_t blk = TX * sizeof(cx<TI>) < 128 ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    const uint32_t t0 = blk % a.ntiles0;
+    const uint32_t outer = blk / a.ntiles0;
     const int tx = tid & (TX - 1);
     const int r0 = tid >> a.log2TX;
     const int rstep = nthr >> a.log2TX;
