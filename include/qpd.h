@@ -229,6 +229,20 @@ int qpd_intensity(qpd_plan* plan, const qpd_fields* fields, double* I_dev, void*
 int qpd_intensity_pass(qpd_plan* plan, const qpd_fields* fields, double* I_dev, void* work_dev, int pass,
                        void* stream);
 
+/* qpd_intensity without a workspace argument.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The intermediate spectra go to a static device
+ * array of QPD_NOALLOC_SCRATCH_BYTES that every GPU the library runs on holds until the
+ * process ends; a plan whose qpd_workspace_bytes exceeds it is rejected with
+ * QPD_ERR_UNSUPPORTED before anything is enqueued.  The call allocates nothing (no hipMalloc,
+ * no hipFree); it uses the current device, which must be the plan's.  All calls on one GPU
+ * share that GPU's array, which keeps the last call's contents: calls that overlap on one GPU
+ * (enqueued on different streams, or from several threads without ordering) race, so keep
+ * them on one stream.  No part of the array has to start at zero: each pass writes every
+ * intermediate before the next pass reads it. */
+#define QPD_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int qpd_intensity_noalloc(qpd_plan* plan, const qpd_fields* fields, double* I_dev, void* stream);
+
 /* Algorithmic bytes one launch of pass `pass` moves per cell: each input read once and
  * each output written once (fields, intensity and source inputs as the plan uses them). */
 double qpd_pass_bytes_per_cell(const qpd_plan* plan, const qpd_fields* fields, int pass);

@@ -974,6 +974,33 @@ int run_intensity(qpd_plan* p, const qpd_fields* f, double* I, void* work, hipSt
     return QPD_OK;
 }
 
+// Scratch of qpd_intensity_noalloc: the runtime gives every device the code object loads on
+// its own copy of this array.  Host code never passes the symbol itself to a kernel or a copy
+// (on the host it names a stand-in, not a device address): each device's address is looked
+// up once, with that device current, and kept in g_scratch.
+__device__ __align__(256) unsigned char scratch[QPD_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch[kMaxScratchDevices] = {};
+
+// The scratch address on the current device (no allocation: a symbol lookup on the first
+// call per device, a table read after it).
+int scratch_of_current_device(void** out) {
+    int dev = 0;
+    QPD_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxScratchDevices) return fail(QPD_ERR_UNSUPPORTED, "device index beyond 63");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch[dev]) {
+        void* ptr = nullptr;
+        QPD_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) return fail(QPD_ERR_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch[dev] = ptr;
+    }
+    *out = g_scratch[dev];
+    return QPD_OK;
+}
+
 // Plain 3-D transform plans, cached by (dims, precision): every field is one group
 // with one chunk and the identity coefficient, so each pass is a batched 1-D FFT.
 constexpr int kMaxPlainFields = 64;
@@ -1479,6 +1506,21 @@ int qpd_intensity_pass(qpd_plan* p, const qpd_fields* f, double* I, void* work, 
     if (pass < 0 || pass > 2) return fail(QPD_ERR_ARG, "pass must be 0, 1 or 2");
     hipStream_t st = static_cast<hipStream_t>(stream);
     return run_intensity(p, f, I, work, st, pass);
+}
+
+int qpd_intensity_noalloc(qpd_plan* p, const qpd_fields* f, double* I, void* stream) {
+    if (!p || !f || !I) return fail(QPD_ERR_ARG, "null argument");
+    if (!f->P || !f->U || !f->phase) return fail(QPD_ERR_ARG, "P, U and phase are required");
+    const size_t need = qpd_workspace_bytes(p);
+    if (need > QPD_NOALLOC_SCRATCH_BYTES)
+        return fail(QPD_ERR_UNSUPPORTED, "the plan needs " + std::to_string(need) +
+                                             " bytes of workspace; qpd_intensity_noalloc holds at most " +
+                                             std::to_string(QPD_NOALLOC_SCRATCH_BYTES) +
+                                             " (use qpd_intensity with a workspace)");
+    void* work = nullptr;
+    const int rc = scratch_of_current_device(&work);
+    if (rc != QPD_OK) return rc;
+    return run_intensity(p, f, I, work, static_cast<hipStream_t>(stream));
 }
 
 double qpd_pass_bytes_per_cell(const qpd_plan* p, const qpd_fields* f, int pass) {

@@ -21,10 +21,15 @@ EXPORTS = (
     "qpd_plan_create", "qpd_plan_create_opts", "qpd_plan_info", "qpd_plan_destroy", "qpd_workspace_bytes",
     "qpd_launches_per_volume",
     "qpd_plan_schedules", "qpd_plan_schedule_name", "qpd_plan_set_schedule", "qpd_plan_get_schedule",
-    "qpd_fast_lengths", "qpd_intensity", "qpd_intensity_pass", "qpd_pass_bytes_per_cell", "qpd_fft3d",
+    "qpd_fast_lengths", "qpd_intensity", "qpd_intensity_pass", "qpd_intensity_noalloc", "qpd_pass_bytes_per_cell",
+    "qpd_fft3d",
     "qpd_fft_length_supported", "qpd_ingest_fields", "qpd_project", "qpd_reverse_axes", "qpd_last_error",
     "qpd_version",
 )
+
+# QPD_NOALLOC_SCRATCH_BYTES: the static per-GPU scratch of qpd_intensity_noalloc
+NOALLOC_SCRATCH_BYTES = 64 << 20
+NO_WORKSPACE = object()  # _check_inputs: the call brings no workspace
 
 ROLES = {"source": 0, "contiguous": 1, "strided": 2, "intensity": 3, "strided1": 4, "split": 5,
          "strided_split": 6, "intensity_split": 7, "strided_groups1": 8, "strided_groups2": 9,
@@ -104,6 +109,8 @@ def load_library():
     lib.qpd_intensity.restype = i32
     lib.qpd_intensity_pass.argtypes = [vp, ctypes.POINTER(Fields), vp, vp, i32, vp]
     lib.qpd_intensity_pass.restype = i32
+    lib.qpd_intensity_noalloc.argtypes = [vp, ctypes.POINTER(Fields), vp, vp]
+    lib.qpd_intensity_noalloc.restype = i32
     lib.qpd_pass_bytes_per_cell.argtypes = [vp, ctypes.POINTER(Fields), i32]
     lib.qpd_pass_bytes_per_cell.restype = ctypes.c_double
     lib.qpd_fft3d.argtypes = [ctypes.POINTER(i64), i32, vp, i32, vp]
@@ -219,11 +226,14 @@ class Plan:
             need(I, "I", torch.float64, n)
         else:
             I = None
-        if not isinstance(work, torch.Tensor) or not work.is_cuda or not work.is_contiguous():
-            raise ValueError("work must be a contiguous CUDA tensor")
-        if work.numel() * work.element_size() < self.workspace_bytes:
-            raise ValueError(f"work holds {work.numel() * work.element_size()} bytes, the plan needs "
-                             f"{self.workspace_bytes}")
+        if work is not NO_WORKSPACE:
+            if not isinstance(work, torch.Tensor) or not work.is_cuda or not work.is_contiguous():
+                raise ValueError("work must be a contiguous CUDA tensor")
+            if work.numel() * work.element_size() < self.workspace_bytes:
+                raise ValueError(f"work holds {work.numel() * work.element_size()} bytes, the plan needs "
+                                 f"{self.workspace_bytes}")
+        else:
+            work = None
         devs = {t.device for t in (P, U, phase, fraction, region, I, work) if t is not None}
         if len(devs) != 1:
             raise ValueError(f"buffers are on several devices: {sorted(str(d) for d in devs)}")
@@ -234,6 +244,18 @@ class Plan:
         fields = self._fields(P, U, phase, fraction, region)
         check(load_library().qpd_intensity(self._h, ctypes.byref(fields), I.data_ptr(), work.data_ptr(),
                                            stream if stream is not None else current_stream()))
+
+    def intensity_noalloc(self, P, U, phase, I, fraction=None, region=None, stream=None):
+        """intensity() without a workspace: the library's static per-GPU scratch
+        (NOALLOC_SCRATCH_BYTES) holds the intermediate spectra and nothing is allocated.
+        Calls that overlap on one GPU share that scratch and race; keep them on one stream."""
+        self._check_inputs(P, U, phase, I, NO_WORKSPACE, fraction, region)
+        if self.workspace_bytes > NOALLOC_SCRATCH_BYTES:
+            raise ValueError(f"the plan needs {self.workspace_bytes} bytes of workspace; the no-allocation "
+                             f"entry point holds at most {NOALLOC_SCRATCH_BYTES}")
+        fields = self._fields(P, U, phase, fraction, region)
+        check(load_library().qpd_intensity_noalloc(self._h, ctypes.byref(fields), I.data_ptr(),
+                                                   stream if stream is not None else current_stream()))
 
     def _fields(self, P, U, phase, fraction, region):
         return Fields(P.data_ptr(), U.data_ptr(), phase.data_ptr(),
