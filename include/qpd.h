@@ -253,11 +253,13 @@ double qpd_pass_bytes_per_cell(const qpd_plan* plan, const qpd_fields* fields, i
  * xp.fft.fftn performs at diffraction.py:105-106, exposed for testing. */
 int qpd_fft3d(const int64_t dims[3], int precision, void* data_dev, int n_fields, void* stream);
 
-/* Whether this build can factor a 1-D transform length (all prime factors <= 17, at most
- * 65536).  A plan needs more: a launch shape for the axis the length is on (contiguous rows
- * up to 4096 cells; FP32 strided axes of any such length, split in two launches where they
- * have no kernel of their own; FP64 strided axes up to 4096); qpd_plan_create reports
- * QPD_ERR_UNSUPPORTED otherwise. */
+/* Whether this build can transform a 1-D length: any length up to 65536 (prime factors up to
+ * 17 as butterfly stages, larger ones as direct stages of O(R) work per output).  A plan
+ * needs more: a launch shape for the axis the length is on (contiguous rows up to 8192
+ * cells; FP32 strided axes of any such length, split in two launches where it factors into
+ * lengths with kernels of their own, else up to 8192; FP64 strided axes up to 8192);
+ * qpd_plan_create reports QPD_ERR_UNSUPPORTED otherwise.  Replaces the any-length transforms
+ * of xp.fft.fftn (diffraction.py:105-106) and HeFFTe (qpop_diffraction_mgpu.cu:154-157). */
 int qpd_fft_length_supported(int64_t n);
 
 /* Message of the calling thread's last error ("" when none). */

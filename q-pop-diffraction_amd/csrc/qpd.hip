@@ -43,7 +43,9 @@ int fail(int code, const std::string& msg) {
     } while (0)
 
 // ---------------------------------------------------------------------------
-// Axis factorisation: radix 8 while possible, then 4 or 2, then the odd primes.
+// Axis factorisation: radix 8 while possible, then 4 or 2, then the odd primes up to 17
+// (butterfly stages), then any larger prime factors in ascending order (direct stages,
+// O(R) work per output: fft_stage_direct).
 bool factorise(int64_t n, std::vector<int>& radices) {
     radices.clear();
     if (n < 1) return false;
@@ -66,7 +68,14 @@ bool factorise(int64_t n, std::vector<int>& radices) {
             m /= p;
         }
     }
-    return m == 1;
+    for (int64_t p = 19; p * p <= m; p += 2) {
+        while (m % p == 0) {
+            radices.push_back((int)p);
+            m /= p;
+        }
+    }
+    if (m > 1) radices.push_back((int)m);
+    return true;
 }
 
 bool make_axis(int64_t N, AxisFFT& ax) {
@@ -137,6 +146,10 @@ int max_lds_bytes() {
 bool stage_fits(const AxisFFT& ax, int lines, int nthr, int mr) {
     for (int s = 0; s < ax.nst; ++s) {
         const int R = ax.st[s].R;
+        if (R > 17) {  // direct stage: mr outputs a thread
+            if (((int64_t)ax.N * lines + nthr - 1) / nthr > mr) return false;
+            continue;
+        }
         const int mb = std::max(1, mr / R);
         const int64_t total = (int64_t)(ax.N / R) * lines;
         if ((total + nthr - 1) / nthr > mb) return false;
@@ -174,11 +187,14 @@ bool choose_launch(const AxisFFT& ax, int nin, bool contiguous, size_t csize, in
     }
     // FP64 axes whose stages are all radix 2, 4 or 8 run the POW2 kernels (8 values a thread)
     // where a shape fits them, else the all-radix ones (16)
-    bool pow2_axis = csize == 16;
+    // lines of more than 4096 points (ept_max 16) run the sixteen-a-thread kernels, whose
+    // stages hold kMaxRegsLong values a thread, and have no POW2 form
+    const bool long_line = ept_max >= 16;
+    bool pow2_axis = csize == 16 && !long_line;
     for (int s = 0; s < ax.nst; ++s) pow2_axis = pow2_axis && (ax.st[s].R == 2 || ax.st[s].R == 4 || ax.st[s].R == 8);
     for (int attempt = pow2_axis ? 0 : 1; attempt < 2; ++attempt) {
         const bool pow2 = attempt == 0;
-        const int mr = pow2 ? kMaxRegsPow2 : kMaxRegs;
+        const int mr = long_line ? kMaxRegsLong : (pow2 ? kMaxRegsPow2 : kMaxRegs);
         for (int pass = 0; pass < 2; ++pass) {
             const int cap = pass == 0 ? lds_soft : lds_hard;
             for (auto [nf, p] : order) {
@@ -201,7 +217,7 @@ bool choose_launch(const AxisFFT& ax, int nin, bool contiguous, size_t csize, in
                     L.nf = nf;
                     L.nthr = nthr;
                     const int need = (int)((elems + nthr - 1) / nthr);
-                    L.ept = need <= 2 ? 2 : (need <= 4 ? 4 : 8);
+                    L.ept = long_line ? 16 : (need <= 2 ? 2 : (need <= 4 ? 4 : 8));
                     if (L.ept > ept_max) continue;
                     L.pow2 = pow2 ? 1 : 0;
                     L.lds = lds;
@@ -606,76 +622,74 @@ void set_fast(Launch& L, const FastEntry* fe, size_t lds) {
 // transform fields (1 output); strided passes have 2 outputs, or 1 when they
 // write the intensity or run a plain transform.
 // The generic kernels of one pass; POW2 instantiations only for FP64 (their FP32 forms
-// do not spill).
+// do not spill).  Elements a thread: the smallest instance holding L.ept; 16 only for lines
+// of more than 4096 points (up to 8192: 512 threads of 16), which spill but run.
+template <typename T>
+using GenericKernel = void (*)(PassArgs<T>);
+
+template <typename T>
+hipError_t launch_ept(const Launch& L, const PassArgs<T>& a, hipStream_t st, GenericKernel<T> k2, GenericKernel<T> k4,
+                      GenericKernel<T> k8, GenericKernel<T> k16) {
+    GenericKernel<T> k = L.ept <= 2 ? k2 : L.ept <= 4 ? k4 : L.ept <= 8 ? k8 : L.ept <= 16 ? k16 : nullptr;
+    if (!k) return hipErrorInvalidValue;
+    return launch_one(k, L, st, a);
+}
+
+template <typename T, bool POW2, bool IO32, int NOUT, bool SOURCE>
+hipError_t launch_contig(const Launch& L, const PassArgs<T>& a, hipStream_t st) {
+    GenericKernel<T> k16 = nullptr;
+    if constexpr (!POW2) k16 = pass_contig<T, 16, NOUT, SOURCE, POW2, IO32>;
+    // two elements a thread where they suffice: the FP64 source kernel holds each element's
+    // cell inputs and accumulators across the groups; the plain transform starts at four
+    return launch_ept<T>(L, a, st,
+                         SOURCE ? pass_contig<T, 2, NOUT, SOURCE, POW2, IO32> : pass_contig<T, 4, NOUT, SOURCE, POW2, IO32>,
+                         pass_contig<T, 4, NOUT, SOURCE, POW2, IO32>, pass_contig<T, 8, NOUT, SOURCE, POW2, IO32>, k16);
+}
+
+template <typename T, bool POW2, bool IO32, int NOUT, bool INTENSITY>
+hipError_t launch_strided(const Launch& L, const PassArgs<T>& a, hipStream_t st) {
+    GenericKernel<T> k16 = nullptr;
+    if constexpr (!POW2) k16 = pass_strided<T, 16, NOUT, INTENSITY, POW2, IO32>;
+    // four elements a thread where they suffice: fewer registers (FP64 kernels spill)
+    return launch_ept<T>(L, a, st, pass_strided<T, 4, NOUT, INTENSITY, POW2, IO32>,
+                         pass_strided<T, 4, NOUT, INTENSITY, POW2, IO32>, pass_strided<T, 8, NOUT, INTENSITY, POW2, IO32>,
+                         k16);
+}
+
+template <typename T, bool POW2, bool IO32, int NZ>
+hipError_t launch_mix(const Launch& L, const PassArgs<T>& a, hipStream_t st) {
+    GenericKernel<T> k16 = nullptr;
+    if constexpr (!POW2) k16 = pass_strided_mix<T, 16, NZ, POW2, IO32>;
+    return launch_ept<T>(L, a, st, pass_strided_mix<T, 4, NZ, POW2, IO32>, pass_strided_mix<T, 4, NZ, POW2, IO32>,
+                         pass_strided_mix<T, 8, NZ, POW2, IO32>, k16);
+}
+
+template <typename T, bool POW2, bool IO32>
+hipError_t launch_generic_io(const Launch& L, const PassArgs<T>& a, bool source, bool intensity, hipStream_t st) {
+    if (L.contiguous) {
+        // FP64 on FP32 arrays: only the source pass (plain transforms never read FP32 spectra)
+        if (source) return a.nout != 3 ? hipErrorInvalidValue : launch_contig<T, POW2, IO32, 3, true>(L, a, st);
+        if (IO32 || a.nout != 1) return hipErrorInvalidValue;
+        return launch_contig<T, POW2, IO32, 1, false>(L, a, st);
+    }
+    if (a.pair_first) {  // pass 1 over source groups: one or two z-classes
+        if (a.ngroups == 1) return launch_mix<T, POW2, IO32, 1>(L, a, st);
+        if (a.ngroups == 2) return launch_mix<T, POW2, IO32, 2>(L, a, st);
+        return hipErrorInvalidValue;
+    }
+    if (intensity) return a.nout != 1 ? hipErrorInvalidValue : launch_strided<T, POW2, IO32, 1, true>(L, a, st);
+    if (a.nout == 2) return launch_strided<T, POW2, IO32, 2, false>(L, a, st);
+    if (IO32 || a.nout != 1) return hipErrorInvalidValue;
+    return launch_strided<T, POW2, IO32, 1, false>(L, a, st);
+}
+
 template <typename T, bool POW2>
 hipError_t launch_generic(const Launch& L, const PassArgs<T>& a, bool source, bool intensity, hipStream_t st) {
     if constexpr (std::is_same<T, double>::value) {
         // an FP32 plan computing in FP64: FP32 P, weights and spectra
-        if (L.io32 && L.contiguous) {
-            if (!source || a.nout != 3 || L.ept > 8) return hipErrorInvalidValue;
-            return L.ept <= 2   ? launch_one(pass_contig<T, 2, 3, true, POW2, true>, L, st, a)
-                   : L.ept <= 4 ? launch_one(pass_contig<T, 4, 3, true, POW2, true>, L, st, a)
-                                : launch_one(pass_contig<T, 8, 3, true, POW2, true>, L, st, a);
-        }
+        if (L.io32) return launch_generic_io<T, POW2, true>(L, a, source, intensity, st);
     }
-    if (L.contiguous) {
-        if (source) {
-            // two elements a thread where they suffice: the FP64 source kernel holds each
-            // element's cell inputs and accumulators across the groups; eight only for rows
-            // longer than 2048 cells
-            if (a.nout != 3 || L.ept > 8) return hipErrorInvalidValue;
-            return L.ept <= 2   ? launch_one(pass_contig<T, 2, 3, true, POW2>, L, st, a)
-                   : L.ept <= 4 ? launch_one(pass_contig<T, 4, 3, true, POW2>, L, st, a)
-                                : launch_one(pass_contig<T, 8, 3, true, POW2>, L, st, a);
-        }
-        if (a.nout != 1 || L.ept > 8) return hipErrorInvalidValue;
-        return L.ept <= 4 ? launch_one(pass_contig<T, 4, 1, false, POW2>, L, st, a)
-                          : launch_one(pass_contig<T, 8, 1, false, POW2>, L, st, a);
-    }
-    if (L.ept > 8) return hipErrorInvalidValue;
-    // four elements a thread where they suffice: fewer registers (FP64 kernels spill)
-    const bool e4 = L.ept <= 4;
-    if constexpr (std::is_same<T, double>::value) {
-        // an FP32 plan computing in FP64: complex FP32 inputs and outputs
-        if (L.io32) {
-            if (a.pair_first) {
-                if (a.ngroups == 1)
-                    return e4 ? launch_one(pass_strided_mix<T, 4, 1, POW2, true>, L, st, a)
-                              : launch_one(pass_strided_mix<T, 8, 1, POW2, true>, L, st, a);
-                if (a.ngroups == 2)
-                    return e4 ? launch_one(pass_strided_mix<T, 4, 2, POW2, true>, L, st, a)
-                              : launch_one(pass_strided_mix<T, 8, 2, POW2, true>, L, st, a);
-                return hipErrorInvalidValue;
-            }
-            if (intensity)
-                return a.nout != 1 ? hipErrorInvalidValue
-                       : e4        ? launch_one(pass_strided<T, 4, 1, true, POW2, true>, L, st, a)
-                                   : launch_one(pass_strided<T, 8, 1, true, POW2, true>, L, st, a);
-            if (a.nout != 2) return hipErrorInvalidValue;
-            return e4 ? launch_one(pass_strided<T, 4, 2, false, POW2, true>, L, st, a)
-                      : launch_one(pass_strided<T, 8, 2, false, POW2, true>, L, st, a);
-        }
-    }
-    if (a.pair_first) {  // pass 1 over source groups: one or two z-classes
-        if (a.ngroups == 1)
-            return e4 ? launch_one(pass_strided_mix<T, 4, 1, POW2>, L, st, a)
-                      : launch_one(pass_strided_mix<T, 8, 1, POW2>, L, st, a);
-        if (a.ngroups == 2)
-            return e4 ? launch_one(pass_strided_mix<T, 4, 2, POW2>, L, st, a)
-                      : launch_one(pass_strided_mix<T, 8, 2, POW2>, L, st, a);
-        return hipErrorInvalidValue;
-    }
-    if (intensity) {
-        if (a.nout != 1) return hipErrorInvalidValue;
-        return e4 ? launch_one(pass_strided<T, 4, 1, true, POW2>, L, st, a)
-                  : launch_one(pass_strided<T, 8, 1, true, POW2>, L, st, a);
-    }
-    if (a.nout == 2)
-        return e4 ? launch_one(pass_strided<T, 4, 2, false, POW2>, L, st, a)
-                  : launch_one(pass_strided<T, 8, 2, false, POW2>, L, st, a);
-    if (a.nout != 1) return hipErrorInvalidValue;
-    return e4 ? launch_one(pass_strided<T, 4, 1, false, POW2>, L, st, a)
-              : launch_one(pass_strided<T, 8, 1, false, POW2>, L, st, a);
+    return launch_generic_io<T, POW2, false>(L, a, source, intensity, st);
 }
 
 template <typename T>
@@ -788,6 +802,10 @@ int pass_candidates(const AxisFFT& ax, int d, int md, const int64_t dims[3], int
     bool fits = choose_launch(ax, nin, contig, csize, lines, soft, hard, ept_max, L);
     if (!fits && contig && nin > 1) {
         ept_max = 8;
+        fits = choose_launch(ax, nin, contig, csize, lines, soft, hard, ept_max, L);
+    }
+    if (!fits) {  // lines of more than 4096 points: sixteen a thread
+        ept_max = 16;
         fits = choose_launch(ax, nin, contig, csize, lines, soft, hard, ept_max, L);
     }
     if (fits) {

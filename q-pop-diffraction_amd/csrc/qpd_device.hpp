@@ -122,6 +122,10 @@ constexpr int kMaxStages = 16;
 // fft_tile (FP64 only: the same 64 VGPRs as 16 FP32 values; they spilled to scratch with 16).
 constexpr int kMaxRegs = 16;
 constexpr int kMaxRegsPow2 = 8;
+// values a thread holds in the stages of the kernels with sixteen elements a thread (lines
+// of more than 4096 points): twice kMaxRegs, so that every radix's butterflies of a line of
+// 8192 points fit 512 threads
+constexpr int kMaxRegsLong = 32;
 
 struct Stage {
     int R;
@@ -354,11 +358,65 @@ __device__ __forceinline__ void fft_stage(cx<T>* __restrict__ tile, const cx<T>*
     __syncthreads();
 }
 
+// A stage of prime radix R above 17 (the leftover factors of any length): each output is
+// formed directly, y_t = sum_r x[j + r NR] w^(r k tws) w^(r t NR) with w = exp(-2 pi i / N)
+// (the stage twiddle and the DFT_R root in one table entry, index r (k tws + t NR) mod N),
+// O(R) work an output.  Output pos = q Ns R + t Ns + k of butterfly j = q Ns + k, as in
+// fft_stage.  A thread forms up to MO outputs in registers, the block synchronises, then they
+// are written in place.  The sum runs in blocks of 16 terms (partial sums added last), which
+// keeps the FP32 rounding of a long sum near that of a tree.
+template <typename T, int MO = kMaxRegs>
+__device__ __forceinline__ void fft_stage_direct(cx<T>* __restrict__ tile, const cx<T>* __restrict__ W, int R,
+                                                 int NR, int stride, int Lc, FastDiv divLc, int Ns, FastDiv divNs,
+                                                 int tws, int tid, int nthr) {
+    const int N = NR * R;
+    const int total = N * Lc;
+    cx<T> y[MO];
+#pragma unroll
+    for (int b = 0; b < MO; ++b) {
+        const int e = tid + b * nthr;
+        y[b] = {(T)0, (T)0};
+        if (e < total) {
+            const int pos = (int)fdiv((uint32_t)e, divLc);
+            const int line = e - pos * Lc;
+            const int qt = (int)fdiv((uint32_t)pos, divNs);  // q R + t
+            const int k = pos - qt * Ns;
+            const int q = qt / R, t = qt - q * R;
+            const cx<T>* x = tile + (q * Ns + k) * stride + line;
+            const int step = k * tws + t * NR;  // < N
+            const int xstep = NR * stride;
+            int idx = 0;
+            cx<T> acc = {(T)0, (T)0};
+            for (int r0 = 0; r0 < R; r0 += 16) {
+                const int r1 = min(R, r0 + 16);
+                cx<T> part = {(T)0, (T)0};
+                for (int r = r0; r < r1; ++r) {
+                    part = part + cmul(x[r * xstep], W[idx]);
+                    idx += step;
+                    if (idx >= N) idx -= N;
+                }
+                acc = acc + part;
+            }
+            y[b] = acc;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < MO; ++b) {
+        const int e = tid + b * nthr;
+        if (e < total) {
+            const int pos = (int)fdiv((uint32_t)e, divLc);
+            tile[pos * stride + (e - pos * Lc)] = y[b];
+        }
+    }
+    __syncthreads();
+}
+
 // Forward transform of lines [0, Lc) of an LDS tile (see fft_stage), natural order in and out.
 // POW2: every stage has radix 2, 4 or 8, and the odd-radix stages are not compiled in (a
 // kernel's registers are those of its largest stage: the FP64 radix-17 stage alone holds
 // about 200 VGPRs).
-template <typename T, bool POW2 = false>
+template <typename T, bool POW2 = false, int MR = kMaxRegs>
 __device__ __forceinline__ void fft_tile(cx<T>* tile, int nst, const Stage* __restrict__ stages,
                                          const cx<T>* __restrict__ W, int stride, int Lc, FastDiv divLc, int tid,
                                          int nthr) {
@@ -375,16 +433,18 @@ __device__ __forceinline__ void fft_tile(cx<T>* tile, int nst, const Stage* __re
             }
         } else {
             switch (sd.R) {
-                case 2: fft_stage<2, T>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
-                case 3: fft_stage<3, T>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
-                case 4: fft_stage<4, T>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
-                case 5: fft_stage<5, T>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
-                case 7: fft_stage<7, T>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
-                case 8: fft_stage<8, T>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
-                case 11: fft_stage<11, T>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
-                case 13: fft_stage<13, T>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
-                case 17: fft_stage<17, T>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
-                default: break;  // the host accepts no other radix
+                case 2: fft_stage<2, T, MR>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
+                case 3: fft_stage<3, T, MR>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
+                case 4: fft_stage<4, T, MR>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
+                case 5: fft_stage<5, T, MR>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
+                case 7: fft_stage<7, T, MR>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
+                case 8: fft_stage<8, T, MR>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
+                case 11: fft_stage<11, T, MR>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
+                case 13: fft_stage<13, T, MR>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
+                case 17: fft_stage<17, T, MR>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
+                default:  // primes above 17
+                    fft_stage_direct<T, MR>(tile, W, sd.R, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr);
+                    break;
             }
         }
     }

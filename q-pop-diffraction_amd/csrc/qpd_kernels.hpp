@@ -282,7 +282,7 @@ __global__ __launch_bounds__(512) void pass_contig(PassArgs<T> a) {
                     }
                 }
                 __syncthreads();
-                fft_tile<T, POW2>(tile, a.nst, a.stages, a.W, stride, nfc * nl, lines_div(a, nfc), tid, nthr);
+                fft_tile<T, POW2, EPT >= 16 ? kMaxRegsLong : kMaxRegs>(tile, a.nst, a.stages, a.W, stride, nfc * nl, lines_div(a, nfc), tid, nthr);
 #pragma unroll
                 for (int m = 0; m < EPT; ++m) {
                     const int e = tid + m * nthr;
@@ -359,7 +359,7 @@ __global__ __launch_bounds__(512) void pass_strided(PassArgs<T> a) {
                         dst[f * TX] = col_ok ? cx_as<T>(inbase[(size_t)f * a.ncell + off]) : cx<T>{(T)0, (T)0};
                 }
                 __syncthreads();
-                fft_tile<T, POW2>(tile, a.nst, a.stages, a.W, stride, nfc * TX, lines_div(a, nfc), tid, nthr);
+                fft_tile<T, POW2, EPT >= 16 ? kMaxRegsLong : kMaxRegs>(tile, a.nst, a.stages, a.W, stride, nfc * TX, lines_div(a, nfc), tid, nthr);
 #pragma unroll
                 for (int m = 0; m < EPT; ++m) {
                     const int idx = r0 + m * rstep;
@@ -453,7 +453,7 @@ __global__ __launch_bounds__(512) void pass_strided_mix(PassArgs<T> a) {
                     dst[f * TX] = col_ok ? cx_as<T>(inbase[(size_t)f * a.ncell + off]) : cx<T>{(T)0, (T)0};
             }
             __syncthreads();
-            fft_tile<T, POW2>(tile, a.nst, a.stages, a.W, stride, nfc * TX, lines_div(a, nfc), tid, nthr);
+            fft_tile<T, POW2, EPT >= 16 ? kMaxRegsLong : kMaxRegs>(tile, a.nst, a.stages, a.W, stride, nfc * TX, lines_div(a, nfc), tid, nthr);
 #pragma unroll
             for (int m = 0; m < EPT; ++m) {
                 const int idx = r0 + m * rstep;

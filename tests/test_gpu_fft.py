@@ -1,9 +1,10 @@
 """The hand-written Stockham 3-D FFT (qpd_fft3d) against numpy.fft, FP64 and FP32.
 
-Lengths cover every radix the library implements (2, 3, 4, 5, 7, 8, 11, 13, 17),
-length-1 axes, and the axis lengths of the workloads (104, 400, 512, 2000, 68).
-FP64 must match to 1e-12 of the largest output; FP32 (FP64-generated twiddles)
-to 2e-6 of it.
+Lengths cover every butterfly radix the library implements (2, 3, 4, 5, 7, 8, 11, 13, 17),
+the direct stages of larger prime factors (19, 23, 38 = 2 x 19, 46, 97, 4001, 5003), rows of
+more than 4096 points (5003, 8000: sixteen points a thread), length-1 axes, and the axis
+lengths of the workloads (104, 400, 512, 2000, 68).  FP64 must match to 1e-12 of the largest
+output; FP32 (FP64-generated twiddles) to 2e-6 of it.
 """
 
 import numpy as np
@@ -14,6 +15,8 @@ pytestmark = pytest.mark.gpu
 SHAPES = [  # (d2, d1, d0) = memory order, d0 fastest
     (1, 1, 1), (1, 1, 8), (4, 4, 4), (3, 5, 7), (11, 13, 17), (8, 16, 512), (13, 8, 104),
     (2, 17, 68), (1, 400, 400), (1, 20, 2000), (5, 9, 25), (6, 10, 12), (104, 4, 6), (17, 3, 11),
+    (1, 1, 19), (3, 23, 38), (46, 2, 19), (97, 5, 23), (1, 2, 4001), (4001, 1, 3), (2, 4001, 2),
+    (1, 2, 5003), (1, 2, 8000), (8000, 2, 1), (3, 8000, 2),
 ]
 
 
@@ -38,8 +41,9 @@ def test_fft3d_matches_numpy(shape, precision):
 
 
 def test_fft3d_rejects_unsupported_length():
+    """Rows of more than 8192 points have no launch shape; the error names the length."""
     import torch
     from qpop_diffraction._native import QPDError, fft3d
-    t = torch.zeros((1, 1, 19), dtype=torch.complex64, device="cuda")
-    with pytest.raises(QPDError):
+    t = torch.zeros((1, 1, 8200), dtype=torch.complex64, device="cuda")
+    with pytest.raises(QPDError, match="8200"):
         fft3d(t)

@@ -35,7 +35,8 @@ def test_library_exports_every_declared_symbol():
 def test_host_only_entry_points():
     lib = _native.load_library()
     assert lib.qpd_version() >= 100
-    for n, ok in [(1, 1), (2, 1), (104, 1), (4000, 1), (68, 1), (19, 0), (2 * 23, 0), (65536, 1), (65536 * 2, 0)]:
+    for n, ok in [(1, 1), (2, 1), (104, 1), (4000, 1), (68, 1), (19, 1), (23, 1), (38, 1), (2 * 23, 1), (97, 1),
+                  (4001, 1), (65521, 1), (65536, 1), (65537, 0), (65536 * 2, 0)]:
         assert lib.qpd_fft_length_supported(n) == ok, n
     assert lib.qpd_last_error() is not None
 
