@@ -146,10 +146,7 @@ int max_lds_bytes() {
 bool stage_fits(const AxisFFT& ax, int lines, int nthr, int mr) {
     for (int s = 0; s < ax.nst; ++s) {
         const int R = ax.st[s].R;
-        if (R > 17) {  // direct stage: mr outputs a thread
-            if (((int64_t)ax.N * lines + nthr - 1) / nthr > mr) return false;
-            continue;
-        }
+        if (R > 17) continue;  // direct stage: any number of outputs a thread
         const int mb = std::max(1, mr / R);
         const int64_t total = (int64_t)(ax.N / R) * lines;
         if ((total + nthr - 1) / nthr > mb) return false;
@@ -190,6 +187,8 @@ bool choose_launch(const AxisFFT& ax, int nin, bool contiguous, size_t csize, in
     // lines of more than 4096 points (ept_max 16) run the sixteen-a-thread kernels, whose
     // stages hold kMaxRegsLong values a thread, and have no POW2 form
     const bool long_line = ept_max >= 16;
+    bool direct = false;
+    for (int s = 0; s < ax.nst; ++s) direct = direct || ax.st[s].R > 17;
     bool pow2_axis = csize == 16 && !long_line;
     for (int s = 0; s < ax.nst; ++s) pow2_axis = pow2_axis && (ax.st[s].R == 2 || ax.st[s].R == 4 || ax.st[s].R == 8);
     for (int attempt = pow2_axis ? 0 : 1; attempt < 2; ++attempt) {
@@ -202,7 +201,8 @@ bool choose_launch(const AxisFFT& ax, int nin, bool contiguous, size_t csize, in
                     if (contiguous && p > 1 && (int64_t)p > n_lines_total) continue;
                     const int64_t elems = (int64_t)p * N;      // one field's tile
                     if (contiguous && elems > 4096 && p > 1) continue;
-                    const size_t lds = (size_t)nf * elems * csize;
+                    // axes with a direct stage (a prime factor above 17) hold a copy of the tile
+                    const size_t lds = (size_t)nf * elems * csize * (direct ? 2 : 1);
                     if (lds > (size_t)cap) continue;
                     // threads: aim at ~ept_max/2 elements per thread, at most 512
                     int nthr = round_up((int)std::min<int64_t>(512, (elems + ept_max / 2 - 1) / (ept_max / 2)), 64);

@@ -362,52 +362,47 @@ __device__ __forceinline__ void fft_stage(cx<T>* __restrict__ tile, const cx<T>*
 // formed directly, y_t = sum_r x[j + r NR] w^(r k tws) w^(r t NR) with w = exp(-2 pi i / N)
 // (the stage twiddle and the DFT_R root in one table entry, index r (k tws + t NR) mod N),
 // O(R) work an output.  Output pos = q Ns R + t Ns + k of butterfly j = q Ns + k, as in
-// fft_stage.  A thread forms up to MO outputs in registers, the block synchronises, then they
-// are written in place.  The sum runs in blocks of 16 terms (partial sums added last), which
-// keeps the FP32 rounding of a long sum near that of a tree.
-template <typename T, int MO = kMaxRegs>
-__device__ __forceinline__ void fft_stage_direct(cx<T>* __restrict__ tile, const cx<T>* __restrict__ W, int R,
-                                                 int NR, int stride, int Lc, FastDiv divLc, int Ns, FastDiv divNs,
-                                                 int tws, int tid, int nthr) {
+// fft_stage.  The stage's inputs are first copied to the second half of the LDS (the host
+// gives kernels of such axes twice the tile, N * stride points after it), so that each
+// thread forms and stores its outputs one at a time, with no outputs held in registers
+// across the barrier (which would cost every kernel that compiles this stage registers).
+// The sum runs in blocks of 16 terms (partial sums added last), which keeps the FP32
+// rounding of a long sum near that of a tree.
+template <typename T>
+__device__ __forceinline__ void fft_stage_direct(cx<T>* __restrict__ tile, const cx<T>* __restrict__ W, int R, int NR,
+                                                  int stride, int Lc, FastDiv divLc, int Ns, FastDiv divNs, int tws,
+                                                  int tid, int nthr) {
     const int N = NR * R;
     const int total = N * Lc;
-    cx<T> y[MO];
-#pragma unroll
-    for (int b = 0; b < MO; ++b) {
-        const int e = tid + b * nthr;
-        y[b] = {(T)0, (T)0};
-        if (e < total) {
-            const int pos = (int)fdiv((uint32_t)e, divLc);
-            const int line = e - pos * Lc;
-            const int qt = (int)fdiv((uint32_t)pos, divNs);  // q R + t
-            const int k = pos - qt * Ns;
-            const int q = qt / R, t = qt - q * R;
-            const cx<T>* x = tile + (q * Ns + k) * stride + line;
-            const int step = k * tws + t * NR;  // < N
-            const int xstep = NR * stride;
-            int idx = 0;
-            cx<T> acc = {(T)0, (T)0};
-            for (int r0 = 0; r0 < R; r0 += 16) {
-                const int r1 = min(R, r0 + 16);
-                cx<T> part = {(T)0, (T)0};
-                for (int r = r0; r < r1; ++r) {
-                    part = part + cmul(x[r * xstep], W[idx]);
-                    idx += step;
-                    if (idx >= N) idx -= N;
-                }
-                acc = acc + part;
-            }
-            y[b] = acc;
-        }
+    cx<T>* src = tile + (size_t)N * stride;
+    for (int e = tid; e < total; e += nthr) {
+        const int pos = (int)fdiv((uint32_t)e, divLc);
+        const int at = pos * stride + (e - pos * Lc);
+        src[at] = tile[at];
     }
     __syncthreads();
-#pragma unroll
-    for (int b = 0; b < MO; ++b) {
-        const int e = tid + b * nthr;
-        if (e < total) {
-            const int pos = (int)fdiv((uint32_t)e, divLc);
-            tile[pos * stride + (e - pos * Lc)] = y[b];
+    for (int e = tid; e < total; e += nthr) {
+        const int pos = (int)fdiv((uint32_t)e, divLc);
+        const int line = e - pos * Lc;
+        const int qt = (int)fdiv((uint32_t)pos, divNs);  // q R + t
+        const int k = pos - qt * Ns;
+        const int q = qt / R, t = qt - q * R;
+        const cx<T>* x = src + (q * Ns + k) * stride + line;
+        const int step = k * tws + t * NR;  // < N
+        const int xstep = NR * stride;
+        int idx = 0;
+        cx<T> acc = {(T)0, (T)0};
+        for (int r0 = 0; r0 < R; r0 += 16) {
+            const int r1 = min(R, r0 + 16);
+            cx<T> part = {(T)0, (T)0};
+            for (int r = r0; r < r1; ++r) {
+                part = part + cmul(x[r * xstep], W[idx]);
+                idx += step;
+                if (idx >= N) idx -= N;
+            }
+            acc = acc + part;
         }
+        tile[pos * stride + line] = acc;
     }
     __syncthreads();
 }
@@ -443,7 +438,7 @@ __device__ __forceinline__ void fft_tile(cx<T>* tile, int nst, const Stage* __re
                 case 13: fft_stage<13, T, MR>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
                 case 17: fft_stage<17, T, MR>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
                 default:  // primes above 17
-                    fft_stage_direct<T, MR>(tile, W, sd.R, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr);
+                    fft_stage_direct<T>(tile, W, sd.R, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr);
                     break;
             }
         }
