@@ -182,15 +182,19 @@ bool choose_launch(const AxisFFT& ax, int nin, bool contiguous, size_t csize, in
         for (int p : per)
             for (int nf : nfs) order.push_back({nf, p});
     }
-    // FP64 axes whose stages are all radix 2, 4 or 8 run the POW2 kernels (8 values a thread)
-    // where a shape fits them, else the all-radix ones (16)
+    // FP64 axes whose stages are all radix 2, 4 or 8 run the kRadixPow2 kernels (8 values a
+    // thread) where a shape fits them, else those of radices up to 8 or of every radix (16)
     // lines of more than 4096 points (ept_max 16) run the sixteen-a-thread kernels, whose
-    // stages hold kMaxRegsLong values a thread, and have no POW2 form
+    // stages hold kMaxRegsLong values a thread, and are compiled for every radix only
     const bool long_line = ept_max >= 16;
     bool direct = false;
     for (int s = 0; s < ax.nst; ++s) direct = direct || ax.st[s].R > 17;
     bool pow2_axis = csize == 16 && !long_line;
-    for (int s = 0; s < ax.nst; ++s) pow2_axis = pow2_axis && (ax.st[s].R == 2 || ax.st[s].R == 4 || ax.st[s].R == 8);
+    bool small_axis = csize == 16 && !long_line;
+    for (int s = 0; s < ax.nst; ++s) {
+        pow2_axis = pow2_axis && (ax.st[s].R == 2 || ax.st[s].R == 4 || ax.st[s].R == 8);
+        small_axis = small_axis && ax.st[s].R <= 8;
+    }
     for (int attempt = pow2_axis ? 0 : 1; attempt < 2; ++attempt) {
         const bool pow2 = attempt == 0;
         const int mr = long_line ? kMaxRegsLong : (pow2 ? kMaxRegsPow2 : kMaxRegs);
@@ -219,7 +223,7 @@ bool choose_launch(const AxisFFT& ax, int nin, bool contiguous, size_t csize, in
                     const int need = (int)((elems + nthr - 1) / nthr);
                     L.ept = long_line ? 16 : (need <= 2 ? 2 : (need <= 4 ? 4 : 8));
                     if (L.ept > ept_max) continue;
-                    L.pow2 = pow2 ? 1 : 0;
+                    L.rset = pow2 ? kRadixPow2 : (small_axis ? kRadixSmall : kRadixAll);
                     L.lds = lds;
                     return true;
                 }
@@ -621,8 +625,8 @@ void set_fast(Launch& L, const FastEntry* fe, size_t lds) {
 // Kernel instance for a pass: contiguous passes compute the source (3 outputs) or
 // transform fields (1 output); strided passes have 2 outputs, or 1 when they
 // write the intensity or run a plain transform.
-// The generic kernels of one pass; POW2 instantiations only for FP64 (their FP32 forms
-// do not spill).  Elements a thread: the smallest instance holding L.ept; 16 only for lines
+// The generic kernels of one pass; the kRadixPow2 and kRadixSmall instantiations only for
+// FP64 (the FP32 forms of every radix spill little).  Elements a thread: the smallest instance holding L.ept; 16 only for lines
 // of more than 4096 points (up to 8192: 512 threads of 16), which spill but run.
 template <typename T>
 using GenericKernel = void (*)(PassArgs<T>);
@@ -635,61 +639,61 @@ hipError_t launch_ept(const Launch& L, const PassArgs<T>& a, hipStream_t st, Gen
     return launch_one(k, L, st, a);
 }
 
-template <typename T, bool POW2, bool IO32, int NOUT, bool SOURCE>
+template <typename T, int RS, bool IO32, int NOUT, bool SOURCE>
 hipError_t launch_contig(const Launch& L, const PassArgs<T>& a, hipStream_t st) {
     GenericKernel<T> k16 = nullptr;
-    if constexpr (!POW2) k16 = pass_contig<T, 16, NOUT, SOURCE, POW2, IO32>;
+    if constexpr (RS == kRadixAll) k16 = pass_contig<T, 16, NOUT, SOURCE, RS, IO32>;
     // two elements a thread where they suffice: the FP64 source kernel holds each element's
     // cell inputs and accumulators across the groups; the plain transform starts at four
     return launch_ept<T>(L, a, st,
-                         SOURCE ? pass_contig<T, 2, NOUT, SOURCE, POW2, IO32> : pass_contig<T, 4, NOUT, SOURCE, POW2, IO32>,
-                         pass_contig<T, 4, NOUT, SOURCE, POW2, IO32>, pass_contig<T, 8, NOUT, SOURCE, POW2, IO32>, k16);
+                         SOURCE ? pass_contig<T, 2, NOUT, SOURCE, RS, IO32> : pass_contig<T, 4, NOUT, SOURCE, RS, IO32>,
+                         pass_contig<T, 4, NOUT, SOURCE, RS, IO32>, pass_contig<T, 8, NOUT, SOURCE, RS, IO32>, k16);
 }
 
-template <typename T, bool POW2, bool IO32, int NOUT, bool INTENSITY>
+template <typename T, int RS, bool IO32, int NOUT, bool INTENSITY>
 hipError_t launch_strided(const Launch& L, const PassArgs<T>& a, hipStream_t st) {
     GenericKernel<T> k16 = nullptr;
-    if constexpr (!POW2) k16 = pass_strided<T, 16, NOUT, INTENSITY, POW2, IO32>;
+    if constexpr (RS == kRadixAll) k16 = pass_strided<T, 16, NOUT, INTENSITY, RS, IO32>;
     // four elements a thread where they suffice: fewer registers (FP64 kernels spill)
-    return launch_ept<T>(L, a, st, pass_strided<T, 4, NOUT, INTENSITY, POW2, IO32>,
-                         pass_strided<T, 4, NOUT, INTENSITY, POW2, IO32>, pass_strided<T, 8, NOUT, INTENSITY, POW2, IO32>,
+    return launch_ept<T>(L, a, st, pass_strided<T, 4, NOUT, INTENSITY, RS, IO32>,
+                         pass_strided<T, 4, NOUT, INTENSITY, RS, IO32>, pass_strided<T, 8, NOUT, INTENSITY, RS, IO32>,
                          k16);
 }
 
-template <typename T, bool POW2, bool IO32, int NZ>
+template <typename T, int RS, bool IO32, int NZ>
 hipError_t launch_mix(const Launch& L, const PassArgs<T>& a, hipStream_t st) {
     GenericKernel<T> k16 = nullptr;
-    if constexpr (!POW2) k16 = pass_strided_mix<T, 16, NZ, POW2, IO32>;
-    return launch_ept<T>(L, a, st, pass_strided_mix<T, 4, NZ, POW2, IO32>, pass_strided_mix<T, 4, NZ, POW2, IO32>,
-                         pass_strided_mix<T, 8, NZ, POW2, IO32>, k16);
+    if constexpr (RS == kRadixAll) k16 = pass_strided_mix<T, 16, NZ, RS, IO32>;
+    return launch_ept<T>(L, a, st, pass_strided_mix<T, 4, NZ, RS, IO32>, pass_strided_mix<T, 4, NZ, RS, IO32>,
+                         pass_strided_mix<T, 8, NZ, RS, IO32>, k16);
 }
 
-template <typename T, bool POW2, bool IO32>
+template <typename T, int RS, bool IO32>
 hipError_t launch_generic_io(const Launch& L, const PassArgs<T>& a, bool source, bool intensity, hipStream_t st) {
     if (L.contiguous) {
         // FP64 on FP32 arrays: only the source pass (plain transforms never read FP32 spectra)
-        if (source) return a.nout != 3 ? hipErrorInvalidValue : launch_contig<T, POW2, IO32, 3, true>(L, a, st);
+        if (source) return a.nout != 3 ? hipErrorInvalidValue : launch_contig<T, RS, IO32, 3, true>(L, a, st);
         if (IO32 || a.nout != 1) return hipErrorInvalidValue;
-        return launch_contig<T, POW2, IO32, 1, false>(L, a, st);
+        return launch_contig<T, RS, IO32, 1, false>(L, a, st);
     }
     if (a.pair_first) {  // pass 1 over source groups: one or two z-classes
-        if (a.ngroups == 1) return launch_mix<T, POW2, IO32, 1>(L, a, st);
-        if (a.ngroups == 2) return launch_mix<T, POW2, IO32, 2>(L, a, st);
+        if (a.ngroups == 1) return launch_mix<T, RS, IO32, 1>(L, a, st);
+        if (a.ngroups == 2) return launch_mix<T, RS, IO32, 2>(L, a, st);
         return hipErrorInvalidValue;
     }
-    if (intensity) return a.nout != 1 ? hipErrorInvalidValue : launch_strided<T, POW2, IO32, 1, true>(L, a, st);
-    if (a.nout == 2) return launch_strided<T, POW2, IO32, 2, false>(L, a, st);
+    if (intensity) return a.nout != 1 ? hipErrorInvalidValue : launch_strided<T, RS, IO32, 1, true>(L, a, st);
+    if (a.nout == 2) return launch_strided<T, RS, IO32, 2, false>(L, a, st);
     if (IO32 || a.nout != 1) return hipErrorInvalidValue;
-    return launch_strided<T, POW2, IO32, 1, false>(L, a, st);
+    return launch_strided<T, RS, IO32, 1, false>(L, a, st);
 }
 
-template <typename T, bool POW2>
+template <typename T, int RS>
 hipError_t launch_generic(const Launch& L, const PassArgs<T>& a, bool source, bool intensity, hipStream_t st) {
     if constexpr (std::is_same<T, double>::value) {
         // an FP32 plan computing in FP64: FP32 P, weights and spectra
-        if (L.io32) return launch_generic_io<T, POW2, true>(L, a, source, intensity, st);
+        if (L.io32) return launch_generic_io<T, RS, true>(L, a, source, intensity, st);
     }
-    return launch_generic_io<T, POW2, false>(L, a, source, intensity, st);
+    return launch_generic_io<T, RS, false>(L, a, source, intensity, st);
 }
 
 template <typename T>
@@ -700,9 +704,11 @@ hipError_t launch_pass(const Launch& L, const PassArgs<T>& a, bool source, bool 
         else
             return L.fe->launch64 ? L.fe->launch64(L, a, st) : hipErrorInvalidValue;
     }
-    if constexpr (std::is_same<T, double>::value)
-        if (L.pow2) return launch_generic<T, true>(L, a, source, intensity, st);
-    return launch_generic<T, false>(L, a, source, intensity, st);
+    if constexpr (std::is_same<T, double>::value) {
+        if (L.rset == kRadixPow2) return launch_generic<T, kRadixPow2>(L, a, source, intensity, st);
+        if (L.rset == kRadixSmall) return launch_generic<T, kRadixSmall>(L, a, source, intensity, st);
+    }
+    return launch_generic<T, kRadixAll>(L, a, source, intensity, st);
 }
 
 // Schedules of pass d with nin inputs per chunk, the default first: the registered
@@ -1118,7 +1124,8 @@ std::string describe_launch(const Launch& L) {
                       L.fe->io32 ? " io32" : "", L.fe->one_buffer ? " 1buf" : "");
     else
         std::snprintf(buf, sizeof(buf), "generic %s%s nl%d nf%d nthr%d ept%d%s%s", L.contiguous ? "contiguous" : "strided",
-                      L.mix ? " groups" : "", L.nl, L.nf, L.nthr, L.ept, L.pow2 ? " pow2" : "", L.io32 ? " io32" : "");
+                      L.mix ? " groups" : "", L.nl, L.nf, L.nthr, L.ept,
+                      L.rset == kRadixPow2 ? " pow2" : (L.rset == kRadixSmall ? " small" : ""), L.io32 ? " io32" : "");
     return buf;
 }
 

@@ -118,7 +118,7 @@ __device__ __forceinline__ void phase_sincos(double t, double& s, double& c) { s
 // table W[m] = exp(-2 pi i m / N).  The stage list lives in device memory and
 // is read with uniform (scalar) loads.
 constexpr int kMaxStages = 16;
-// Complex values a thread holds during one stage: 16, or 8 in the POW2 instantiations of
+// Complex values a thread holds during one stage: 16, or 8 in the kRadixPow2 instantiations of
 // fft_tile (FP64 only: the same 64 VGPRs as 16 FP32 values; they spilled to scratch with 16).
 constexpr int kMaxRegs = 16;
 constexpr int kMaxRegsPow2 = 8;
@@ -407,11 +407,15 @@ __device__ __forceinline__ void fft_stage_direct(cx<T>* __restrict__ tile, const
     __syncthreads();
 }
 
-// Forward transform of lines [0, Lc) of an LDS tile (see fft_stage), natural order in and out.
-// POW2: every stage has radix 2, 4 or 8, and the odd-radix stages are not compiled in (a
-// kernel's registers are those of its largest stage: the FP64 radix-17 stage alone holds
-// about 200 VGPRs).
-template <typename T, bool POW2 = false, int MR = kMaxRegs>
+// Radix sets of the generic kernels: every radix (with the direct stages of primes above 17),
+// radices 2, 4 and 8 only, or the radices up to 8 (2, 3, 4, 5, 7, 8).  A kernel's registers
+// are those of its largest stage (the FP64 radix-17 stage alone holds about 200 VGPRs), so
+// FP64 axes of small radices run instantiations without the large ones.
+constexpr int kRadixAll = 0, kRadixPow2 = 1, kRadixSmall = 2;
+
+// Forward transform of lines [0, Lc) of an LDS tile (see fft_stage), natural order in and out,
+// for an axis whose radices are all in the set RS (the host picks RS per axis).
+template <typename T, int RS = kRadixAll, int MR = kMaxRegs>
 __device__ __forceinline__ void fft_tile(cx<T>* tile, int nst, const Stage* __restrict__ stages,
                                          const cx<T>* __restrict__ W, int stride, int Lc, FastDiv divLc, int tid,
                                          int nthr) {
@@ -419,12 +423,12 @@ __device__ __forceinline__ void fft_tile(cx<T>* tile, int nst, const Stage* __re
         const Stage sd = stages[s];
         const FastDiv dn = {(uint32_t)sd.Ns, sd.ns_m};
         const int Ns = sd.Ns, tws = sd.tws, NR = sd.NR;
-        if constexpr (POW2) {
+        if constexpr (RS == kRadixPow2) {
             switch (sd.R) {
                 case 2: fft_stage<2, T, kMaxRegsPow2>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
                 case 4: fft_stage<4, T, kMaxRegsPow2>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
                 case 8: fft_stage<8, T, kMaxRegsPow2>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
-                default: break;  // the host sets POW2 only for plans of these radices
+                default: break;  // the host picks this set only for axes of these radices
             }
         } else {
             switch (sd.R) {
@@ -434,12 +438,18 @@ __device__ __forceinline__ void fft_tile(cx<T>* tile, int nst, const Stage* __re
                 case 5: fft_stage<5, T, MR>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
                 case 7: fft_stage<7, T, MR>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
                 case 8: fft_stage<8, T, MR>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
-                case 11: fft_stage<11, T, MR>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
-                case 13: fft_stage<13, T, MR>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
-                case 17: fft_stage<17, T, MR>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
-                default:  // primes above 17
-                    fft_stage_direct<T>(tile, W, sd.R, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr);
-                    break;
+                default:
+                    if constexpr (RS == kRadixAll) {
+                        switch (sd.R) {
+                            case 11: fft_stage<11, T, MR>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
+                            case 13: fft_stage<13, T, MR>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
+                            case 17: fft_stage<17, T, MR>(tile, W, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr); break;
+                            default:  // primes above 17
+                                fft_stage_direct<T>(tile, W, sd.R, NR, stride, Lc, divLc, Ns, dn, tws, tid, nthr);
+                                break;
+                        }
+                    }
+                    break;  // kRadixSmall: the host picks this set only for axes of radices up to 8
             }
         }
     }

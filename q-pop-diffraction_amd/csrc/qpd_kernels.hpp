@@ -218,12 +218,12 @@ __device__ __forceinline__ cx<T> chunk_coef(const PassArgs<T>& a, int ct, int id
 }
 
 // ---------------------------------------------------------------------------
-// Pass along the contiguous dim: block = nl consecutive rows of n0 points.  POW2: the
-// axis has only radix-2/4/8 stages (fft_tile).
+// Pass along the contiguous dim: block = nl consecutive rows of n0 points.  RS: the set of
+// radices of the axis (fft_tile).
 // LDS tile: point idx of line (f, row) at tile[idx * (nf*nl) + f*nl + row].
 // Element e = tid + m*nthr of a field's tile is (row, idx) = divmod(e, N).
 // IO32: FP64 arithmetic on FP32 arrays (inputs, P and weights, outputs).
-template <typename T, int EPT, int NOUT, bool SOURCE, bool POW2 = false, bool IO32 = false>
+template <typename T, int EPT, int NOUT, bool SOURCE, int RS = kRadixAll, bool IO32 = false>
 __global__ __launch_bounds__(512) void pass_contig(PassArgs<T> a) {
     using TS = io_t<T, IO32>;
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -282,7 +282,7 @@ __global__ __launch_bounds__(512) void pass_contig(PassArgs<T> a) {
                     }
                 }
                 __syncthreads();
-                fft_tile<T, POW2, EPT >= 16 ? kMaxRegsLong : kMaxRegs>(tile, a.nst, a.stages, a.W, stride, nfc * nl, lines_div(a, nfc), tid, nthr);
+                fft_tile<T, RS, EPT >= 16 ? kMaxRegsLong : kMaxRegs>(tile, a.nst, a.stages, a.W, stride, nfc * nl, lines_div(a, nfc), tid, nthr);
 #pragma unroll
                 for (int m = 0; m < EPT; ++m) {
                     const int e = tid + m * nthr;
@@ -316,7 +316,7 @@ __global__ __launch_bounds__(512) void pass_contig(PassArgs<T> a) {
 // keeps one column tx = tid % TX and the points idx = tid / TX + m * (nthr / TX).
 // INTENSITY: the single output is the amplitude; write fftshifted max(|A|^2, 1e-50).
 // IO32: FP64 arithmetic on complex FP32 inputs and outputs.
-template <typename T, int EPT, int NOUT, bool INTENSITY, bool POW2 = false, bool IO32 = false>
+template <typename T, int EPT, int NOUT, bool INTENSITY, int RS = kRadixAll, bool IO32 = false>
 __global__ __launch_bounds__(512) void pass_strided(PassArgs<T> a) {
     using TI = io_t<T, IO32>;
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -359,7 +359,7 @@ __global__ __launch_bounds__(512) void pass_strided(PassArgs<T> a) {
                         dst[f * TX] = col_ok ? cx_as<T>(inbase[(size_t)f * a.ncell + off]) : cx<T>{(T)0, (T)0};
                 }
                 __syncthreads();
-                fft_tile<T, POW2, EPT >= 16 ? kMaxRegsLong : kMaxRegs>(tile, a.nst, a.stages, a.W, stride, nfc * TX, lines_div(a, nfc), tid, nthr);
+                fft_tile<T, RS, EPT >= 16 ? kMaxRegsLong : kMaxRegs>(tile, a.nst, a.stages, a.W, stride, nfc * TX, lines_div(a, nfc), tid, nthr);
 #pragma unroll
                 for (int m = 0; m < EPT; ++m) {
                     const int idx = r0 + m * rstep;
@@ -411,7 +411,7 @@ __global__ __launch_bounds__(512) void pass_strided(PassArgs<T> a) {
 //   out_{z,1} = sum_s sum_{Y in z} c_Y(k1) X_{Y,s}(k0) F1 C_s,
 // the outputs of the class form's pass 1.
 // IO32: FP64 arithmetic on complex FP32 inputs and outputs.
-template <typename T, int EPT, int NZ, bool POW2 = false, bool IO32 = false>
+template <typename T, int EPT, int NZ, int RS = kRadixAll, bool IO32 = false>
 __global__ __launch_bounds__(512) void pass_strided_mix(PassArgs<T> a) {
     using TI = io_t<T, IO32>;
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -437,9 +437,6 @@ __global__ __launch_bounds__(512) void pass_strided_mix(PassArgs<T> a) {
 #pragma unroll
         for (int z = 0; z < NZ; ++z) acc[m][z][0] = acc[m][z][1] = {(T)0, (T)0};
     for (int s = 0; s < a.nsrc; ++s) {
-        cx<T> tA[EPT], tC[EPT];
-#pragma unroll
-        for (int m = 0; m < EPT; ++m) tA[m] = tC[m] = {(T)0, (T)0};
         for (int j0 = 0; j0 < 3; j0 += a.nf) {
             const int nfc = min(a.nf, 3 - j0);
             const cx<TI>* inbase = reinterpret_cast<const cx<TI>*>(a.in) + ((size_t)s * 3 + j0) * a.ncell;
@@ -453,41 +450,35 @@ __global__ __launch_bounds__(512) void pass_strided_mix(PassArgs<T> a) {
                     dst[f * TX] = col_ok ? cx_as<T>(inbase[(size_t)f * a.ncell + off]) : cx<T>{(T)0, (T)0};
             }
             __syncthreads();
-            fft_tile<T, POW2, EPT >= 16 ? kMaxRegsLong : kMaxRegs>(tile, a.nst, a.stages, a.W, stride, nfc * TX, lines_div(a, nfc), tid, nthr);
-#pragma unroll
-            for (int m = 0; m < EPT; ++m) {
-                const int idx = r0 + m * rstep;
-                if (idx >= N) continue;
-                const T kv = a.k[idx];
-                const cx<T>* srcp = tile + idx * stride + tx;
+            fft_tile<T, RS, EPT >= 16 ? kMaxRegsLong : kMaxRegs>(tile, a.nst, a.stages, a.W, stride, nfc * TX,
+                                                                  lines_div(a, nfc), tid, nthr);
+            // each transformed field goes into the sums as it is read from the tile (no
+            // per-group values are held across fields): F A and -i k F B into out_{z,0},
+            // F C into out_{z,1}
+            if (col_ok) {
                 for (int f = 0; f < nfc; ++f) {
                     const int jj = j0 + f;
-                    const cx<T> v = srcp[f * TX];
-                    if (jj == 0)
-                        tA[m] = tA[m] + v;
-                    else if (jj == 1)
-                        tA[m] = tA[m] + cscale(negi(v), kv);
-                    else
-                        tC[m] = v;
+#pragma unroll
+                    for (int z = 0; z < NZ; ++z) {
+                        for (int p = a.pair_first[s * NZ + z]; p < a.pair_first[s * NZ + z + 1]; ++p) {
+                            const int y = a.pair_y[p];
+                            const cx<T> xc = a.mixc[(size_t)(y * a.nsrc + s) * a.n0 + i0];
+#pragma unroll
+                            for (int m = 0; m < EPT; ++m) {
+                                const int idx = r0 + m * rstep;
+                                if (idx >= N) continue;
+                                cx<T> t = cmul(cmul(a.coef[(size_t)y * a.Nc + idx], xc), tile[idx * stride + f * TX + tx]);
+                                if (jj == 1) t = cscale(negi(t), a.k[idx]);
+                                if (jj <= 1)
+                                    acc[m][z][0] = acc[m][z][0] + t;
+                                else
+                                    acc[m][z][1] = acc[m][z][1] + t;
+                            }
+                        }
+                    }
                 }
             }
             __syncthreads();
-        }
-        if (!col_ok) continue;  // no barrier follows within this group
-#pragma unroll
-        for (int z = 0; z < NZ; ++z) {
-            for (int p = a.pair_first[s * NZ + z]; p < a.pair_first[s * NZ + z + 1]; ++p) {
-                const int y = a.pair_y[p];
-                const cx<T> xc = a.mixc[(size_t)(y * a.nsrc + s) * a.n0 + i0];
-#pragma unroll
-                for (int m = 0; m < EPT; ++m) {
-                    const int idx = r0 + m * rstep;
-                    if (idx >= N) continue;
-                    const cx<T> cf = cmul(a.coef[(size_t)y * a.Nc + idx], xc);
-                    acc[m][z][0] = acc[m][z][0] + cmul(cf, tA[m]);
-                    acc[m][z][1] = acc[m][z][1] + cmul(cf, tC[m]);
-                }
-            }
         }
     }
     if (!col_ok) return;

@@ -44,7 +44,7 @@ struct Launch {
     int nf = 0;                     // inputs per LDS fill (generic)
     int nthr = 0;
     int ept = 0;                    // elements per thread, 4 / 8 (generic)
-    int pow2 = 0;                   // generic FP64: every stage radix 2, 4 or 8 (fft_tile POW2)
+    int rset = 0;                   // generic FP64: the radix set of the axis (kRadixAll, kRadixPow2, kRadixSmall)
     int io32 = 0;                   // generic FP64 kernel on FP32 arrays (FP32 plan computing in FP64)
     int mix = 0;                    // generic strided pass 1 over source groups (pass_strided_mix)
     size_t lds = 0;
