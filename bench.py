@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--no-large", action="store_true",
                     help="skip the 4000 x 4000 x 68 polar-vortex volume (about 140 GB of HBM) and its FP64 crop")
     ap.add_argument("--large-grid", type=int, nargs=3, default=(4000, 4000, 68), metavar=("NX", "NY", "NZ"))
+    ap.add_argument("--large-only", action="store_true",
+                    help="run only the large-volume leg (for profiling it); prints its JSON object")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first copy BYTES device to device three times (a kernel of known traffic for counters)")
     return ap.parse_args()
@@ -310,26 +312,51 @@ def serve_leg(args):
     return res
 
 
-TRAFFIC_FILE = ROOT / "profiles" / "volume_512x512x104_fp32_traffic.json"
-
-
 def measured_traffic(grid, pass_index, schedule):
-    """HBM bytes per launch of a kernel, from the committed rocprofv3 counter summary, or None.
+    """HBM bytes per launch of a kernel, from the committed rocprofv3 counter summaries
+    (profiles/*_traffic.json), or None.
 
-    The summary holds FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by what the same
+    A summary holds FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by what the same
     counter reads for a device copy of known size; it is used only when it was taken for the
     same grid, pass and schedule.
     """
-    try:
-        t = json.loads(TRAFFIC_FILE.read_text())
-    except (OSError, ValueError):
-        return None
-    if list(t.get("grid", [])) != list(grid):
-        return None
-    for k in t.get("kernels", []):
-        if k.get("pass") == pass_index and k.get("schedule") == schedule:
-            return k.get("bytes")
+    for f in sorted((ROOT / "profiles").glob("*_traffic.json")):
+        try:
+            t = json.loads(f.read_text())
+        except (OSError, ValueError):
+            continue
+        if list(t.get("grid", [])) != list(grid):
+            continue
+        for k in t.get("kernels", []):
+            if k.get("pass") == pass_index and k.get("schedule") == schedule:
+                return k.get("bytes")
     return None
+
+
+def copy_ceiling(nbytes=1 << 30, reps=5):
+    """Device-to-device copy of nbytes (torch copy_, one read and one write of each byte),
+    timed with HIP events: the bandwidth a plain streaming kernel reaches on this GPU."""
+    import torch
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record(st)
+    e1.synchronize()
+    sec = e0.elapsed_time(e1) / reps * 1e-3
+    del src, dst
+    return 2 * nbytes / sec / 1e9
+
+
+def roofline_of(name, gb_s, bytes_per_launch, traffic, ceiling):
+    return {"bound": "hbm", "kernel": name, "achieved": gb_s, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": gb_s / HBM_PEAK_GBS, "traffic": traffic, "bytes_per_launch": bytes_per_launch,
+            "copy_ceiling": {"GB_s": ceiling, "frac": gb_s / ceiling,
+                             "kernel": "device-to-device copy of 1 GiB (torch copy_), HIP events"}}
 
 
 def ingest_leg(nx, ny, nz, reps=10):
@@ -432,7 +459,9 @@ def rsm_leg(I, dims, spacing):
 
 
 def large_leg(args):
-    """One large volume on one GPU: FP32 time, and its error against FP64 on a 512 x 512 x nz crop around G."""
+    """One large volume on one GPU in the default arithmetic and with FP32 transforms: time per
+    volume, per-pass rates, and each one's error against FP64 on a 512 x 512 x nz crop around G."""
+    import numpy as np
     import torch
     from qpop_diffraction import Engine, Volume, workloads
     from qpop_diffraction.autotune import ScheduleCache
@@ -447,34 +476,52 @@ def large_leg(args):
     torch.cuda.synchronize()
     t_gen = time.perf_counter() - t0
     log(f"large: {v.name} generated in {t_gen:.1f} s")
-    eng = Engine(v.params, v.atoms)
     vol = Volume(v.dims, v.axis, v.P, v.U, v.phase, precision=32)
-    out = torch.empty(vol.n, dtype=torch.float64, device="cuda")
-    tuned = eng.tune(vol, cache=ScheduleCache(None), out=out)
-    plan = eng.plan(vol.dims, vol.axis, 32)
-    sec = time_steps(lambda: eng.intensity(vol, out), 3, 1)
-    passes = pass_times(plan, vol, out, eng.workspace(plan), 2)
-    eng.intensity(vol, out)
-    I32 = crop_of(out, v.dims)
-    hbm = torch.cuda.max_memory_allocated() - base
-    log(f"large: {sec * 1e3:.1f} ms/volume, passes {passes}, peak HBM {hbm / 1e9:.1f} GB")
-    del out
-    eng._work.clear()
-    torch.cuda.empty_cache()
+    ceiling = copy_ceiling()
+
+    def one(arith):
+        eng = Engine(v.params, v.atoms, arith=arith)
+        out = torch.empty(vol.n, dtype=torch.float64, device="cuda")
+        tuned = eng.tune(vol, cache=ScheduleCache(None), out=out)
+        plan = eng.plan(vol.dims, vol.axis, 32)
+        sec = time_steps(lambda: eng.intensity(vol, out), 3, 1)
+        passes = pass_times(plan, vol, out, eng.workspace(plan), 2)
+        eng.intensity(vol, out)
+        crop = crop_of(out, v.dims)
+        hbm = torch.cuda.max_memory_allocated() - base
+        names = [p["chosen"] for p in tuned["passes"]]
+        bpc = [plan.pass_bytes_per_cell(p, vol.P, vol.U, vol.phase) for p in range(3)]
+        gbs = [bpc[p] * vol.n / (passes[p] * 1e-3) / 1e9 for p in range(3)]
+        dom = int(np.argmax(passes))
+        log(f"large {arith}: {sec * 1e3:.1f} ms/volume, passes {passes}, peak HBM {hbm / 1e9:.1f} GB")
+        res = {"arithmetic": f"{plan.compute_bits}-bit transforms, {plan.storage_bits}-bit spectra",
+               "ms_per_volume": sec * 1e3, "voxels_per_s": vol.n / sec, "passes_ms": passes, "schedules": names,
+               "GB_s": gbs, "bytes_per_cell": bpc,
+               "traffic": [measured_traffic((nx, ny, nz), p, names[p]) for p in range(3)],
+               "roofline": roofline_of(f"pass {dom}: {names[dom]}", gbs[dom], bpc[dom] * vol.n,
+                                       measured_traffic((nx, ny, nz), dom, names[dom]), ceiling),
+               "peak_hbm_bytes": hbm}
+        del out
+        eng._work.clear()
+        torch.cuda.empty_cache()
+        return res, crop, eng.tables
+
+    res, I32, tables = one("auto")
+    res_f, I32f, _ = one("fp32")
     t0 = time.perf_counter()
-    ref = fp64_crop(v, v.params, eng.tables, log=log)
+    ref = fp64_crop(v, v.params, tables, log=log)
     t_ref = time.perf_counter() - t0
     dl, rel = max_dlog10(I32, ref)
-    bpc = [plan.pass_bytes_per_cell(p, vol.P, vol.U, vol.phase) for p in range(3)]
-    n = vol.n
-    return {"workload": f"{v.name}, layout z fastest, FP32, reflection 002",
-            "grid": [nx, ny, nz], "ms_per_volume": sec * 1e3, "voxels_per_s": n / sec,
-            "passes_ms": passes, "schedules": [p["chosen"] for p in tuned["passes"]],
-            "GB_s": [bpc[p] * n / (passes[p] * 1e-3) / 1e9 for p in range(3)],
-            "peak_hbm_bytes": hbm, "generate_s": t_gen,
-            "accuracy_vs_fp64_crop": {"crop": [512, 512, nz], "max_abs_dlog10_I": dl, "max_abs_dI_over_peak": rel,
-                                      "floor": "voxels >= 1e-10 of the crop's peak",
-                                      "fp64_crop_s": t_ref}}
+    dlf, relf = max_dlog10(I32f, ref)
+    floor = "voxels >= 1e-10 of the crop's peak"
+    res_f["accuracy_vs_fp64_crop"] = {"max_abs_dlog10_I": dlf, "max_abs_dI_over_peak": relf, "floor": floor}
+    res_f["note"] = "explicit FP32 arithmetic (arith='fp32'): no accuracy bound"
+    res.update({"workload": f"{v.name}, layout z fastest, FP32, reflection 002", "grid": [nx, ny, nz],
+                "generate_s": t_gen,
+                "accuracy_vs_fp64_crop": {"crop": [512, 512, nz], "max_abs_dlog10_I": dl, "max_abs_dI_over_peak": rel,
+                                          "floor": floor, "fp64_crop_s": t_ref},
+                "fp32_transforms": res_f})
+    return res
 
 
 def main():
@@ -502,6 +549,16 @@ def run():
     if not torch.cuda.is_available():
         print(json.dumps({"error": "no GPU"}), file=sys.__stdout__)
         sys.exit(1)
+    if args.copy_calibration > 0:
+        src = torch.ones(args.copy_calibration // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy calibration: 3 x {args.copy_calibration} bytes")
+        del src, dst
+    if args.large_only:
+        return {"large": large_leg(args)}
     from qpop_diffraction import Engine, Volume
     from qpop_diffraction import workloads
 
@@ -514,14 +571,6 @@ def run():
     plan = engine.plan(vol.dims, vol.axis, 32)
     work = engine.workspace(plan)
     out = torch.empty(n, dtype=torch.float64, device="cuda")
-    if args.copy_calibration > 0:
-        src = torch.ones(args.copy_calibration // 4, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        log(f"copy calibration: 3 x {args.copy_calibration} bytes")
-        del src, dst
     from qpop_diffraction.autotune import ScheduleCache
     if args.no_tune:
         tuned = {"passes": [{"pass": d, "chosen": plan.schedules(d)[plan.schedule(d)]} for d in range(3)]}
@@ -542,6 +591,10 @@ def run():
     sec_f = time_steps(lambda: eng_f.intensity(vol, out_f), args.steps, args.warmup)
     plan_f = eng_f.plan(vol.dims, vol.axis, 32)
     passes_f = pass_times(plan_f, vol, out_f, eng_f.workspace(plan_f), max(5, args.steps))
+    bpc_f = [plan_f.pass_bytes_per_cell(p, vol.P, vol.U, vol.phase) for p in range(3)]
+    gbs_f = [bpc_f[p] * n / (passes_f[p] * 1e-3) / 1e9 for p in range(3)]
+    dom_f = int(np.argmax(passes_f))
+    names_f = [plan_f.schedules(p)[plan_f.schedule(p)] for p in range(3)]
     eng_f.intensity(vol, out_f)
     I32f = out_f.clone()
     del out_f, eng_f
@@ -551,6 +604,8 @@ def run():
     gbs = [bpc[p] * n / (passes[p] * 1e-3) / 1e9 for p in range(3)]
     dom = int(np.argmax(passes))
     log(f"passes ms {passes}, GB/s {gbs}")
+    ceiling = copy_ceiling()
+    log(f"copy ceiling {ceiling:.0f} GB/s")
 
     result = {
         "metric": "voxels/s at 512x512x104, FP32 (ms/volume FP32 and FP64 beside it)",
@@ -572,14 +627,16 @@ def run():
                    "pass_dims": list(plan.pass_dims), "launches_per_volume": plan.launches},
         "ms_per_volume_fp32": sec * 1e3,
         "fp32_transforms": {"ms_per_volume": sec_f * 1e3, "voxels_per_s": n / sec_f, "passes_ms": passes_f,
-                            "pass_dims": list(plan_f.pass_dims),
+                            "pass_dims": list(plan_f.pass_dims), "schedules": names_f,
+                            "roofline": roofline_of(f"pass {dom_f}: {names_f[dom_f]}", gbs_f[dom_f],
+                                                    bpc_f[dom_f] * n,
+                                                    measured_traffic((nx, ny, nz), dom_f, names_f[dom_f]), ceiling),
                             "note": "explicit FP32 arithmetic (arith='fp32'): no accuracy bound"},
         "kernels": [{"pass": p, "ms": passes[p], "bytes_per_cell": bpc[p], "GB_s": gbs[p],
                      "schedule": tuned["passes"][p]["chosen"]} for p in range(3)],
-        "roofline": {"bound": "hbm", "kernel": f"pass {dom}: {tuned['passes'][dom]['chosen']}",
-                     "achieved": gbs[dom], "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs[dom] / HBM_PEAK_GBS,
-                     "traffic": measured_traffic((nx, ny, nz), dom, tuned["passes"][dom]["chosen"]),
-                     "bytes_per_launch": bpc[dom] * n},
+        "roofline": roofline_of(f"pass {dom}: {tuned['passes'][dom]['chosen']}", gbs[dom], bpc[dom] * n,
+                                measured_traffic((nx, ny, nz), dom, tuned["passes"][dom]["chosen"]), ceiling),
+        "traffic": [measured_traffic((nx, ny, nz), p, tuned["passes"][p]["chosen"]) for p in range(3)],
     }
     log("output path")
     result["output"] = output_leg(out, (nx, ny, nz))
