@@ -121,10 +121,11 @@ def cpu_baseline(w, crop):
             t = timed(*g)
             grids.append({"grid": list(g), "s": t, "voxels_per_s": g[0] * g[1] * g[2] / t})
     n = cx * cy * nz
+    what = (f"the whole {nx}x{ny}x{nz} volume" if (cx, cy) == (nx, ny)
+            else f"the {cx}x{cy}x{nz} corner crop of the same volume")
     return {"value": n / dt, "unit": "voxels/s", "cores": 1, "kind": "port",
-            "sample": f"FP64 NumPy oracle (restatement of the reference diffraction()) on the "
-                      f"{cx}x{cy}x{nz} corner crop of the same volume, {dt:.1f} s; host has {os.cpu_count()} "
-                      f"cores, numpy.fft and the elementwise steps use 1",
+            "sample": f"FP64 NumPy oracle (restatement of the reference diffraction()) on {what}, {dt:.1f} s; "
+                      f"host has {os.cpu_count()} cores, numpy.fft and the elementwise steps use 1",
             "grids": grids}
 
 
