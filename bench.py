@@ -207,7 +207,7 @@ def batch_leg(args):
         del s64, w64, out64
         plans64 = PlanCache(64)
     n_jobs = len(b.jobs)
-    arith = sorted({f"{j.plan.compute_bits}/{j.plan.storage_bits}" for j in b.jobs})
+    arith = sorted({f"{'-'.join(map(str, j.plan.pass_bits))}/{j.plan.storage_bits}" for j in b.jobs})
     res = {
         "workload": "48 two-variant 90-degree twin-stack snapshots (lateral 128-512 step 64, depth 24-104 from "
                     f"{list(workloads.DEPTHS)}) x reflections 002, 103, 202, 113 = {n_jobs} volumes, FP32 inputs",
@@ -495,7 +495,8 @@ def large_leg(args):
         gbs = [bpc[p] * vol.n / (passes[p] * 1e-3) / 1e9 for p in range(3)]
         dom = int(np.argmax(passes))
         log(f"large {arith}: {sec * 1e3:.1f} ms/volume, passes {passes}, peak HBM {hbm / 1e9:.1f} GB")
-        res = {"arithmetic": f"{plan.compute_bits}-bit transforms, {plan.storage_bits}-bit spectra",
+        res = {"arithmetic": f"{'/'.join(map(str, plan.pass_bits))}-bit transforms (passes 0/1/2), "
+                             f"{plan.storage_bits}-bit spectra, estimate {plan.dlog10_estimate:.1e}",
                "ms_per_volume": sec * 1e3, "voxels_per_s": vol.n / sec, "passes_ms": passes, "schedules": names,
                "GB_s": gbs, "bytes_per_cell": bpc,
                "traffic": [measured_traffic((nx, ny, nz), p, names[p]) for p in range(3)],
@@ -623,7 +624,8 @@ def run():
         "data": "synthetic (seeded, generated on the GPU)",
         "config": {"workload": f"two-phase Voronoi volume {nx}x{ny}x{nz}, 40 grains, 5-atom PbTiO3-like cell, "
                                f"hkl 002, one volume per step", "grid": [nx, ny, nz], "precision": "fp32",
-                   "arithmetic": f"FP32 inputs and spectra between passes, {plan.compute_bits}-bit transforms "
+                   "arithmetic": f"FP32 inputs and spectra between passes, "
+                                 f"{'/'.join(map(str, plan.pass_bits))}-bit transforms (passes 0/1/2) "
                                  f"(the default of an FP32 plan)",
                    "pass_dims": list(plan.pass_dims), "launches_per_volume": plan.launches},
         "ms_per_volume_fp32": sec * 1e3,

@@ -181,6 +181,12 @@ int qpd_plan_create_opts(qpd_plan** plan, const qpd_grid* grid, const qpd_crysta
 int qpd_plan_info(const qpd_plan* plan, int pass_dims[3], int* compute_bits, int* storage_bits,
                   double* dlog10_estimate);
 
+/* The bits of each pass's transforms (32 or 64), in pass order (qpd_plan_info's pass_dims).
+ * The automatic arithmetic of an FP32 plan may give FP32 transforms to long strided passes
+ * (about 3200 points and more) while its error estimate stays within its bound; its
+ * compute_bits above is pass 0's.  No counterpart in the reference. */
+int qpd_plan_pass_bits(const qpd_plan* plan, int bits[3]);
+
 /* Release a plan (synchronises the device). */
 int qpd_plan_destroy(qpd_plan* plan);
 

@@ -925,7 +925,8 @@ int run_pass(qpd_plan* p, const qpd_fields* f, double* I, void* work, int d, hip
     if (pp.nlaunch == 2) {
         // step 1: N2-point sub-transforms of every input field, twiddled, in place (FP32 plans
         // without a tail only: the first step rewrites its input fields in their own precision)
-        if (p->mixed) return fail(QPD_ERR_UNSUPPORTED, "split pass in a plan computing in FP64 on FP32 spectra");
+        if (p->mixed && p->csz[d] == 16)
+            return fail(QPD_ERR_UNSUPPORTED, "split pass in a plan computing in FP64 on FP32 spectra");
         const uint32_t N1 = (uint32_t)pp.N1, N2 = (uint32_t)pp.N2;
         PassArgs<T> s1 = a;
         s1.N = (int)N2;
@@ -1270,7 +1271,17 @@ int qpd_plan_create_opts(qpd_plan** plan, const qpd_grid* grid, const qpd_crysta
     // plane is one cell).  The constants are the worst cases measured per term (NumPy
     // emulation of FP32 storage with FP64 transforms).  A plan whose estimate exceeds kBound,
     // 0.6 of the 1e-3 target, stores its spectra in FP64 too.
-    constexpr double kLineErr = 5.8e-3, kPlaneErr = 3.5e-3, kBound = 6e-4;
+    // A long strided pass may still run FP32 transforms: their errors on a line are about
+    // eps sqrt(log2 N) times the rms of the line's outputs, so the worst case (a voxel at the
+    // floor on the line of a single bright peak) falls as sqrt(log2 N / N); kTxErr is that
+    // worst case's constant, measured on the 1024 x 1024 x 1 hexagonal-twin example (1.39e-3
+    // over two 1024-point FP32 passes, the worst result of FP32 transforms in the tests).  The
+    // automatic arithmetic gives FP32 transforms to pass 1, then to pass 2, while the estimate
+    // with their terms stays within kBound (passes of about 3200 points and more; one of the two
+    // 4000-point passes of a 4000 x 4000 x 68 grid).  Axes with a direct stage (a prime factor
+    // above 17, whose outputs are long FP32 sums) keep FP64 transforms: the model is the
+    // butterflies'.
+    constexpr double kLineErr = 5.8e-3, kPlaneErr = 3.5e-3, kBound = 6e-4, kTxErr = 9.8e-3;
     auto spread = [](double k, double n) { return n > 1.0 ? k / std::sqrt(n) : 0.0; };
     auto estimate = [&](int pd1) {
         const double nl = (double)grid->dims[3 - pd1], n1 = (double)grid->dims[pd1];
@@ -1293,18 +1304,32 @@ int qpd_plan_create_opts(qpd_plan** plan, const qpd_grid* grid, const qpd_crysta
                                                  " has a prime factor above 17 or is too long");
         }
     }
+    bool fp32_pass[3] = {false, false, false};  // passes of a mixed plan that keep FP32 transforms
     if (precision == 32) {
         int a = arith;
-        const double est = estimate(p->pd[1]);
-        if (a == QPD_ARITH_AUTO) a = est <= kBound ? QPD_ARITH_MIXED : QPD_ARITH_FP64;
+        double est = estimate(p->pd[1]);
+        const bool automatic = a == QPD_ARITH_AUTO;
+        if (automatic) a = est <= kBound ? QPD_ARITH_MIXED : QPD_ARITH_FP64;
         p->mixed = a == QPD_ARITH_MIXED ? 1 : 0;
         p->wide = a == QPD_ARITH_FP64 ? 1 : 0;
+        if (automatic && p->mixed) {
+            for (int d = 1; d <= 2; ++d) {
+                const double n = (double)grid->dims[p->pd[d]];
+                const double tx = kTxErr * std::sqrt(std::log2(n) / n);
+                bool butterflies = true;
+                for (int s = 0; s < p->fft[d].nst; ++s) butterflies = butterflies && p->fft[d].st[s].R <= 17;
+                if (n > 1.0 && butterflies && std::hypot(est, tx) <= kBound) {
+                    fp32_pass[d] = true;
+                    est = std::hypot(est, tx);
+                }
+            }
+        }
         // an explicit FP32 plan has no bound: its FP32 transforms leave about 1e-3 at voxels
         // 1e-10 of the peak on lines that carry a strong one
         p->est_dlog10 = p->wide ? 0.0 : p->mixed ? est : -1.0;
     }
     const bool f64 = precision == 64 || p->wide || p->mixed;
-    for (int d = 0; d < 3; ++d) p->csz[d] = f64 ? sizeof(cx<double>) : sizeof(cx<float>);
+    for (int d = 0; d < 3; ++d) p->csz[d] = f64 && !fp32_pass[d] ? sizeof(cx<double>) : sizeof(cx<float>);
     p->ssz = (precision == 64 || p->wide) ? sizeof(cx<double>) : sizeof(cx<float>);
     p->n_phase = c->n_phase;
     p->n_atom = c->n_atom;
@@ -1318,7 +1343,7 @@ int qpd_plan_create_opts(qpd_plan** plan, const qpd_grid* grid, const qpd_crysta
     const int tab_entries = c->n_atom * c->n_phase * kAtomTab;
     int rc = QPD_OK;
     for (int d = 0; d < 3 && rc == QPD_OK; ++d) {
-        const bool io32 = p->mixed != 0;
+        const bool io32 = p->mixed != 0 && p->csz[d] == 16;
         rc = pass_candidates(p->fft[d], d, p->pd[d], p->dims, nin[d], p->csz[d], io32, true, tab_entries, true,
                              p->cand[d], p->grouped ? mix_role : -1);
         if (d == 1 && p->grouped && rc != QPD_OK) {
@@ -1341,8 +1366,11 @@ int qpd_plan_create_opts(qpd_plan** plan, const qpd_grid* grid, const qpd_crysta
             p->pass[d] = p->cand[d][0];
         }
     }
-    if (rc == QPD_OK && p->csz[0] == 8) rc = build_tables<float>(p, grid, c, p->tab[0]);
-    if (rc == QPD_OK && p->csz[0] == 16) rc = build_tables<double>(p, grid, c, p->tab[1]);
+    // the tables of each element type some pass computes in
+    bool need32 = false, need64 = false;
+    for (int d = 0; d < 3; ++d) (p->csz[d] == 16 ? need64 : need32) = true;
+    if (rc == QPD_OK && need32) rc = build_tables<float>(p, grid, c, p->tab[0]);
+    if (rc == QPD_OK && need64) rc = build_tables<double>(p, grid, c, p->tab[1]);
     if (rc != QPD_OK) {
         for (PlanTables& t : p->tab)
             if (t.dev) (void)hipFree(t.dev);
@@ -1360,6 +1388,12 @@ int qpd_plan_info(const qpd_plan* p, int pass_dims[3], int* compute_bits, int* s
     if (compute_bits) *compute_bits = p->csz[0] == 16 ? 64 : 32;
     if (storage_bits) *storage_bits = p->ssz == 16 ? 64 : 32;
     if (dlog10_estimate) *dlog10_estimate = p->est_dlog10;
+    return QPD_OK;
+}
+
+int qpd_plan_pass_bits(const qpd_plan* p, int bits[3]) {
+    if (!p || !bits) return fail(QPD_ERR_ARG, "null argument");
+    for (int d = 0; d < 3; ++d) bits[d] = p->csz[d] == 16 ? 64 : 32;
     return QPD_OK;
 }
 

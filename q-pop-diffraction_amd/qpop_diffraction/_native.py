@@ -18,7 +18,8 @@ LIB_PATH = Path(os.environ.get("QPD_LIB", Path(__file__).resolve().parent / "lib
 
 # Entry points declared in include/qpd.h
 EXPORTS = (
-    "qpd_plan_create", "qpd_plan_create_opts", "qpd_plan_info", "qpd_plan_destroy", "qpd_workspace_bytes",
+    "qpd_plan_create", "qpd_plan_create_opts", "qpd_plan_info", "qpd_plan_pass_bits", "qpd_plan_destroy",
+    "qpd_workspace_bytes",
     "qpd_launches_per_volume",
     "qpd_plan_schedules", "qpd_plan_schedule_name", "qpd_plan_set_schedule", "qpd_plan_get_schedule",
     "qpd_fast_lengths", "qpd_intensity", "qpd_intensity_pass", "qpd_intensity_noalloc", "qpd_pass_bytes_per_cell",
@@ -89,6 +90,8 @@ def load_library():
     lib.qpd_plan_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                   ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)]
     lib.qpd_plan_info.restype = i32
+    lib.qpd_plan_pass_bits.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
+    lib.qpd_plan_pass_bits.restype = i32
     lib.qpd_plan_destroy.argtypes = [vp]
     lib.qpd_plan_destroy.restype = i32
     lib.qpd_workspace_bytes.argtypes = [vp]
@@ -192,6 +195,9 @@ class Plan:
         self.compute_bits = int(cbits.value)        # precision of the transforms
         self.storage_bits = int(sbits.value)        # precision of the spectra between passes
         self.dlog10_estimate = float(est.value)     # the error estimate of that arithmetic
+        pb = (ctypes.c_int * 3)()
+        check(lib.qpd_plan_pass_bits(handle, pb))
+        self.pass_bits = tuple(int(v) for v in pb)   # bits of each pass's transforms, in pass order
 
     def _check_inputs(self, P, U, phase, I, work, fraction, region, which=None):
         """Shapes, dtypes, device and contiguity of the buffers a call reads or writes (every

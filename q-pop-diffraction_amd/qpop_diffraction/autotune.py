@@ -71,7 +71,7 @@ def plan_key(plan, fields_kind=""):
     h = hashlib.sha1(np.round(plan._R, 12).tobytes()).hexdigest()[:12]
     return (f"{device_name()}|fp{plan.precision}|dims{'x'.join(map(str, plan.dims))}|"
             f"axis{''.join(map(str, plan.axis))}|passes{''.join(map(str, plan.pass_dims))}|"
-            f"bits{plan.compute_bits}-{plan.storage_bits}|atoms{plan.n_atom}x{plan.n_phase}:{h}{fields_kind}")
+            f"bits{'-'.join(map(str, plan.pass_bits))}-{plan.storage_bits}|atoms{plan.n_atom}x{plan.n_phase}:{h}{fields_kind}")
 
 
 def time_pass(plan, which, P, U, phase, out, work, fraction=None, region=None, reps=3, warmup=1):

@@ -8,7 +8,10 @@ An FP32 plan transforms the contiguous dim first; its arithmetic is
            hypot(5.8e-3 / sqrt(N_last), 3.5e-3 / sqrt(N1 N_last)) (a term vanishes on one cell);
   * fp64:  FP64 transforms and spectra, the FP32 inputs widened on the device.
 The default is mixed where its estimate is at most 6e-4, else fp64, with the pass order of the
-smaller estimate among the orders that keep source groups.  Each arithmetic and order is
+smaller estimate among the orders that keep source groups; a mixed plan then gives FP32
+transforms to pass 1, then pass 2, while the estimate with a worst-case FP32-transform term of
+9.8e-3 sqrt(log2 N / N) for each stays within 6e-4 (strided passes of about 3200 points and more,
+whose prime factors are at most 17).  Each arithmetic and order is
 checked against the oracle: 1e-5 of the peak everywhere; |dlog10 I| <= 1e-3 for the ones the
 plan would choose; fp64 at the reference's FP64 parity bound (rtol 1e-9, 1e-12 of the peak;
 tests/test_mgpu.py:51-53 of the reference) against the oracle fed the same FP32-rounded order
@@ -35,10 +38,26 @@ def estimate(dims, pd1):
 
 
 def expected(dims):
-    """(pass dims, compute bits, storage bits) of the default plan (PbTiO3: source groups in
-    either order)."""
+    """(pass dims, bits of each pass's transforms, storage bits) of the default plan (PbTiO3:
+    source groups in either order)."""
     pd = (0, 2, 1) if estimate(dims, 2) < estimate(dims, 1) else (0, 1, 2)
-    return pd, 64, 32 if estimate(dims, pd[1]) <= 6e-4 else 64
+    est = estimate(dims, pd[1])
+    if est > 6e-4:
+        return pd, (64, 64, 64), 64
+    bits = [64, 64, 64]
+    def butterflies(n):  # every prime factor at most 17 (no direct stage)
+        for q in (2, 3, 5, 7, 11, 13, 17):
+            while n % q == 0:
+                n //= q
+        return n == 1
+
+    for d in (1, 2):
+        n = dims[pd[d]]
+        tx = 9.8e-3 * math.sqrt(math.log2(n) / n) if n > 1 and butterflies(n) else 1.0
+        if math.hypot(est, tx) <= 6e-4:
+            bits[d] = 32
+            est = math.hypot(est, tx)
+    return pd, tuple(bits), 32
 
 
 @pytest.mark.parametrize("dims", [(512, 512, 104), (128, 128, 30), (448, 448, 88), (256, 256, 64),
@@ -48,11 +67,12 @@ def test_plan_choice(dims):
     refl = Reflection((0, 0, 2), workloads.pto_atoms((0, 0, 2)))
     snap = workloads.snapshot_spec(*dims)
     plan = PlanCache(32).get(snap, 0, refl)
-    pd, cb, sb = expected(dims)
-    print(f"{dims}: passes {plan.pass_dims} bits {plan.compute_bits}/{plan.storage_bits} "
+    pd, pb, sb = expected(dims)
+    print(f"{dims}: passes {plan.pass_dims} bits {plan.pass_bits}/{plan.storage_bits} "
           f"estimate {plan.dlog10_estimate:.1e}", flush=True)
     assert plan.pass_dims == pd
-    assert (plan.compute_bits, plan.storage_bits) == (cb, sb)
+    assert (plan.pass_bits, plan.storage_bits) == (pb, sb)
+    assert plan.compute_bits == pb[0]
     assert 0.0 <= plan.dlog10_estimate <= 6e-4
 
 
@@ -133,3 +153,26 @@ def test_wide_inputs_with_weights(case):
         reg = to_xyz(region, 1).ravel()
     ref = diffraction_fp64(w.params, w.atoms, oPhase, P, U, reg)[0].transpose(2, 1, 0)
     np.testing.assert_allclose(I, ref, rtol=1e-9, atol=1e-12 * ref.max())
+
+
+def test_long_pass_fp32_transforms():
+    """A 3300-point strided axis: the default plan gives that pass FP32 transforms (its
+    worst-case term keeps the estimate within 6e-4) and the volume meets the FP32 bounds
+    against the oracle; the same grid with FP64 transforms everywhere beside it."""
+    from qpd_oracle import diffraction_fp64
+    dims = (16, 3300, 40)
+    pd, pb, sb = expected(dims)
+    assert 32 in pb and pb[0] == 64, pb
+    refl = Reflection((1, 0, 3), workloads.pto_atoms((1, 0, 3)))
+    snap = workloads.twin_snapshot(("long",) + dims, *dims, seed=31)
+    oPhase, P, U, region = workloads.host_inputs(snap)
+    ref = diffraction_fp64(snap.params, refl.atoms, oPhase, P, U, region)[0]
+    for arith in ("auto", "mixed"):
+        I, plan = run(snap, refl, "auto", arith)
+        print(f"{dims} {arith}: passes {plan.pass_dims} bits {plan.pass_bits} estimate "
+              f"{plan.dlog10_estimate:.1e}", flush=True)
+        if arith == "auto":
+            assert plan.pass_bits == pb
+        else:
+            assert plan.pass_bits == (64, 64, 64)
+        check_fp32(I, ref, f"{dims} {arith} bits {plan.pass_bits}")
