@@ -224,6 +224,31 @@ __device__ __forceinline__ void cell_source_f(const PassArgs<float>& a, const fl
     }
 }
 
+// The source inputs of an FP64 kernel on FP32 arrays seen as an FP32 pass's arguments (the
+// FP32 source helpers read P, U, the phases, the weights, qC, the axes and n_phase only).
+template <typename T>
+__device__ __forceinline__ PassArgs<float> source_view(const PassArgs<T>& a) {
+    if constexpr (std::is_same<T, float>::value) {
+        return a;
+    } else {
+        PassArgs<float> f{};
+        f.P = reinterpret_cast<const float*>(a.P);
+        f.U = a.U;
+        f.phase = a.phase;
+        f.fraction = reinterpret_cast<const float*>(a.fraction);
+        f.region = reinterpret_cast<const float*>(a.region);
+        f.ncell = a.ncell;
+        f.qC0 = a.qC0;
+        f.qC1 = a.qC1;
+        f.qC2 = a.qC2;
+        f.ax0 = a.ax0;
+        f.ax1 = a.ax1;
+        f.ax2 = a.ax2;
+        f.n_phase = a.n_phase;
+        return f;
+    }
+}
+
 // Workgroup barrier that orders LDS only.  __syncthreads() is a release/acquire fence
 // on every address space, so the compiler waits for all outstanding global loads
 // (vmcnt 0) before each barrier, which ends a register prefetch at the first LDS
@@ -545,7 +570,11 @@ __global__ __launch_bounds__(4 * G * SZ::TPL) void fast_source(PassArgs<T> a) {
     cx<T>* bufA = reinterpret_cast<cx<T>*>(smem_raw);
     cx<T>* bufB = bufA + TILE;
     cx<T>* Wl = bufB + TILE;
-    T* tabl = reinterpret_cast<T*>(Wl + N);  // atom table (n_atom * n_phase * kAtomTab)
+    // FP32 source (FP32 kernels, and FP64 ones on FP32 arrays): the FP32 atom table
+    constexpr bool F32 = std::is_same<T, float>::value;
+    constexpr bool SRC32 = F32 || IO32;
+    using TT = std::conditional_t<SRC32, float, T>;
+    TT* tabl = reinterpret_cast<TT*>(Wl + N);  // atom table (n_atom * n_phase * kAtomTab)
     const int tid = threadIdx.x;
     const int f = tid / (G * TPL);
     const int g = (tid / TPL) % G;
@@ -556,19 +585,22 @@ __global__ __launch_bounds__(4 * G * SZ::TPL) void fast_source(PassArgs<T> a) {
     const uint32_t row = row0 + g;
     load_twiddles<T, N>(Wl, a.W, tid, NTHR);
     const int ntab = a.n_atom_tab;
-    for (int m = tid; m < ntab; m += NTHR) tabl[m] = a.atomtab[m];
+    for (int m = tid; m < ntab; m += NTHR) tabl[m] = (TT)a.atomtab[m];
 
     // source cells of this thread: block cell e = tid + m * NTHR -> (row e / N, x e % N);
-    // FP32 splits the phase (CellInputsF), FP64 forms it whole per atom (cell_source)
-    constexpr bool F32 = std::is_same<T, float>::value;
-    using CellIn = std::conditional_t<F32, CellInputsF, CellInputs<T>>;
+    // the FP32 source splits the phase (CellInputsF) and is widened as it enters the FP64
+    // transforms of an FP64 kernel on FP32 arrays (FP64 phase reduction, FP32 values: the
+    // rounding is relative per cell, like the storage's); FP64 plans form it whole per atom
+    // (cell_source)
+    const PassArgs<float> af = source_view(a);
+    using CellIn = std::conditional_t<SRC32, CellInputsF, CellInputs<T>>;
     CellIn ci[CPT];
 #pragma unroll
     for (int m = 0; m < CPT; ++m) {
         const int e = tid + m * NTHR;
         const bool valid = e < G * N && row0 + e / N < nrows;
-        if constexpr (F32)
-            cell_load_f(a, cell0 + e, valid, ci[m]);
+        if constexpr (SRC32)
+            cell_load_f(af, cell0 + e, valid, ci[m]);
         else
             cell_load<T, IO32>(a, cell0 + e, valid, ci[m]);
     }
@@ -590,10 +622,16 @@ __global__ __launch_bounds__(4 * G * SZ::TPL) void fast_source(PassArgs<T> a) {
                 const int e = tid + m * NTHR;
                 if (e < G * N) {
                     cx<T> v0, v1, v2, v3;
-                    if constexpr (F32)
-                        cell_source_f(a, tabl, ci[m], cell0 + e, src, v0, v1, v2, v3);
-                    else
+                    if constexpr (SRC32) {
+                        cx<float> s0, s1, s2, s3;
+                        cell_source_f(af, tabl, ci[m], cell0 + e, src, s0, s1, s2, s3);
+                        v0 = cx_as<T>(s0);
+                        v1 = cx_as<T>(s1);
+                        v2 = cx_as<T>(s2);
+                        v3 = cx_as<T>(s3);
+                    } else {
                         cell_source<T, IO32>(a, tabl, ci[m], cell0 + e, src, v0, v1, v2, v3);
+                    }
                     const int rr = e / N, x = e - rr * N;
                     X[lds_addr<false, SZ, 0>(0 * G + rr, x)] = v0;
                     X[lds_addr<false, SZ, 0>(1 * G + rr, x)] = v1;
@@ -755,31 +793,6 @@ __device__ __forceinline__ void wave_line_transform(cx<T>* line, cx<T> (&v)[SZ::
                 }
             }
         }
-    }
-}
-
-// The source inputs of an FP64 kernel on FP32 arrays seen as an FP32 pass's arguments (the
-// FP32 source helpers read P, U, the phases, the weights, qC, the axes and n_phase only).
-template <typename T>
-__device__ __forceinline__ PassArgs<float> source_view(const PassArgs<T>& a) {
-    if constexpr (std::is_same<T, float>::value) {
-        return a;
-    } else {
-        PassArgs<float> f{};
-        f.P = reinterpret_cast<const float*>(a.P);
-        f.U = a.U;
-        f.phase = a.phase;
-        f.fraction = reinterpret_cast<const float*>(a.fraction);
-        f.region = reinterpret_cast<const float*>(a.region);
-        f.ncell = a.ncell;
-        f.qC0 = a.qC0;
-        f.qC1 = a.qC1;
-        f.qC2 = a.qC2;
-        f.ax0 = a.ax0;
-        f.ax1 = a.ax1;
-        f.ax2 = a.ax2;
-        f.n_phase = a.n_phase;
-        return f;
     }
 }
 
