@@ -297,6 +297,13 @@ struct qpd_plan {
     double qC[3];
     size_t w1_bytes() const { return (size_t)n1fields() * (size_t)ncell * ssz; }
     size_t w2_bytes() const { return (size_t)2 * nZ * (size_t)ncell * ssz; }
+    // a pass-2 FP64 split's intermediate (2 nZ complex FP64 fields) starts at pass 0's outputs,
+    // dead by then, which the workspace places after pass 1's: the bytes it needs beyond them
+    int split64 = 0;
+    size_t split64_bytes() const {
+        const size_t need = split64 ? (size_t)2 * nZ * (size_t)ncell * sizeof(cx<double>) : 0;
+        return need > w1_bytes() ? need - w1_bytes() : 0;
+    }
     // the widened inputs of a wide plan: P (3 n), fractions (n_phase n), region (n), doubles
     size_t wide_bytes() const { return wide ? (size_t)(4 + n_phase) * (size_t)ncell * sizeof(double) : 0; }
 };
@@ -765,13 +772,18 @@ int pass_candidates(const AxisFFT& ax, int d, int md, const int64_t dims[3], int
             }
             const bool splittable =
                 role == kRoleStrided || role == kRoleIntensity || role == kRoleMix1 || role == kRoleMix2;
-            if (!contig && allow_split && splittable && csize == 8) {
+            // FP64 transforms on FP32 spectra split only the intensity pass of a long axis: its
+            // first step writes an FP64 intermediate (FP32 storage there would round partial
+            // transforms of the line) into the workspace (qpd_plan::split64_bytes)
+            const bool split64 = csize == 16 && io32 && role == kRoleIntensity && N > 1024;
+            if (!contig && allow_split && splittable && (csize == 8 || split64)) {
                 std::vector<std::pair<double, PassPlan>> splits;
                 for (int64_t n1 = 2; n1 <= N / 2; ++n1) {
                     if (N % n1) continue;
                     const int64_t n2 = N / n1;
                     for (const FastEntry& ea : fast_registry()) {
                         if (ea.N != n2 || ea.role != kRoleSplit || (int)ea.lds > hard) continue;
+                        if ((size_t)ea.real_bytes * 2 != csize || ea.io32 != split64) continue;
                         if (!lane_offsets_fit(ea.G, n1, row_stride, csize)) continue;
                         const int role_b = role == kRoleStrided     ? kRoleStridedB
                                            : role == kRoleIntensity ? kRoleIntensityB
@@ -779,6 +791,7 @@ int pass_candidates(const AxisFFT& ax, int d, int md, const int64_t dims[3], int
                                                                     : kRoleMix2B;
                         for (const FastEntry& eb : fast_registry()) {
                             if (eb.N != n1 || eb.role != role_b || (int)eb.lds > hard) continue;
+                            if ((size_t)eb.real_bytes * 2 != csize || eb.io32) continue;
                             if (!lane_offsets_fit(eb.G, n2, row_stride, csize)) continue;
                             PassPlan pp;
                             pp.nlaunch = 2;
@@ -868,12 +881,14 @@ void set_strided(PassArgs<T>& a, int d, const int64_t dims[3]) {
 }
 
 // Enqueue pass d of a volume in element type T (the pass's own precision).  The workspace
-// holds pass 0's outputs (w1, pass-0 precision) followed by pass 1's (w2, pass-1 precision).
+// holds pass 1's outputs (w2) followed by pass 0's (w1), both in the storage precision, then
+// the FP64 intermediate of a split intensity pass beyond w1 (which it overwrites: pass 0's
+// outputs are dead once pass 1 has run), or the widened inputs of a wide plan.
 template <typename T>
 int run_pass(qpd_plan* p, const qpd_fields* f, double* I, void* work, int d, hipStream_t st) {
     const PlanTables& tb = p->tables(d);
-    unsigned char* w1 = static_cast<unsigned char*>(work);
-    unsigned char* w2 = w1 + p->w1_bytes();
+    unsigned char* w2 = static_cast<unsigned char*>(work);
+    unsigned char* w1 = w2 + p->w2_bytes();
     const int md = p->pd[d];
     const PassPlan& pp = p->pass[d];
     const Launch& L = pp.L[pp.nlaunch - 1];  // the launch that accumulates
@@ -923,10 +938,12 @@ int run_pass(qpd_plan* p, const qpd_fields* f, double* I, void* work, int d, hip
         }
     }
     if (pp.nlaunch == 2) {
-        // step 1: N2-point sub-transforms of every input field, twiddled, in place (FP32 plans
-        // without a tail only: the first step rewrites its input fields in their own precision)
-        if (p->mixed && p->csz[d] == 16)
-            return fail(QPD_ERR_UNSUPPORTED, "split pass in a plan computing in FP64 on FP32 spectra");
+        // step 1: N2-point sub-transforms of every input field, twiddled, in place; a pass
+        // computing in FP64 on FP32 spectra (the intensity pass only) writes them to the FP64
+        // intermediate at w1 instead, and step 2 reads that
+        const bool split64 = p->mixed && p->csz[d] == 16;
+        if (split64 && (d != 2 || !p->split64))
+            return fail(QPD_ERR_UNSUPPORTED, "FP64 split of a pass other than the intensity pass");
         const uint32_t N1 = (uint32_t)pp.N1, N2 = (uint32_t)pp.N2;
         PassArgs<T> s1 = a;
         s1.N = (int)N2;
@@ -938,9 +955,10 @@ int run_pass(qpd_plan* p, const qpd_fields* f, double* I, void* work, int d, hip
         s1.divQ = make_fastdiv(N1);
         s1.in_qmul = s1.out_qmul = 1;
         s1.in_step = s1.out_step = N1;
-        s1.out = const_cast<cx<T>*>(a.in);
+        s1.out = split64 ? reinterpret_cast<cx<T>*>(w1) : const_cast<cx<T>*>(a.in);
         s1.nfields = (d == 1 ? p->n1fields() : 2 * p->nZ);
         QPD_HIP(launch_pass<T>(pp.L[0], s1, false, false, st));
+        if (split64) a.in = reinterpret_cast<const cx<T>*>(w1);
         // step 2: N1-point transforms over rows N1 q + y1 into rows q + N2 k1, accumulated
         a.N = (int)N1;
         a.W = static_cast<const cx<T>*>(tb.Wlen.at(pp.N1));
@@ -1366,6 +1384,10 @@ int qpd_plan_create_opts(qpd_plan** plan, const qpd_grid* grid, const qpd_crysta
             p->pass[d] = p->cand[d][0];
         }
     }
+    // a split among the intensity pass's schedules of a plan computing in FP64 on FP32 spectra
+    // needs the workspace of its FP64 intermediate
+    if (rc == QPD_OK && p->mixed && p->csz[2] == 16)
+        for (const PassPlan& c : p->cand[2]) p->split64 = p->split64 || c.nlaunch == 2;
     // the tables of each element type some pass computes in
     bool need32 = false, need64 = false;
     for (int d = 0; d < 3; ++d) (p->csz[d] == 16 ? need64 : need32) = true;
@@ -1408,7 +1430,7 @@ int qpd_plan_destroy(qpd_plan* plan) {
 
 size_t qpd_workspace_bytes(const qpd_plan* p) {
     if (!p) return 0;
-    return p->w1_bytes() + p->w2_bytes() + p->wide_bytes();
+    return p->w1_bytes() + p->w2_bytes() + p->wide_bytes() + p->split64_bytes();
 }
 
 

@@ -1277,8 +1277,11 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided_mix(PassArgs<T> a) {
 // transformed (length N = SZ::N) and output k2 is multiplied by exp(-2 pi i q k2 / Nc)
 // and written back to row q + N1 k2 (in_qmul = out_qmul = 1, in_step = out_step = N1).
 // Each block reads all of a line before it writes it, and blocks own disjoint lines.
-template <typename T, class SZ, int G>
+// IO32 (FP64 only): reads the FP32 fields at a.in and writes the FP64 results at the same
+// places of a.out (a separate buffer), so the intermediate keeps FP64 precision.
+template <typename T, class SZ, int G, bool IO32 = false>
 __global__ __launch_bounds__(G* SZ::TPL) void fast_split(PassArgs<T> a) {
+    using TI = io_t<T, IO32>;
     constexpr int N = SZ::N, R0 = SZ::rad(0), NR0 = SZ::nr(0), LP = SZ::strided_lp(G);
     constexpr int TILE = SZ::strided_tile(G);
     constexpr bool ALT = (chunk_writes<SZ, false>() & 1) == 1;
@@ -1301,15 +1304,16 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_split(PassArgs<T> a) {
     const uint32_t jw = (uint32_t)__builtin_amdgcn_readfirstlane(tid / 64) * (64 / G);
     const uint32_t jl = (uint32_t)(tid % 64) / G;
     const size_t tile0 = (size_t)t0 * G + (size_t)outer * a.OS + (size_t)q * a.in_qmul * a.S;
-    const uint32_t in_lane = ((col_ok ? i0 : a.n0 - 1) - t0 * G + jl * rstride) * (uint32_t)sizeof(cx<T>);
+    const uint32_t in_lane = ((col_ok ? i0 : a.n0 - 1) - t0 * G + jl * rstride) * (uint32_t)sizeof(cx<TI>);
     load_twiddles<T, N>(Wl, a.W, tid, G * SZ::TPL);
     lds_barrier();
     constexpr bool PF = true;
+    const cx<TI>* src = IO32 ? reinterpret_cast<const cx<TI>*>(a.in) : reinterpret_cast<const cx<TI>*>(a.out);
     auto load_field = [&](int f, cx<T>(&dst)[1][SZ::RMAX]) {
-        const cx<T>* in = a.out + (size_t)f * a.ncell + tile0;
+        const cx<TI>* in = src + (size_t)f * a.ncell + tile0;
         if (j < NR0) {
 #pragma unroll
-            for (int r = 0; r < R0; ++r) dst[0][r] = load_at(in + (size_t)(jw + r * NR0) * rstride, in_lane);
+            for (int r = 0; r < R0; ++r) dst[0][r] = cx_as<T>(load_at(in + (size_t)(jw + r * NR0) * rstride, in_lane));
         }
     };
     cx<T> nxt[1][SZ::RMAX];

@@ -76,6 +76,10 @@ hipError_t fast_launch64(const Launch& L, const PassArgs<double>& a, hipStream_t
         return launch_one(fast_strided<double, SZ, G, 3, false, false, true, IO32, ONEBUF>, L, st, a);
     else if constexpr (ROLE == kRoleIntensity)
         return launch_one(fast_strided<double, SZ, G, 2, true, false, PF, IO32, ONEBUF>, L, st, a);
+    else if constexpr (ROLE == kRoleSplit)
+        return launch_one(fast_split<double, SZ, G, IO32>, L, st, a);
+    else if constexpr (ROLE == kRoleIntensityB)
+        return launch_one(fast_strided<double, SZ, G, 2, true, true, PF, IO32>, L, st, a);
     else if constexpr (ROLE == kRoleMix1)
         return launch_one(fast_strided_mix<double, SZ, G, 1, false, IO32, ONEBUF>, L, st, a);
     else
@@ -198,6 +202,12 @@ void add_fast_roles(std::vector<FastEntry>& v, const char* radices) {
     if constexpr ((ROLES >> kRoleIntensity) & 1u) add_fast64_variants<SZ, kRoleIntensity>(v, radices);
     if constexpr ((ROLES >> kRoleMix1) & 1u) add_fast64_variants<SZ, kRoleMix1>(v, radices);
     if constexpr ((ROLES >> kRoleMix2) & 1u) add_fast64_variants<SZ, kRoleMix2>(v, radices);
+    // FP64 split steps of the intensity pass of an FP32 plan computing in FP64: the first
+    // reads the FP32 spectra and writes an FP64 intermediate (io32), the second reads that
+    if constexpr ((ROLES >> kRoleSplit) & 1u)
+        add_fast64<SZ, kRoleSplit, fast_tile(kRoleSplit, SZ::TPL) / 2, true, true>(v, radices);
+    if constexpr ((ROLES >> kRoleIntensityB) & 1u)
+        add_fast64<SZ, kRoleIntensityB, fast_tile(kRoleIntensityB, SZ::TPL) / 2, true, false>(v, radices);
     if constexpr ((ROLES >> kRoleMix1B) & 1u) add_fast<SZ, kRoleMix1B, fast_tile(kRoleMix1B, SZ::TPL)>(v, radices);
     if constexpr ((ROLES >> kRoleMix2B) & 1u) add_fast<SZ, kRoleMix2B, fast_tile(kRoleMix2B, SZ::TPL)>(v, radices);
 }

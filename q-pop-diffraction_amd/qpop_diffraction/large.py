@@ -62,11 +62,11 @@ def fp64_crop(vol, params, tables, crop=(512, 512), slab=200, log=None):
     red = _plan((nz, cy, nx), (2, 1, 0), params, tables, 64)
     n_red = nz * cy * nx
     work_red = torch.zeros(red.workspace_bytes // 16, dtype=torch.complex128, device="cuda")
-    # workspace of an FP64 plan: the pass-0 outputs, then the 2 nZ pass-1 outputs (pass 2's inputs)
+    # workspace of an FP64 plan: the 2 nZ pass-1 outputs (pass 2's inputs) first, then the
+    # pass-0 outputs
     z0 = torch.zeros((3, 1), dtype=torch.float64, device="cuda")
     i0 = torch.zeros(1, dtype=torch.int8, device="cuda")
     n_in2 = int(round((red.pass_bytes_per_cell(2, z0, z0, i0) - 8.0) / 16.0))
-    n_fields = red.workspace_bytes // 16 // n_red
     plane = nz * ny
     for x0 in range(0, nx, slab):
         x1 = min(nx, x0 + slab)
@@ -80,10 +80,9 @@ def fp64_crop(vol, params, tables, crop=(512, 512), slab=200, log=None):
         dummy = torch.empty(1, dtype=torch.float64, device="cuda")
         sp.run_pass(0, P, U, ph, dummy, work)
         sp.run_pass(1, P, U, ph, dummy, work)
-        n1 = sp.workspace_bytes // 16 // n_s - n_in2  # fields before the pass-1 outputs
-        out1 = work[n1 * n_s:].view(n_in2, w, ny, nz)
+        out1 = work[:n_in2 * n_s].view(n_in2, w, ny, nz)
         sel = out1.index_select(2, ky_full) * (n_s / n_full)  # (fields, w, cy, nz)
-        dst = work_red[(n_fields - n_in2) * n_red:].view(n_in2, nx, cy, nz)
+        dst = work_red[:n_in2 * n_red].view(n_in2, nx, cy, nz)
         dst[:, x0:x1].index_copy_(2, ky_red, sel)
         del work, P, U, ph, out1, sel
         sp.close()

@@ -61,7 +61,8 @@ def test_fft3d_every_strided_length():
     print(f"strided lengths: worst {worst:.2e}")
 
 
-def volume_check(refl, nx, ny, nz, seed=0, expect=None, schedules=None, pick=None, order="memory", arith="fp32"):
+def volume_check(refl, nx, ny, nz, seed=0, expect=None, schedules=None, pick=None, order="memory", arith="fp32",
+                 strict=False):
     """FP32 volume of a twin snapshot against the oracle, with the kernels under test selected;
     returns max |dI| / peak.  The same snapshot through the default plan is held to the FP32
     bounds.
@@ -69,7 +70,7 @@ def volume_check(refl, nx, ny, nz, seed=0, expect=None, schedules=None, pick=Non
     order, arith: plan options (by default the memory order and FP32 in every pass, so that
     pass d runs along memory dim d in FP32); expect: {pass: prefix of the default schedule's
     name}; pick: {pass: suffix} selects the first schedule whose name ends with it (which must
-    exist)."""
+    exist); strict: the plan under test is also held to the FP32 bounds (|dlog10 I| included)."""
     import torch
     from qpd_oracle import diffraction_fp64
     snap = workloads.twin_snapshot(("len", nx, ny, nz), nx, ny, nz, seed=seed)
@@ -93,6 +94,8 @@ def volume_check(refl, nx, ny, nz, seed=0, expect=None, schedules=None, pick=Non
     I = out.cpu().numpy().reshape(nz, ny, nx).transpose(2, 1, 0)
     ref = diffraction_fp64(snap.params, refl.atoms, *workloads.host_inputs(snap))[0]
     assert np.argmax(I) == np.argmax(ref), (nx, ny, nz)
+    if strict:
+        check_fp32(I, ref, f"plan under test {(nx, ny, nz)} {[plan.schedules(d)[plan.schedule(d)] for d in range(3)]}")
     dplans = PlanCache(32)
     dplan = dplans.get(snap, 0, refl)
     dwork = torch.empty(dplan.workspace_bytes, dtype=torch.uint8, device="cuda")
@@ -166,6 +169,17 @@ def test_split_long_strided(refl, N, dim):
     shape[dim] = N
     print(f"split N={N} on dim {dim}, grid {shape}", flush=True)
     err = volume_check(refl, *shape, expect={dim: "split"})
+    print(f"  max|dI|/peak {err:.2e}")
+    assert err <= FP32_PEAK_TOL
+
+
+@pytest.mark.parametrize("N", [2000, 4000])
+def test_split_intensity_fp64(refl, N):
+    """The intensity pass of a plan computing in FP64 on FP32 spectra, split in two launches
+    with an FP64 intermediate in the workspace (its first step reads the FP32 spectra)."""
+    shape = [5, 4, N]
+    print(f"fp64 split N={N} on dim 2, grid {shape}", flush=True)
+    err = volume_check(refl, *shape, expect={2: "split"}, arith="mixed", strict=True)
     print(f"  max|dI|/peak {err:.2e}")
     assert err <= FP32_PEAK_TOL
 
