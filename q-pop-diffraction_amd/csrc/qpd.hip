@@ -727,7 +727,7 @@ hipError_t launch_pass(const Launch& L, const PassArgs<T>& a, bool source, bool 
 // (an FP32 plan computing in FP64).
 int pass_candidates(const AxisFFT& ax, int d, int md, const int64_t dims[3], int nin, size_t csize, bool io32,
                     bool intensity_last, int atomtab_entries, bool allow_split, std::vector<PassPlan>& out,
-                    int mix_role = -1) {
+                    int mix_role = -1, int nsrc = 0) {
     out.clear();
     const int hard = max_lds_bytes();
     const int soft = std::min(hard, 65536);
@@ -759,6 +759,7 @@ int pass_candidates(const AxisFFT& ax, int d, int md, const int64_t dims[3], int
                 if (rl < 0) continue;
                 for (const FastEntry& e : fast_registry()) {
                     if (e.N != N || e.role != rl || (size_t)e.real_bytes * 2 != csize || e.io32 != io32) continue;
+                    if (e.nsrc != 0 && e.nsrc != nsrc) continue;
                     if (rl == kRoleSourceW && (pos == 0) != (kernel_scratch_bytes(e.kernel) <= 32)) continue;
                     const bool src = rl == kRoleSource || rl == kRoleSourceW;
                     const size_t lds = e.lds + (src ? csize / 2 * (size_t)atomtab_entries : 0);
@@ -1138,9 +1139,10 @@ std::string describe_launch(const Launch& L) {
                                                "split-step", "strided", "intensity", "groups", "groups",
                                                "groups", "groups", "source-wave"};
     if (L.fe)
-        std::snprintf(buf, sizeof(buf), "fast %d (%s) G%d %s%s%s%s%s", L.fe->N, L.fe->radices, L.fe->G,
-                      role_name[L.fe->role], L.fe->no_prefetch ? " np" : "", L.fe->real_bytes == 8 ? " f64" : "",
-                      L.fe->io32 ? " io32" : "", L.fe->one_buffer ? " 1buf" : "");
+        std::snprintf(buf, sizeof(buf), "fast %d (%s) G%d %s%s%s%s%s%s", L.fe->N, L.fe->radices, L.fe->G,
+                      role_name[L.fe->role], L.fe->nsrc ? (L.fe->nsrc == 2 ? " held2" : " held3") : "",
+                      L.fe->no_prefetch ? " np" : "", L.fe->real_bytes == 8 ? " f64" : "", L.fe->io32 ? " io32" : "",
+                      L.fe->one_buffer ? " 1buf" : "");
     else
         std::snprintf(buf, sizeof(buf), "generic %s%s nl%d nf%d nthr%d ept%d%s%s", L.contiguous ? "contiguous" : "strided",
                       L.mix ? " groups" : "", L.nl, L.nf, L.nthr, L.ept,
@@ -1363,7 +1365,7 @@ int qpd_plan_create_opts(qpd_plan** plan, const qpd_grid* grid, const qpd_crysta
     for (int d = 0; d < 3 && rc == QPD_OK; ++d) {
         const bool io32 = p->mixed != 0 && p->csz[d] == 16;
         rc = pass_candidates(p->fft[d], d, p->pd[d], p->dims, nin[d], p->csz[d], io32, true, tab_entries, true,
-                             p->cand[d], p->grouped ? mix_role : -1);
+                             p->cand[d], p->grouped ? mix_role : -1, p->grouped ? p->nS : 0);
         if (d == 1 && p->grouped && rc != QPD_OK) {
             // no source-group kernel (direct or split) for this length: keep the class form
             g_error.clear();

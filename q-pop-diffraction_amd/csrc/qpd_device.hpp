@@ -249,11 +249,83 @@ struct OddRoots {
 template <int R>
 __device__ constexpr OddRoots<R> kOddRoots{};
 
+// The same constants for the FP64 DFTs, evaluated in double-double arithmetic (a value is an
+// unevaluated sum hi + lo of two doubles, about 106 bits; Dekker's product split, Knuth's
+// two-sum) and rounded to double: the double nearest cos, sin(2 pi m / R), which is the value
+// of the host's twiddle table (cosl / sinl rounded to double).  As literals they take no
+// vector registers: an FP64 radix-13 stage otherwise holds its twelve in 24 VGPRs.
+struct DD {
+    double hi, lo;
+};
+constexpr DD dd_two_sum(double a, double b) {
+    const double s = a + b, bb = s - a;
+    return {s, (a - (s - bb)) + (b - bb)};
+}
+constexpr DD dd_fast_two_sum(double a, double b) {
+    const double s = a + b;
+    return {s, b - (s - a)};
+}
+constexpr DD dd_split(double a) {
+    const double t = 134217729.0 * a;  // 2^27 + 1
+    const double hi = t - (t - a);
+    return {hi, a - hi};
+}
+constexpr DD dd_two_prod(double a, double b) {
+    const double p = a * b;
+    const DD as = dd_split(a), bs = dd_split(b);
+    return {p, ((as.hi * bs.hi - p) + as.hi * bs.lo + as.lo * bs.hi) + as.lo * bs.lo};
+}
+constexpr DD dd_add(DD a, DD b) {
+    DD s = dd_two_sum(a.hi, b.hi);
+    const DD t = dd_two_sum(a.lo, b.lo);
+    s = dd_fast_two_sum(s.hi, s.lo + t.hi);
+    return dd_fast_two_sum(s.hi, s.lo + t.lo);
+}
+constexpr DD dd_mul(DD a, DD b) {
+    const DD p = dd_two_prod(a.hi, b.hi);
+    return dd_fast_two_sum(p.hi, p.lo + (a.hi * b.lo + a.lo * b.hi));
+}
+constexpr DD dd_div(DD a, double b) {  // b a small integer
+    const double q1 = a.hi / b;
+    const DD p = dd_two_prod(q1, b);
+    const DD r = dd_add(a, {-p.hi, -p.lo});
+    return dd_fast_two_sum(q1, r.hi / b);
+}
+// sin (odd) or cos of x, |x| <= pi / 2, by its Taylor series to 27 terms
+constexpr double dd_taylor(DD x, bool odd) {
+    const DD x2 = dd_mul(x, x);
+    DD term = odd ? x : DD{1.0, 0.0};
+    DD sum = term;
+    for (int k = 1; k < 28; ++k) {
+        const double d = odd ? (2.0 * k) * (2.0 * k + 1.0) : (2.0 * k - 1.0) * (2.0 * k);
+        term = dd_div(dd_mul(term, {-x2.hi, -x2.lo}), d);
+        sum = dd_add(sum, term);
+    }
+    return sum.hi + sum.lo;
+}
+
+template <int R>
+struct OddRoots64 {
+    double c[R / 2 + 1], s[R / 2 + 1];
+    constexpr OddRoots64() : c(), s() {
+        constexpr DD pi = {3.141592653589793116, 1.2246467991473532e-16};
+        for (int m = 0; m <= R / 2; ++m) {
+            const bool past = 4 * m > R;  // angle above pi / 2
+            const DD x = dd_div(dd_mul(pi, {(double)(past ? R - 2 * m : 2 * m), 0.0}), (double)R);
+            c[m] = past ? -dd_taylor(x, false) : dd_taylor(x, false);
+            s[m] = dd_taylor(x, true);
+        }
+    }
+};
+
+template <int R>
+__device__ constexpr OddRoots64<R> kOddRoots64{};
+
 // Odd-length DFT y[t] = sum_s v[s] exp(-2 pi i t s / R) using the symmetric pairs
-// a_s = v[s] + v[R-s], b_s = v[s] - v[R-s]; cr[m], sr[m] = cos, sin(2 pi m / R) (FP32 uses
-// the compile-time values of kOddRoots instead).
+// a_s = v[s] + v[R-s], b_s = v[s] - v[R-s], with cos, sin(2 pi m / R) the compile-time
+// constants kOddRoots (FP32) or kOddRoots64 (FP64).
 template <int R, typename T>
-__device__ __forceinline__ void dft_odd(cx<T>* v, const T* cr, const T* sr) {
+__device__ __forceinline__ void dft_odd(cx<T>* v) {
     constexpr int H = (R - 1) / 2;
     cx<T> a[H + 1], b[H + 1];
     cx<T> y0 = v[0];
@@ -273,9 +345,9 @@ __device__ __forceinline__ void dft_odd(cx<T>* v, const T* cr, const T* sr) {
         for (int s = 1; s <= H; ++s) {
             const int m = (t * s) % R;
             const int mm = m <= H ? m : R - m;
-            constexpr bool lit = std::is_same<T, float>::value;
-            const T c = lit ? (T)kOddRoots<R>.c[mm] : cr[mm];
-            const T s1 = lit ? (T)kOddRoots<R>.s[mm] : sr[mm];
+            constexpr bool f32 = std::is_same<T, float>::value;
+            const T c = f32 ? (T)kOddRoots<R>.c[mm] : (T)kOddRoots64<R>.c[mm];
+            const T s1 = f32 ? (T)kOddRoots<R>.s[mm] : (T)kOddRoots64<R>.s[mm];
             const T sn = m <= H ? s1 : -s1;
             A.x += a[s].x * c;
             A.y += a[s].y * c;
@@ -290,7 +362,7 @@ __device__ __forceinline__ void dft_odd(cx<T>* v, const T* cr, const T* sr) {
 }
 
 template <int R, typename T>
-__device__ __forceinline__ void dft(cx<T>* v, const T* cr, const T* sr) {
+__device__ __forceinline__ void dft(cx<T>* v) {
     if constexpr (R == 2) {
         dft2(v);
     } else if constexpr (R == 4) {
@@ -300,7 +372,7 @@ __device__ __forceinline__ void dft(cx<T>* v, const T* cr, const T* sr) {
     } else if constexpr (R == 8) {
         dft8(v);
     } else {
-        dft_odd<R>(v, cr, sr);
+        dft_odd<R>(v);
     }
 }
 
@@ -315,15 +387,6 @@ __device__ __forceinline__ void fft_stage(cx<T>* __restrict__ tile, const cx<T>*
                                           int tid, int nthr) {
     constexpr int MB = (MR / R) > 0 ? (MR / R) : 1;
     const int total = NR * Lc;
-    T cr[R / 2 + 1], sr[R / 2 + 1];
-    if constexpr ((R & 1) == 1) {
-#pragma unroll
-        for (int m = 1; m <= R / 2; ++m) {
-            cx<T> w = W[m * NR];
-            cr[m] = w.x;
-            sr[m] = -w.y;
-        }
-    }
     cx<T> v[MB][R];
 #pragma unroll
     for (int b = 0; b < MB; ++b) {
@@ -349,7 +412,7 @@ __device__ __forceinline__ void fft_stage(cx<T>* __restrict__ tile, const cx<T>*
 #pragma unroll
                 for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], W[r * base]);
             }
-            dft<R>(v[b], cr, sr);
+            dft<R>(v[b]);
             const int o = q * Ns * R + k;
 #pragma unroll
             for (int r = 0; r < R; ++r) tile[(o + r * Ns) * stride + line] = v[b][r];

@@ -307,16 +307,6 @@ template <typename T, class SZ, int S, int F>
 __device__ __forceinline__ void stage_compute(cx<T> (&v)[F][SZ::RMAX], int j, const cx<T>* __restrict__ W) {
     constexpr int R = SZ::rad(S);
     constexpr int NS = SZ::ns(S);
-    constexpr int NR = SZ::nr(S);
-    T cr[R / 2 + 1], sr[R / 2 + 1];
-    if constexpr ((R & 1) == 1) {
-#pragma unroll
-        for (int m = 1; m <= R / 2; ++m) {
-            const cx<T> w = W[m * NR];
-            cr[m] = w.x;
-            sr[m] = -w.y;
-        }
-    }
     if constexpr (NS > 1) {
         constexpr int TWS = SZ::N / (NS * R);
         const int k = j % NS;
@@ -333,7 +323,7 @@ __device__ __forceinline__ void stage_compute(cx<T> (&v)[F][SZ::RMAX], int j, co
         cx<T> t[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) t[r] = v[f][r];
-        dft<R>(t, cr, sr);
+        dft<R>(t);
 #pragma unroll
         for (int r = 0; r < R; ++r) v[f][r] = t[r];
     }
@@ -688,7 +678,7 @@ __global__ __launch_bounds__(4 * G * SZ::TPL) void fast_source(PassArgs<T> a) {
 // A, B and C are stored from registers.  With one-hot phases both groups of a pair come
 // from one read of the cells (see the one-hot branch); fractional phases take one group
 // per read.  The twiddles of stage s sit in LDS as entry (r - 1) NS + k, so the lanes of a
-// wave read consecutive entries; odd radices keep their DFT constants after them.
+// wave read consecutive entries.
 constexpr int kSrcWaves = 4;  // waves per block of the one-wave-per-row source pass
 
 __device__ __forceinline__ void wave_sync() {
@@ -698,23 +688,21 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 // Layout of the per-stage twiddle table: stage s starts at off(s) with ntw(s) twiddles
-// W_N^{r k N / (NS R)} at (r - 1) NS + k, then ncs(s) constants W_N^{m N / R}, m = 1..R/2.
+// W_N^{r k N / (NS R)} at (r - 1) NS + k.
 template <class SZ>
 struct StageTw {
     static constexpr int ntw(int s) { return SZ::ns(s) > 1 ? (SZ::rad(s) - 1) * SZ::ns(s) : 0; }
-    static constexpr int ncs(int s) { return (SZ::rad(s) & 1) ? SZ::rad(s) / 2 : 0; }
-    static constexpr int off(int s) { return s == 0 ? 0 : off(s - 1) + ntw(s - 1) + ncs(s - 1); }
+    static constexpr int off(int s) { return s == 0 ? 0 : off(s - 1) + ntw(s - 1); }
     static constexpr int total() { return off(SZ::NST); }
 };
 
 template <typename T, class SZ, int S>
 __device__ __forceinline__ void fill_stage_tw(cx<T>* tw, const cx<T>* __restrict__ W, int tid, int nthr) {
     if constexpr (S < SZ::NST) {
-        constexpr int R = SZ::rad(S), NS = SZ::ns(S), NR = SZ::nr(S);
-        constexpr int OFF = StageTw<SZ>::off(S), NTW = StageTw<SZ>::ntw(S), NCS = StageTw<SZ>::ncs(S);
+        constexpr int R = SZ::rad(S), NS = SZ::ns(S);
+        constexpr int OFF = StageTw<SZ>::off(S), NTW = StageTw<SZ>::ntw(S);
         constexpr int TWS = SZ::N / (NS * R);
-        for (int e = tid; e < NTW + NCS; e += nthr)
-            tw[OFF + e] = e < NTW ? W[(e / NS + 1) * (e % NS) * TWS] : W[(e - NTW + 1) * NR];
+        for (int e = tid; e < NTW; e += nthr) tw[OFF + e] = W[(e / NS + 1) * (e % NS) * TWS];
     }
 }
 
@@ -722,22 +710,13 @@ __device__ __forceinline__ void fill_stage_tw(cx<T>* tw, const cx<T>* __restrict
 template <typename T, class SZ, int S>
 __device__ __forceinline__ void stage_compute_tw(cx<T> (&v)[SZ::RMAX], int j, const cx<T>* tw) {
     constexpr int R = SZ::rad(S), NS = SZ::ns(S);
-    constexpr int OFF = StageTw<SZ>::off(S), NTW = StageTw<SZ>::ntw(S);
-    T cr[R / 2 + 1], sr[R / 2 + 1];
-    if constexpr ((R & 1) == 1) {
-#pragma unroll
-        for (int m = 1; m <= R / 2; ++m) {
-            const cx<T> w = tw[OFF + NTW + m - 1];
-            cr[m] = w.x;
-            sr[m] = -w.y;
-        }
-    }
+    constexpr int OFF = StageTw<SZ>::off(S);
     if constexpr (NS > 1) {
         const int k = j % NS;
 #pragma unroll
         for (int r = 1; r < R; ++r) v[r] = cmul(v[r], tw[OFF + (r - 1) * NS + k]);
     }
-    dft<R>(v, cr, sr);
+    dft<R>(v);
 }
 
 // All stages of one line: the stage-0 inputs x[j + r N/R0] are in v on entry, the outputs
@@ -1269,6 +1248,115 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided_mix(PassArgs<T> a) {
             for (int r = 0; r < RL; ++r)
                 store_at(o + (size_t)((out0 + (jw + r * NRL) * out_step) * a.S), out_lane, cx_as<TI>(acc[z][oq][r]));
         }
+}
+
+// ---------------------------------------------------------------------------
+// Pass 1 over NS source groups (NS known at compile time), forming the same outputs as
+// fast_strided_mix with fewer values held per thread.  With the column's mix factors
+// M_{Z,S}(k0, k1) = sum over y-classes Y in Z of c_Y(k1) X_{Y,S}(k0),
+//   out_{Z,0} = sum_S M_{Z,S} T_S,  T_S = F1 A_S - i k1 F1 B_S,
+//   out_{Z,1} = sum_S M_{Z,S} F1 C_S.
+// The inputs are taken in the order A_0 B_0 A_1 B_1 ... then C_0 C_1 ...: once every
+// group's T_S is held, the first outputs of all z-classes are formed and stored, and the
+// held values are reused for the F1 C_S.  A thread holds NS RL group values (the last
+// transform of a round runs beside NS - 1 of them), where fast_strided_mix holds NZ 2 RL
+// sums and 2 RL group values; the prefetched line stays in the arrays' own type.
+template <typename T, class SZ, int G, int NZ, int NS, bool IO32 = false, bool ONEBUF = false>
+__global__ __launch_bounds__(G* SZ::TPL) void fast_mix_held(PassArgs<T> a) {
+    using TI = io_t<T, IO32>;
+    constexpr int N = SZ::N, R0 = SZ::rad(0), NR0 = SZ::nr(0), LP = SZ::strided_lp(G), RL = SZ::RL, NRL = SZ::NRL;
+    constexpr int TILE = SZ::strided_tile(G);
+    constexpr bool ALT = !ONEBUF && (chunk_writes<SZ, false>() & 1) == 1;
+    constexpr int NITEMS = 3 * NS;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T>* bufA = reinterpret_cast<cx<T>*>(smem_raw);
+    cx<T>* bufB = ONEBUF ? bufA : bufA + TILE;
+    cx<T>* Wl = bufB + TILE;
+    const int tid = threadIdx.x;
+    const int g = tid % G, j = tid / G;
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t t0 = blk % a.ntiles0;
+    const uint32_t outer = blk / a.ntiles0;
+    const uint32_t i0 = t0 * G + g;
+    const bool col_ok = i0 < a.n0;
+    // row addresses: wave-uniform pointer + 32-bit lane offset, as in fast_strided
+    static_assert(64 % G == 0, "a wave holds whole rows of the tile");
+    const uint32_t jw = (uint32_t)__builtin_amdgcn_readfirstlane(tid / 64) * (64 / G);
+    const uint32_t jl = (uint32_t)(tid % 64) / G;
+    const size_t tile0 = (size_t)t0 * G + (size_t)outer * a.OS;
+    const uint32_t in_lane = ((col_ok ? i0 : a.n0 - 1) - t0 * G + jl * a.S) * (uint32_t)sizeof(cx<TI>);
+    const uint32_t out_lane = (g + jl * a.S) * (uint32_t)sizeof(cx<TI>);
+    const uint32_t coff = jl * (uint32_t)sizeof(cx<T>);
+    const uint32_t koff = jl * (uint32_t)sizeof(T);
+    load_twiddles<T, N>(Wl, a.W, tid, G * SZ::TPL);
+    lds_barrier();
+    // item it -> input field: A_s, B_s for s = it / 2 while it < 2 NS, then C_{it - 2 NS}
+    auto load_item = [&](int it, cx<TI>(&dst)[SZ::RMAX]) {
+        const int fld = it < 2 * NS ? (it >> 1) * 3 + (it & 1) : (it - 2 * NS) * 3 + 2;
+        const cx<TI>* in = reinterpret_cast<const cx<TI>*>(a.in) + (size_t)fld * a.ncell + tile0;
+        if (j < NR0) {
+#pragma unroll
+            for (int r = 0; r < R0; ++r) dst[r] = load_at(in + (size_t)((jw + r * NR0) * a.S), in_lane);
+        }
+    };
+    // out_{Z,oq} = sum_S M_{Z,S} held[S] for every z-class, stored at once.  The pair lists
+    // are walked by the whole wave (their entries stay in scalar registers).
+    auto mix_store = [&](int oq, const cx<T>(&held)[NS][RL]) {
+#pragma unroll
+        for (int z = 0; z < NZ; ++z) {
+            cx<T> acc[RL];
+#pragma unroll
+            for (int r = 0; r < RL; ++r) acc[r] = {(T)0, (T)0};
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                const int p1 = __builtin_amdgcn_readfirstlane(a.pair_first[s * NZ + z + 1]);
+                for (int p = __builtin_amdgcn_readfirstlane(a.pair_first[s * NZ + z]); p < p1; ++p) {
+                    const int y = __builtin_amdgcn_readfirstlane(a.pair_y[p]);
+                    const cx<T>* xrow = a.mixc + (size_t)(y * NS + s) * a.n0 + t0 * G;
+                    const cx<T>* cy = a.coef + (size_t)y * a.Nc;
+                    if (j < NRL && col_ok) {
+                        const cx<T> xc = load_at(xrow, g * (uint32_t)sizeof(cx<T>));
+#pragma unroll
+                        for (int r = 0; r < RL; ++r)
+                            acc[r] = acc[r] + cmul(cmul(load_at(cy + (jw + r * NRL), coff), xc), held[s][r]);
+                    }
+                }
+            }
+            if (j < NRL && col_ok) {
+                cx<TI>* o = reinterpret_cast<cx<TI>*>(a.out) + (size_t)(2 * z + oq) * a.ncell + tile0;
+#pragma unroll
+                for (int r = 0; r < RL; ++r) store_at(o + (size_t)((jw + r * NRL) * a.S), out_lane, cx_as<TI>(acc[r]));
+            }
+        }
+    };
+    cx<T> held[NS][RL];
+    cx<TI> nxt[SZ::RMAX];
+    load_item(0, nxt);
+    int par = 0;
+#pragma unroll 1
+    for (int it = 0; it < NITEMS; ++it) {
+        cx<T> v[1][SZ::RMAX];
+#pragma unroll
+        for (int r = 0; r < SZ::RMAX; ++r) v[0][r] = cx_as<T>(nxt[r]);
+        if (it + 1 < NITEMS) load_item(it + 1, nxt);
+        transform<T, SZ, 1, true, LP, false, ONEBUF>(par ? bufB : bufA, par ? bufA : bufB, v, j, g, 0, Wl);
+        if (ALT) par ^= 1;
+        if (j < NRL) {
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                if (it == 2 * s || it == 2 * NS + s) {
+#pragma unroll
+                    for (int r = 0; r < RL; ++r) held[s][r] = v[0][r];
+                } else if (it == 2 * s + 1) {
+#pragma unroll
+                    for (int r = 0; r < RL; ++r)
+                        held[s][r] = held[s][r] + cscale(negi(v[0][r]), load_at(a.k + (jw + r * NRL), koff));
+                }
+            }
+        }
+        if (it == 2 * NS - 1) mix_store(0, held);
+        if (it == NITEMS - 1) mix_store(1, held);
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -129,6 +129,43 @@ void add_fast64_variants(std::vector<FastEntry>& v, const char* radices) {
     add_fast64_tiles<SZ, ROLE, true>(v, radices);
 }
 
+// Pass 1 over exactly NS source groups with the group spectra held (fast_mix_held), listed
+// before the general pass over source groups: FP32 at the strided tile, FP64 (and FP64 on
+// FP32 arrays) at half of it.
+template <class SZ, int NZ, int NS, int G>
+hipError_t held_launch(const Launch& L, const PassArgs<float>& a, hipStream_t st) {
+    return launch_one(fast_mix_held<float, SZ, G, NZ, NS>, L, st, a);
+}
+template <class SZ, int NZ, int NS, int G, bool IO32>
+hipError_t held_launch64(const Launch& L, const PassArgs<double>& a, hipStream_t st) {
+    return launch_one(fast_mix_held<double, SZ, G, NZ, NS, IO32>, L, st, a);
+}
+
+template <class SZ, int ROLE, int NS>
+void add_held(std::vector<FastEntry>& v, const char* radices) {
+    constexpr int NZ = ROLE == kRoleMix1 ? 1 : 2;
+    constexpr int G = fast_tile(ROLE, SZ::TPL);
+    constexpr size_t tile = (size_t)SZ::strided_tile(G), tile64 = (size_t)SZ::strided_tile(G / 2);
+    if constexpr (G >= 1 && (2 * tile + SZ::N) * 8 <= 160 * 1024) {
+        FastEntry e{SZ::N, ROLE, G, G * SZ::TPL, SZ::TPL, (2 * tile + SZ::N) * 8, radices,
+                    &held_launch<SZ, NZ, NS, G>, (const void*)fast_mix_held<float, SZ, G, NZ, NS>, false};
+        e.nsrc = NS;
+        v.push_back(e);
+    }
+    if constexpr (G / 2 >= 4 && (G / 2) * SZ::TPL <= 256 && (2 * tile64 + SZ::N) * 16 <= 160 * 1024) {
+        for (const bool io32 : {true, false}) {
+            if (!io32 && NS > 2) continue;
+            FastEntry e{SZ::N, ROLE, G / 2, (G / 2) * SZ::TPL, SZ::TPL, (2 * tile64 + SZ::N) * 16, radices, nullptr,
+                        io32 ? (const void*)fast_mix_held<double, SZ, G / 2, NZ, NS, true>
+                             : (const void*)fast_mix_held<double, SZ, G / 2, NZ, NS, false>,
+                        false, 8, io32 ? &held_launch64<SZ, NZ, NS, G / 2, true> : &held_launch64<SZ, NZ, NS, G / 2, false>,
+                        io32};
+            e.nsrc = NS;
+            v.push_back(e);
+        }
+    }
+}
+
 // The one-wave-per-row source pass (lengths whose lines fit one wave and whose first
 // radix is at most 8, so that the four fields' stage-0 inputs fit in registers): G is the
 // rows of a block; the LDS holds one line per row and the per-stage twiddle table.
@@ -195,8 +232,16 @@ void add_fast_roles(std::vector<FastEntry>& v, const char* radices) {
         add_fast<SZ, kRoleStridedB, fast_tile(kRoleStridedB, SZ::TPL)>(v, radices);
     if constexpr ((ROLES >> kRoleIntensityB) & 1u)
         add_fast<SZ, kRoleIntensityB, fast_tile(kRoleIntensityB, SZ::TPL)>(v, radices);
-    if constexpr ((ROLES >> kRoleMix1) & 1u) add_fast_variants<SZ, kRoleMix1>(v, radices);
-    if constexpr ((ROLES >> kRoleMix2) & 1u) add_fast_variants<SZ, kRoleMix2>(v, radices);
+    if constexpr ((ROLES >> kRoleMix1) & 1u) {
+        add_held<SZ, kRoleMix1, 2>(v, radices);
+        add_held<SZ, kRoleMix1, 3>(v, radices);
+        add_fast_variants<SZ, kRoleMix1>(v, radices);
+    }
+    if constexpr ((ROLES >> kRoleMix2) & 1u) {
+        add_held<SZ, kRoleMix2, 2>(v, radices);
+        add_held<SZ, kRoleMix2, 3>(v, radices);
+        add_fast_variants<SZ, kRoleMix2>(v, radices);
+    }
     if constexpr ((ROLES >> kRoleSource) & 1u) add_fast64_variants<SZ, kRoleSource>(v, radices);
     if constexpr ((ROLES >> kRoleStrided) & 1u) add_fast64_variants<SZ, kRoleStrided>(v, radices);
     if constexpr ((ROLES >> kRoleIntensity) & 1u) add_fast64_variants<SZ, kRoleIntensity>(v, radices);
