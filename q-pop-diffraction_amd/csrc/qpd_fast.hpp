@@ -1296,7 +1296,13 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_mix_held(PassArgs<T> a) {
         const cx<TI>* in = reinterpret_cast<const cx<TI>*>(a.in) + (size_t)fld * a.ncell + tile0;
         if (j < NR0) {
 #pragma unroll
-            for (int r = 0; r < R0; ++r) dst[r] = load_at(in + (size_t)((jw + r * NR0) * a.S), in_lane);
+            for (int r = 0; r < R0; ++r) {
+                // the row offsets stay uniform values even where the compiler keeps them in
+                // vector registers (no scalar registers left), so the loads keep one scalar
+                // resource instead of a loop over the lanes' resources
+                const uint32_t off = __builtin_amdgcn_readfirstlane((jw + r * NR0) * a.S);
+                dst[r] = load_at(in + (size_t)off, in_lane);
+            }
         }
     };
     // out_{Z,oq} = sum_S M_{Z,S} held[S] for every z-class, stored at once.  The pair lists
