@@ -1139,8 +1139,9 @@ std::string describe_launch(const Launch& L) {
                                                "split-step", "strided", "intensity", "groups", "groups",
                                                "groups", "groups", "source-wave"};
     if (L.fe)
-        std::snprintf(buf, sizeof(buf), "fast %d (%s) G%d %s%s%s%s%s%s", L.fe->N, L.fe->radices, L.fe->G,
+        std::snprintf(buf, sizeof(buf), "fast %d (%s) G%d %s%s%s%s%s%s%s", L.fe->N, L.fe->radices, L.fe->G,
                       role_name[L.fe->role], L.fe->nsrc ? (L.fe->nsrc == 2 ? " held2" : " held3") : "",
+                      L.fe->split > 1 ? (" split" + std::to_string(L.fe->split)).c_str() : "",
                       L.fe->no_prefetch ? " np" : "", L.fe->real_bytes == 8 ? " f64" : "", L.fe->io32 ? " io32" : "",
                       L.fe->one_buffer ? " 1buf" : "");
     else

@@ -719,12 +719,157 @@ __device__ __forceinline__ void stage_compute_tw(cx<T> (&v)[SZ::RMAX], int j, co
     dft<R>(v);
 }
 
+// A last stage of prime radix R shared by P lanes per butterfly (the last stage of a line
+// with few butterflies leaves most lanes idle: 68 = 4 x 17 has four radix-17 butterflies on
+// seventeen lanes).  With g a primitive root mod R, the P cosets of the subgroup <g^P> split
+// the outputs 1..R-1; lane q of a butterfly computes outputs g^q t0 and -g^q t0 for the
+// representatives t0 of that subgroup (up to sign), so that every lane runs the same
+// constants: X[g^q t0] = sum_s x[g^-q s] w^(t0 s), i.e. the lane reads its inputs in the
+// order r = g^-q s mod R and forms the standard symmetric-pair sums of X[t0].  Lane 0 also
+// forms X[0].  (R - 1) / P must be even, so that the subgroup holds -1.
+template <int R, int P>
+struct SplitOdd {
+    static constexpr int H = (R - 1) / 2;
+    static constexpr int PPL = (R - 1) / (2 * P);  // output pairs a lane
+    static constexpr int SLOTS = 2 * PPL + 1;      // X[t0], X[-t0] per pair, then X[0] (lane 0)
+    static constexpr int pw(int b, int e) { return e == 0 ? 1 : (pw(b, e - 1) * b) % R; }
+    static constexpr bool primitive(int g) {
+        for (int e = 1; e < R - 1; ++e)
+            if (pw(g, e) == 1) return false;
+        return true;
+    }
+    static constexpr int root() {
+        for (int g = 2; g < R; ++g)
+            if (primitive(g)) return g;
+        return 0;
+    }
+    static constexpr int G = root();
+    static constexpr int GINV = pw(G, R - 2);
+    // the subgroup's representatives up to sign (the members at most H)
+    static constexpr int t0(int i) {
+        int n = 0;
+        for (int e = 0; e < (R - 1) / P; ++e) {
+            const int m = pw(G, P * e);
+            if (m <= H) {
+                if (n == i) return m;
+                ++n;
+            }
+        }
+        return 0;
+    }
+    static_assert(R >= 5 && (R - 1) % (2 * P) == 0, "the subgroup must hold -1");
+};
+
+// Slot i of lane q (q < P): its output index t in 0..R-1 (slot 2k: g^q t0(k), 2k+1: -g^q t0(k),
+// 2 PPL: 0); -1 where the lane has no such output (X[0] on lanes q > 0).
+template <int R, int P>
+__device__ __forceinline__ int split_slot_index(int q, int i) {
+    using SO = SplitOdd<R, P>;
+    int gq = 1;
+#pragma unroll
+    for (int e = 0; e + 1 < P; ++e) gq = e < q ? (gq * SO::G) % R : gq;
+    if (i == 2 * SO::PPL) return q == 0 ? 0 : -1;
+    const int t = (gq * SO::t0(i >> 1)) % R;
+    return (i & 1) ? R - t : t;
+}
+
+// Slots 2K, 2K + 1 (X[t0], X[-t0] with t0 = t0(K), a compile-time constant) and the pairs after
+// them, from the symmetric sums av, bv of the lane's inputs v.
+template <typename T, int R, int P, int K>
+__device__ __forceinline__ void split_pairs(cx<T>* v, const cx<T>* av, const cx<T>* bv, std::integral_constant<int, K>) {
+    using SO = SplitOdd<R, P>;
+    if constexpr (K < SO::PPL) {
+        constexpr int H = SO::H, t = SO::t0(K);
+        constexpr bool f32 = std::is_same<T, float>::value;
+        cx<T> A = v[0];
+        cx<T> B = {(T)0, (T)0};
+#pragma unroll
+        for (int s = 1; s <= H; ++s) {
+            const int m = (t * s) % R;
+            const int mm = m <= H ? m : R - m;
+            const T c = f32 ? (T)kOddRoots<R>.c[mm] : (T)kOddRoots64<R>.c[mm];
+            const T s1 = f32 ? (T)kOddRoots<R>.s[mm] : (T)kOddRoots64<R>.s[mm];
+            const T sn = m <= H ? s1 : -s1;
+            A.x += av[s].x * c;
+            A.y += av[s].y * c;
+            B.x += bv[s].x * sn;
+            B.y += bv[s].y * sn;
+        }
+        // v[0] is read by every pair: the later pairs are formed before this one is stored
+        split_pairs<T, R, P>(v, av, bv, std::integral_constant<int, K + 1>{});
+        v[2 * K] = {A.x + B.y, A.y - B.x};
+        v[2 * K + 1] = {A.x - B.y, A.y + B.x};
+    }
+}
+
+// The last stage of a line, split over P lanes per butterfly: lane (b, q) = (j % NRL, j / NRL)
+// reads the R inputs of butterfly b in its order, twiddles them and leaves its slots in v.
+template <typename T, class SZ, int P>
+__device__ __forceinline__ void last_stage_split(const cx<T>* line, cx<T> (&v)[SZ::RMAX], int b, int q,
+                                                 const cx<T>* tw) {
+    constexpr int L = SZ::NST - 1, R = SZ::rad(L), NR = SZ::nr(L), NS = SZ::ns(L);
+    constexpr int OFF = StageTw<SZ>::off(L);
+    using SO = SplitOdd<R, P>;
+    int gi = 1;  // g^-q mod R
+#pragma unroll
+    for (int e = 0; e + 1 < P; ++e) gi = e < q ? (gi * SO::GINV) % R : gi;
+    v[0] = line[lds_addr<false, SZ, 0>(0, b)];
+    int r = 0;
+#pragma unroll
+    for (int s = 1; s < R; ++s) {
+        r += gi;
+        r = r >= R ? r - R : r;  // r = g^-q s mod R
+        const cx<T> x = line[lds_addr<false, SZ, 0>(0, b + r * NR)];
+        v[s] = NS > 1 ? cmul(x, tw[OFF + (r - 1) * NS + b]) : x;
+    }
+    constexpr int H = SO::H;
+    cx<T> av[H + 1], bv[H + 1];
+    cx<T> y0 = v[0];
+#pragma unroll
+    for (int s = 1; s <= H; ++s) {
+        av[s] = v[s] + v[R - s];
+        bv[s] = v[s] - v[R - s];
+        y0 = y0 + av[s];
+    }
+    split_pairs<T, R, P>(v, av, bv, std::integral_constant<int, 0>{});
+    v[2 * SO::PPL] = y0;
+}
+
+// Outputs a lane holds after a line transform, at the row's point index
+// out_index(j, i) for slot i < out_slots (i.e. X[j + i N/RL] when the last stage is not split).
+template <class SZ, int P>
+struct LineOut {
+    static constexpr int L = SZ::NST - 1;
+    static constexpr int NRL = SZ::NRL;
+    static constexpr int slots() {
+        if constexpr (P > 1)
+            return SplitOdd<SZ::rad(L), P>::SLOTS;
+        else
+            return SZ::RL;
+    }
+    static constexpr int SLOTS = slots();
+    // the lane holds outputs at all
+    __device__ static bool owns(int j) { return j < P * NRL; }
+    // point index of slot i, or -1
+    __device__ static int index(int j, int i) {
+        if constexpr (P > 1) {
+            const int t = split_slot_index<SZ::rad(L), P>(j / NRL, i);
+            return t < 0 ? -1 : j % NRL + t * NRL;
+        } else {
+            return j + i * NRL;
+        }
+    }
+};
+
 // All stages of one line: the stage-0 inputs x[j + r N/R0] are in v on entry, the outputs
 // X[j + r N/RL] on return.  act: the lane owns a line (inactive lanes touch no memory).
-template <typename T, class SZ>
+// P > 1: the last stage is split over P lanes per butterfly (last_stage_split) and the lane's
+// outputs are those of LineOut<SZ, P>.
+template <typename T, class SZ, int P = 1>
 __device__ __forceinline__ void wave_line_transform(cx<T>* line, cx<T> (&v)[SZ::RMAX], int j, bool act,
                                                     const cx<T>* tw) {
     constexpr int NST = SZ::NST;
+    static_assert(P == 1 || NST == 2, "split last stages are built for two-stage lines");
     wave_sync();  // the previous line's reads come before this line's writes
     const bool a0 = act && j < SZ::nr(0);
     if (a0) stage_compute_tw<T, SZ, 0>(v, j, tw);
@@ -735,6 +880,10 @@ __device__ __forceinline__ void wave_line_transform(cx<T>* line, cx<T> (&v)[SZ::
             for (int r = 0; r < R; ++r) line[lds_addr<false, SZ, 0>(0, j * R + r)] = v[r];
         }
         wave_sync();
+        if constexpr (P > 1) {
+            if (act && j < P * SZ::nr(1)) last_stage_split<T, SZ, P>(line, v, j % SZ::nr(1), j / SZ::nr(1), tw);
+            return;
+        }
         const bool a1 = act && j < SZ::nr(1);
         if (a1) {
             constexpr int R = SZ::rad(1), NR = SZ::nr(1), NS = SZ::ns(1);
@@ -784,14 +933,15 @@ __device__ __forceinline__ void wave_line_transform(cx<T>* line, cx<T> (&v)[SZ::
 // IO32 (FP64 only): FP64 transforms on FP32 P, weights and outputs; the source is formed as in
 // the FP32 kernel (FP64 phase reduction, FP32 values, held in FP32 registers) and each field
 // is widened just before its transform, which keeps two waves per SIMD.
-template <typename T, class SZ, int ROWS, bool IO32 = false>
+// P > 1: the last stage (prime radix) is split over P lanes per butterfly (LineOut).
+template <typename T, class SZ, int ROWS, bool IO32 = false, int P = 1>
 __global__ __launch_bounds__(64 * kSrcWaves) __attribute__((amdgpu_waves_per_eu(
     sizeof(T) == 8 ? (IO32 ? 2 : 1) : (SZ::RMAX >= 11 ? 2 : 3)))) void fast_source_wave(PassArgs<T> a) {
     constexpr bool F32 = std::is_same<T, float>::value;
     constexpr bool SRC32 = F32 || IO32;  // the source formed in FP32 (FP64 phase reduction)
     using TS = io_t<T, IO32>;
     using TT = std::conditional_t<SRC32, float, T>;  // atom-table entries
-    constexpr int N = SZ::N, TPL = SZ::TPL, R0 = SZ::rad(0), NR0 = SZ::nr(0), RL = SZ::RL, NRL = SZ::NRL;
+    constexpr int N = SZ::N, TPL = SZ::TPL, R0 = SZ::rad(0), NR0 = SZ::nr(0);
     constexpr int NTHR = 64 * kSrcWaves;
     constexpr int NTW = StageTw<SZ>::total();
     static_assert(ROWS >= 1 && ROWS * TPL <= 64, "a wave holds whole rows");
@@ -827,63 +977,76 @@ __global__ __launch_bounds__(64 * kSrcWaves) __attribute__((amdgpu_waves_per_eu(
     const uint32_t cell0 = row * (uint32_t)N;
 
     const bool has0 = live && j < NR0;  // the lane owns stage-0 inputs
-    const T* kj = a.k + j;
+    // the lane's outputs: slot i of a line transform is point out_at(i) of the row (< 0: none)
+    using LO = LineOut<SZ, P>;
+    constexpr int NOS = LO::SLOTS;
+    const bool has_out = live && LO::owns(j);
+    int out_at[NOS];
+#pragma unroll
+    for (int i = 0; i < NOS; ++i) out_at[i] = LO::index(j, i);
     // fractional phases: transform group grp's four fields held in v (F[E] waits in
     // registers for F[U_d0]) and store A, B, C
     auto emit = [&](cx<T>(&v)[4][SZ::RMAX], int grp) {
-        cx<TS>* oA = reinterpret_cast<cx<TS>*>(a.out) + (size_t)grp * 3 * a.ncell + (size_t)cell0 + (size_t)j;
+        cx<TS>* oA = reinterpret_cast<cx<TS>*>(a.out) + (size_t)grp * 3 * a.ncell + (size_t)cell0;
         cx<TS>* oB = oA + a.ncell;
         cx<TS>* oC = oB + a.ncell;
-        wave_line_transform<T, SZ>(line, v[0], j, inrow, tw);
-        wave_line_transform<T, SZ>(line, v[1], j, inrow, tw);
-        if (live && j < NRL) {
+        wave_line_transform<T, SZ, P>(line, v[0], j, inrow, tw);
+        wave_line_transform<T, SZ, P>(line, v[1], j, inrow, tw);
+        if (has_out) {
 #pragma unroll
-            for (int r = 0; r < RL; ++r) oA[r * NRL] = cx_as<TS>(v[0][r] + cscale(negi(v[1][r]), kj[r * NRL]));
+            for (int i = 0; i < NOS; ++i)
+                if (P == 1 || out_at[i] >= 0) oA[out_at[i]] = cx_as<TS>(v[0][i] + cscale(negi(v[1][i]), a.k[out_at[i]]));
         }
-        wave_line_transform<T, SZ>(line, v[2], j, inrow, tw);
-        if (live && j < NRL) {
+        wave_line_transform<T, SZ, P>(line, v[2], j, inrow, tw);
+        if (has_out) {
 #pragma unroll
-            for (int r = 0; r < RL; ++r) oB[r * NRL] = cx_as<TS>(v[2][r]);
+            for (int i = 0; i < NOS; ++i)
+                if (P == 1 || out_at[i] >= 0) oB[out_at[i]] = cx_as<TS>(v[2][i]);
         }
-        wave_line_transform<T, SZ>(line, v[3], j, inrow, tw);
-        if (live && j < NRL) {
+        wave_line_transform<T, SZ, P>(line, v[3], j, inrow, tw);
+        if (has_out) {
 #pragma unroll
-            for (int r = 0; r < RL; ++r) oC[r * NRL] = cx_as<TS>(v[3][r]);
+            for (int i = 0; i < NOS; ++i)
+                if (P == 1 || out_at[i] >= 0) oC[out_at[i]] = cx_as<TS>(v[3][i]);
         }
     };
     // one-hot phases: a group's fields are e and e u_d; each field e u_d is formed just
     // before its transform (in T from the held e and u, FP32 for an FP32 source), and
     // F[e u_0] waits for F[e], which is transformed last
     auto emit_eu = [&](auto& e, auto& u, int grp, uint32_t lane_cell) {
-        cx<TS>* oA = reinterpret_cast<cx<TS>*>(a.out) + (size_t)grp * 3 * a.ncell + (size_t)lane_cell;
+        // lane_cell - j is the row's first cell
+        cx<TS>* oA = reinterpret_cast<cx<TS>*>(a.out) + (size_t)grp * 3 * a.ncell + (size_t)(lane_cell - (uint32_t)j);
         cx<TS>* oB = oA + a.ncell;
         cx<TS>* oC = oB + a.ncell;
-        cx<T> w[SZ::RMAX], f0[RL];
+        cx<T> w[SZ::RMAX], f0[NOS];
 #pragma unroll
         for (int r = 0; r < R0; ++r) w[r] = cscale(cx_as<T>(e[r]), (T)u[0][r]);
-        wave_line_transform<T, SZ>(line, w, j, inrow, tw);
+        wave_line_transform<T, SZ, P>(line, w, j, inrow, tw);
 #pragma unroll
-        for (int r = 0; r < RL; ++r) f0[r] = w[r];
+        for (int i = 0; i < NOS; ++i) f0[i] = w[i];
 #pragma unroll
         for (int r = 0; r < R0; ++r) w[r] = cscale(cx_as<T>(e[r]), (T)u[1][r]);
-        wave_line_transform<T, SZ>(line, w, j, inrow, tw);
-        if (live && j < NRL) {
+        wave_line_transform<T, SZ, P>(line, w, j, inrow, tw);
+        if (has_out) {
 #pragma unroll
-            for (int r = 0; r < RL; ++r) oB[r * NRL] = cx_as<TS>(w[r]);
+            for (int i = 0; i < NOS; ++i)
+                if (P == 1 || out_at[i] >= 0) oB[out_at[i]] = cx_as<TS>(w[i]);
         }
 #pragma unroll
         for (int r = 0; r < R0; ++r) w[r] = cscale(cx_as<T>(e[r]), (T)u[2][r]);
-        wave_line_transform<T, SZ>(line, w, j, inrow, tw);
-        if (live && j < NRL) {
+        wave_line_transform<T, SZ, P>(line, w, j, inrow, tw);
+        if (has_out) {
 #pragma unroll
-            for (int r = 0; r < RL; ++r) oC[r * NRL] = cx_as<TS>(w[r]);
+            for (int i = 0; i < NOS; ++i)
+                if (P == 1 || out_at[i] >= 0) oC[out_at[i]] = cx_as<TS>(w[i]);
         }
 #pragma unroll
         for (int r = 0; r < R0; ++r) w[r] = cx_as<T>(e[r]);
-        wave_line_transform<T, SZ>(line, w, j, inrow, tw);
-        if (live && j < NRL) {
+        wave_line_transform<T, SZ, P>(line, w, j, inrow, tw);
+        if (has_out) {
 #pragma unroll
-            for (int r = 0; r < RL; ++r) oA[r * NRL] = cx_as<TS>(w[r] + cscale(negi(f0[r]), kj[r * NRL]));
+            for (int i = 0; i < NOS; ++i)
+                if (P == 1 || out_at[i] >= 0) oA[out_at[i]] = cx_as<TS>(w[i] + cscale(negi(f0[i]), a.k[out_at[i]]));
         }
     };
     if constexpr (!SRC32) {

@@ -166,14 +166,52 @@ void add_held(std::vector<FastEntry>& v, const char* radices) {
     }
 }
 
+// The largest P <= TPL / NRL for which the last stage of a two-stage length splits over P
+// lanes per butterfly (a prime last radix R with (R - 1) divisible by 2 P), else 1.
+template <class SZ>
+constexpr int split_lanes() {
+    constexpr int R = SZ::RL;
+    bool prime = R >= 5;
+    for (int d = 2; d * d <= R; ++d) prime = prime && R % d != 0;
+    if (SZ::NST != 2 || !prime) return 1;
+    for (int p = SZ::TPL / SZ::NRL; p >= 2; --p)
+        if ((R - 1) % (2 * p) == 0) return p;
+    return 1;
+}
+
+template <class SZ, int ROWS, int P>
+hipError_t srcw_launch(const Launch& L, const PassArgs<float>& a, hipStream_t st) {
+    return launch_one(fast_source_wave<float, SZ, ROWS, false, P>, L, st, a);
+}
+template <class SZ, int ROWS, bool IO32, int P>
+hipError_t srcw_launch64(const Launch& L, const PassArgs<double>& a, hipStream_t st) {
+    return launch_one(fast_source_wave<double, SZ, ROWS, IO32, P>, L, st, a);
+}
+
 // The one-wave-per-row source pass (lengths whose lines fit one wave and whose first
 // radix is at most 8, so that the four fields' stage-0 inputs fit in registers): G is the
-// rows of a block; the LDS holds one line per row and the per-stage twiddle table.
+// rows of a block; the LDS holds one line per row and the per-stage twiddle table.  Lengths
+// whose last stage splits over P lanes (split_lanes) list that form first.
 template <class SZ>
 void add_source_wave(std::vector<FastEntry>& v, const char* radices) {
     if constexpr (SZ::TPL <= 64 && SZ::rad(0) <= 8) {
         constexpr int rows = 64 / SZ::TPL;
         constexpr size_t cells = (size_t)kSrcWaves * rows * SZ::PITCH + StageTw<SZ>::total();
+        constexpr int P = split_lanes<SZ>();
+        if constexpr (P > 1) {
+            FastEntry e32{SZ::N, kRoleSourceW, kSrcWaves * rows, 64 * kSrcWaves, SZ::TPL, cells * 8, radices,
+                          &srcw_launch<SZ, rows, P>, (const void*)fast_source_wave<float, SZ, rows, false, P>, false};
+            FastEntry e64{SZ::N, kRoleSourceW, kSrcWaves * rows, 64 * kSrcWaves, SZ::TPL, cells * 16, radices, nullptr,
+                          (const void*)fast_source_wave<double, SZ, rows, false, P>, false, 8,
+                          &srcw_launch64<SZ, rows, false, P>};
+            FastEntry e64io{SZ::N, kRoleSourceW, kSrcWaves * rows, 64 * kSrcWaves, SZ::TPL, cells * 16, radices,
+                            nullptr, (const void*)fast_source_wave<double, SZ, rows, true, P>, false, 8,
+                            &srcw_launch64<SZ, rows, true, P>, true};
+            for (FastEntry* e : {&e32, &e64, &e64io}) {
+                e->split = P;
+                v.push_back(*e);
+            }
+        }
         v.push_back({SZ::N, kRoleSourceW, kSrcWaves * rows, 64 * kSrcWaves, SZ::TPL, cells * 8, radices,
                      &fast_launch<SZ, kRoleSourceW, kSrcWaves * rows>,
                      (const void*)fast_source_wave<float, SZ, rows>, false});

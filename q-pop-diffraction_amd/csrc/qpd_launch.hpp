@@ -79,6 +79,7 @@ struct FastEntry {
     bool io32 = false;   // FP64 kernel on FP32 arrays (P, weights, spectra): FP32 plans computing in FP64
     bool one_buffer = false;  // strided FP64 kernel with one LDS stage buffer
     int nsrc = 0;             // pass 1 over source groups: the group count it requires (0: any)
+    int split = 1;            // one-wave-per-row source pass: lanes per last-stage butterfly
 };
 
 // Raise a kernel's dynamic LDS limit once per (kernel, device) and size.
