@@ -32,6 +32,15 @@ __device__ __forceinline__ cx<T> negi(cx<T> a) { return {a.y, -a.x}; }
 template <typename T, typename S>
 __device__ __forceinline__ cx<T> cx_as(cx<S> a) { return {(T)a.x, (T)a.y}; }
 
+// Entry i (wave-uniform) of a table that no kernel writes while it runs, read through the
+// constant address space, i.e. as a scalar load.  A table entry read as a vector load is
+// waited for with the vector-memory counter, which also waits for every vector load issued
+// before it (a prefetched input line included); a scalar load waits on its own counter.
+template <typename V>
+__device__ __forceinline__ V uload(const V* p, int i) {
+    return ((const __attribute__((address_space(4))) V*)p)[i];
+}
+
 // Element type of the arrays an FP64 kernel reads and writes: with IO32 it computes in FP64
 // and keeps its complex fields, P, the phase fractions and the region weight in FP32 (an FP32
 // plan whose transforms run in FP64).

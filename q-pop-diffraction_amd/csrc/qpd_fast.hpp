@@ -301,6 +301,25 @@ __device__ __forceinline__ int lds_addr(int line, int idx) {
         return line * SZ::PITCH + idx + (idx >> 3);
 }
 
+// Layout of the per-stage twiddle table: stage s starts at off(s) with ntw(s) twiddles
+// W_N^{r k N / (NS R)} at (r - 1) NS + k.
+template <class SZ>
+struct StageTw {
+    static constexpr int ntw(int s) { return SZ::ns(s) > 1 ? (SZ::rad(s) - 1) * SZ::ns(s) : 0; }
+    static constexpr int off(int s) { return s == 0 ? 0 : off(s - 1) + ntw(s - 1); }
+    static constexpr int total() { return off(SZ::NST); }
+};
+
+template <typename T, class SZ, int S>
+__device__ __forceinline__ void fill_stage_tw(cx<T>* tw, const cx<T>* __restrict__ W, int tid, int nthr) {
+    if constexpr (S < SZ::NST) {
+        constexpr int R = SZ::rad(S), NS = SZ::ns(S);
+        constexpr int OFF = StageTw<SZ>::off(S), NTW = StageTw<SZ>::ntw(S);
+        constexpr int TWS = SZ::N / (NS * R);
+        for (int e = tid; e < NTW; e += nthr) tw[OFF + e] = W[(e / NS + 1) * (e % NS) * TWS];
+    }
+}
+
 // Stage s on F lines held in registers v[f][r] (inputs of butterfly j already loaded):
 // twiddle (shared by the F lines), then DFT_R.
 template <typename T, class SZ, int S, int F>
@@ -308,11 +327,11 @@ __device__ __forceinline__ void stage_compute(cx<T> (&v)[F][SZ::RMAX], int j, co
     constexpr int R = SZ::rad(S);
     constexpr int NS = SZ::ns(S);
     if constexpr (NS > 1) {
-        constexpr int TWS = SZ::N / (NS * R);
+        constexpr int OFF = StageTw<SZ>::off(S);
         const int k = j % NS;
         cx<T> w[R];
 #pragma unroll
-        for (int r = 1; r < R; ++r) w[r] = W[r * k * TWS];
+        for (int r = 1; r < R; ++r) w[r] = W[OFF + (r - 1) * NS + k];
 #pragma unroll
         for (int f = 0; f < F; ++f)
 #pragma unroll
@@ -458,10 +477,16 @@ __device__ __forceinline__ void transform_packed(cx<T>* X, cx<T>* Y, cx<T> (&v)[
     }
 }
 
-// Copy the twiddle table W[0, N) into LDS (caller synchronises before use).
-template <typename T, int N>
+// Fill the per-stage twiddle table (StageTw layout, N - R0 entries at most) in LDS from the
+// table W[m] = exp(-2 pi i m / N) (caller synchronises before use).  Entry (r - 1) NS + k of
+// stage s holds the twiddle of input r of butterfly k, so the lanes of a wave, which take
+// consecutive k, read consecutive entries (W[r k N / (NS R)] itself is strided by r N / (NS R)
+// entries between them, up to 16-way LDS bank conflicts for complex FP64 values).
+template <typename T, class SZ>
 __device__ __forceinline__ void load_twiddles(cx<T>* dst, const cx<T>* __restrict__ src, int tid, int nthr) {
-    for (int m = tid; m < N; m += nthr) dst[m] = src[m];
+    fill_stage_tw<T, SZ, 1>(dst, src, tid, nthr);
+    fill_stage_tw<T, SZ, 2>(dst, src, tid, nthr);
+    fill_stage_tw<T, SZ, 3>(dst, src, tid, nthr);
 }
 
 // Multiply the last-stage outputs by the chunk coefficient and add them to the
@@ -505,7 +530,7 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_contig(PassArgs<T> a) {
     const uint32_t row = blockIdx.x * G + g;
     const bool row_ok = row < nrows;
     const uint32_t cell0 = row * (uint32_t)N;
-    load_twiddles<T, N>(Wl, a.W, tid, G * TPL);
+    load_twiddles<T, SZ>(Wl, a.W, tid, G * TPL);
     lds_barrier();
     int par = 0;
     for (int grp = 0; grp < a.ngroups; ++grp) {
@@ -514,9 +539,9 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_contig(PassArgs<T> a) {
         for (int o = 0; o < NOUT; ++o)
 #pragma unroll
             for (int r = 0; r < SZ::RL; ++r) acc[o][r] = {(T)0, (T)0};
-        for (int c = a.group_first[grp]; c < a.group_first[grp + 1]; ++c) {
-            const int src = a.chunk_src[c];
-            const int ct = a.chunk_coef[c];
+        for (int c = uload(a.group_first, grp); c < uload(a.group_first, grp + 1); ++c) {
+            const int src = uload(a.chunk_src, c);
+            const int ct = uload(a.chunk_coef, c);
             cx<T> v[F][SZ::RMAX];
             if (j < NR0) {
                 const cx<T>* in = a.in + (size_t)src * a.nin * a.ncell + cell0 + j;
@@ -573,7 +598,7 @@ __global__ __launch_bounds__(4 * G * SZ::TPL) void fast_source(PassArgs<T> a) {
     const uint32_t row0 = blockIdx.x * G;
     const uint32_t cell0 = row0 * (uint32_t)N;
     const uint32_t row = row0 + g;
-    load_twiddles<T, N>(Wl, a.W, tid, NTHR);
+    load_twiddles<T, SZ>(Wl, a.W, tid, NTHR);
     const int ntab = a.n_atom_tab;
     for (int m = tid; m < ntab; m += NTHR) tabl[m] = (TT)a.atomtab[m];
 
@@ -601,9 +626,9 @@ __global__ __launch_bounds__(4 * G * SZ::TPL) void fast_source(PassArgs<T> a) {
         cx<T> acc[RL];
 #pragma unroll
         for (int r = 0; r < RL; ++r) acc[r] = {(T)0, (T)0};
-        for (int c = a.group_first[grp]; c < a.group_first[grp + 1]; ++c) {
-            const int src = a.chunk_src[c];
-            const int ct = a.chunk_coef[c];
+        for (int c = uload(a.group_first, grp); c < uload(a.group_first, grp + 1); ++c) {
+            const int src = uload(a.chunk_src, c);
+            const int ct = uload(a.chunk_coef, c);
             cx<T>* X = par ? bufB : bufA;
             cx<T>* Y = par ? bufA : bufB;
             if (ALT) par ^= 1;
@@ -685,25 +710,6 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// Layout of the per-stage twiddle table: stage s starts at off(s) with ntw(s) twiddles
-// W_N^{r k N / (NS R)} at (r - 1) NS + k.
-template <class SZ>
-struct StageTw {
-    static constexpr int ntw(int s) { return SZ::ns(s) > 1 ? (SZ::rad(s) - 1) * SZ::ns(s) : 0; }
-    static constexpr int off(int s) { return s == 0 ? 0 : off(s - 1) + ntw(s - 1); }
-    static constexpr int total() { return off(SZ::NST); }
-};
-
-template <typename T, class SZ, int S>
-__device__ __forceinline__ void fill_stage_tw(cx<T>* tw, const cx<T>* __restrict__ W, int tid, int nthr) {
-    if constexpr (S < SZ::NST) {
-        constexpr int R = SZ::rad(S), NS = SZ::ns(S);
-        constexpr int OFF = StageTw<SZ>::off(S), NTW = StageTw<SZ>::ntw(S);
-        constexpr int TWS = SZ::N / (NS * R);
-        for (int e = tid; e < NTW; e += nthr) tw[OFF + e] = W[(e / NS + 1) * (e % NS) * TWS];
-    }
 }
 
 // Stage S of one line held in v (the same arithmetic as stage_compute).
@@ -1054,7 +1060,7 @@ __global__ __launch_bounds__(64 * kSrcWaves) __attribute__((amdgpu_waves_per_eu(
         // (5 values a cell) of a pair of groups per read of the cells, fractional ones the
         // four fields of one group
         for (int grp = 0; grp < a.ngroups; ++grp) {
-            const int src = a.chunk_src[a.group_first[grp]];
+            const int src = uload(a.chunk_src, uload(a.group_first, grp));
             // at most kAhead cells' loads in flight: cell r + kAhead is requested once cell r
             // has been formed (the scheduling barriers keep the compiler from hoisting them all)
             constexpr int kAhead = R0 < 2 ? R0 : 2;
@@ -1064,7 +1070,7 @@ __global__ __launch_bounds__(64 * kSrcWaves) __attribute__((amdgpu_waves_per_eu(
             if (!a.fraction) {
                 // a pair of groups from one read of the cells
                 const bool pair = grp + 1 < a.ngroups;
-                const int srcB = pair ? a.chunk_src[a.group_first[grp + 1]] : src;
+                const int srcB = pair ? uload(a.chunk_src, uload(a.group_first, grp + 1)) : src;
                 cx<T> eA[SZ::RMAX], eB[SZ::RMAX];
                 T uA[3][R0], uB[3][R0];
 #pragma unroll
@@ -1105,8 +1111,8 @@ __global__ __launch_bounds__(64 * kSrcWaves) __attribute__((amdgpu_waves_per_eu(
         const PassArgs<float> af = source_view(a);
         for (int grp = 0; grp < a.ngroups; grp += 2) {
             const bool pair = grp + 1 < a.ngroups;
-            const int srcA = a.chunk_src[a.group_first[grp]];
-            const int srcB = pair ? a.chunk_src[a.group_first[grp + 1]] : srcA;
+            const int srcA = uload(a.chunk_src, uload(a.group_first, grp));
+            const int srcB = pair ? uload(a.chunk_src, uload(a.group_first, grp + 1)) : srcA;
             // the lane's first cell is re-formed in every iteration (the empty asm hides its
             // value), so only this 32-bit offset, not the 64-bit cell and output pointers
             // made from it, is held across the transforms
@@ -1139,7 +1145,7 @@ __global__ __launch_bounds__(64 * kSrcWaves) __attribute__((amdgpu_waves_per_eu(
         const PassArgs<float> af = source_view(a);
         const CellPtrsF cp = cell_ptrs_f(af, (size_t)cell0 + (size_t)j);
         for (int grp = 0; grp < a.ngroups; ++grp) {
-            const int src = a.chunk_src[a.group_first[grp]];
+            const int src = uload(a.chunk_src, uload(a.group_first, grp));
             cx<T> v[4][SZ::RMAX];
             RawCellF raw[R0];
 #pragma unroll
@@ -1208,20 +1214,22 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided(PassArgs<T> a) {
     const size_t tile0 = (size_t)t0 * G + (size_t)outer * a.OS;
     const uint32_t in_lane = ((col_ok ? i0 : a.n0 - 1) - t0 * G + jl * in_step * a.S) * (uint32_t)sizeof(cx<TI>);
     const uint32_t c_lane = jl * c_step;
-    load_twiddles<T, N>(Wl, a.W, tid, G * SZ::TPL);
+    load_twiddles<T, SZ>(Wl, a.W, tid, G * SZ::TPL);
     lds_barrier();
     // inputs are consumed in the order (chunk, input)
-    const int nitems = a.group_first[a.ngroups] * NIN;
-    auto load_item = [&](int it, cx<T>(&dst)[1][SZ::RMAX]) {
+    const int nitems = uload(a.group_first, a.ngroups) * NIN;
+    // the prefetched line stays in the arrays' own type until it is used (a conversion at the
+    // load would wait for it there)
+    auto load_item = [&](int it, cx<TI>(&dst)[SZ::RMAX]) {
         const int c = it / NIN, f = it - (it / NIN) * NIN;
-        const cx<TI>* in = reinterpret_cast<const cx<TI>*>(a.in) + ((size_t)a.chunk_src[c] * a.nin + f) * a.ncell + tile0;
+        const cx<TI>* in = reinterpret_cast<const cx<TI>*>(a.in) + ((size_t)uload(a.chunk_src, c) * a.nin + f) * a.ncell + tile0;
         if (j < NR0) {
 #pragma unroll
             for (int r = 0; r < R0; ++r)
-                dst[0][r] = cx_as<T>(load_at(in + (size_t)((in0 + (jw + r * NR0) * in_step) * a.S), in_lane));
+                dst[r] = load_at(in + (size_t)((in0 + (jw + r * NR0) * in_step) * a.S), in_lane);
         }
     };
-    cx<T> nxt[1][SZ::RMAX];
+    cx<TI> nxt[SZ::RMAX];
     if constexpr (PF) load_item(0, nxt);
     int it = 0;
     int par = 0;
@@ -1231,18 +1239,24 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided(PassArgs<T> a) {
         for (int o = 0; o < NOUT; ++o)
 #pragma unroll
             for (int r = 0; r < SZ::RL; ++r) acc[o][r] = {(T)0, (T)0};
-        for (int c = a.group_first[grp]; c < a.group_first[grp + 1]; ++c) {
-            const int ct = a.chunk_coef[c];
+        for (int c = uload(a.group_first, grp); c < uload(a.group_first, grp + 1); ++c) {
+            const int ct = uload(a.chunk_coef, c);
             const cx<T>* coef = a.coef + (ct >= 0 ? (size_t)ct * a.Nc : 0);
 #pragma unroll
             for (int f = 0; f < NIN; ++f, ++it) {
                 cx<T> v[1][SZ::RMAX];
                 if constexpr (PF) {
 #pragma unroll
-                    for (int r = 0; r < SZ::RMAX; ++r) v[0][r] = nxt[0][r];
+                    for (int r = 0; r < SZ::RMAX; ++r) v[0][r] = cx_as<T>(nxt[r]);
                     if (it + 1 < nitems) load_item(it + 1, nxt);
                 } else {
-                    load_item(it, v);
+                    const cx<TI>* in =
+                        reinterpret_cast<const cx<TI>*>(a.in) + ((size_t)uload(a.chunk_src, c) * a.nin + f) * a.ncell + tile0;
+                    if (j < NR0) {
+#pragma unroll
+                        for (int r = 0; r < R0; ++r)
+                            v[0][r] = cx_as<T>(load_at(in + (size_t)((in0 + (jw + r * NR0) * in_step) * a.S), in_lane));
+                    }
                 }
                 transform<T, SZ, 1, true, LP, false, ONEBUF>(par ? bufB : bufA, par ? bufA : bufB, v, j, g, 0, Wl);
                 if (ALT) par ^= 1;
@@ -1333,16 +1347,15 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided_mix(PassArgs<T> a) {
     const size_t tile0 = (size_t)t0 * G + (size_t)outer * a.OS;
     const uint32_t in_lane = ((col_ok ? i0 : a.n0 - 1) - t0 * G + jl * in_step * a.S) * (uint32_t)sizeof(cx<TI>);
     const uint32_t c_lane = jl * c_step;
-    load_twiddles<T, N>(Wl, a.W, tid, G * SZ::TPL);
+    load_twiddles<T, SZ>(Wl, a.W, tid, G * SZ::TPL);
     lds_barrier();
     constexpr bool PF = true;
     const int nitems = 3 * a.nsrc;
-    auto load_item = [&](int it, cx<T>(&dst)[1][SZ::RMAX]) {
+    auto load_item = [&](int it, cx<TI>(&dst)[SZ::RMAX]) {
         const cx<TI>* in = reinterpret_cast<const cx<TI>*>(a.in) + (size_t)it * a.ncell + tile0;
         if (j < NR0) {
 #pragma unroll
-            for (int r = 0; r < R0; ++r)
-                dst[0][r] = cx_as<T>(load_at(in + (size_t)((in0 + (jw + r * NR0) * in_step) * a.S), in_lane));
+            for (int r = 0; r < R0; ++r) dst[r] = load_at(in + (size_t)((in0 + (jw + r * NR0) * in_step) * a.S), in_lane);
         }
     };
     cx<T> acc[NZ][2][RL];
@@ -1350,7 +1363,7 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided_mix(PassArgs<T> a) {
     for (int z = 0; z < NZ; ++z)
 #pragma unroll
         for (int r = 0; r < RL; ++r) acc[z][0][r] = acc[z][1][r] = {(T)0, (T)0};
-    cx<T> nxt[1][SZ::RMAX];
+    cx<TI> nxt[SZ::RMAX];
     if constexpr (PF) load_item(0, nxt);
     int it = 0;
     int par = 0;
@@ -1361,10 +1374,13 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided_mix(PassArgs<T> a) {
             cx<T> v[1][SZ::RMAX];
             if constexpr (PF) {
 #pragma unroll
-                for (int r = 0; r < SZ::RMAX; ++r) v[0][r] = nxt[0][r];
+                for (int r = 0; r < SZ::RMAX; ++r) v[0][r] = cx_as<T>(nxt[r]);
                 if (it + 1 < nitems) load_item(it + 1, nxt);
             } else {
-                load_item(it, v);
+                cx<TI> cur[SZ::RMAX];
+                load_item(it, cur);
+#pragma unroll
+                for (int r = 0; r < SZ::RMAX; ++r) v[0][r] = cx_as<T>(cur[r]);
             }
             transform<T, SZ, 1, true, LP, false, ONEBUF>(par ? bufB : bufA, par ? bufA : bufB, v, j, g, 0, Wl);
             if (ALT) par ^= 1;
@@ -1382,9 +1398,9 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided_mix(PassArgs<T> a) {
         // registers); only the lanes of stored points accumulate
 #pragma unroll
         for (int z = 0; z < NZ; ++z) {
-            const int p1 = __builtin_amdgcn_readfirstlane(a.pair_first[s * NZ + z + 1]);
-            for (int p = __builtin_amdgcn_readfirstlane(a.pair_first[s * NZ + z]); p < p1; ++p) {
-                const int y = __builtin_amdgcn_readfirstlane(a.pair_y[p]);
+            const int p1 = uload(a.pair_first, s * NZ + z + 1);
+            for (int p = uload(a.pair_first, s * NZ + z); p < p1; ++p) {
+                const int y = uload(a.pair_y, p);
                 const cx<T>* xrow = a.mixc + (size_t)(y * a.nsrc + s) * a.n0 + t0 * G;
                 const cx<T>* cy = a.coef + (size_t)y * a.Nc + c0;
                 if (j < NRL && col_ok) {
@@ -1451,7 +1467,7 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_mix_held(PassArgs<T> a) {
     const uint32_t out_lane = (g + jl * a.S) * (uint32_t)sizeof(cx<TI>);
     const uint32_t coff = jl * (uint32_t)sizeof(cx<T>);
     const uint32_t koff = jl * (uint32_t)sizeof(T);
-    load_twiddles<T, N>(Wl, a.W, tid, G * SZ::TPL);
+    load_twiddles<T, SZ>(Wl, a.W, tid, G * SZ::TPL);
     lds_barrier();
     // item it -> input field: A_s, B_s for s = it / 2 while it < 2 NS, then C_{it - 2 NS}
     auto load_item = [&](int it, cx<TI>(&dst)[SZ::RMAX]) {
@@ -1478,9 +1494,9 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_mix_held(PassArgs<T> a) {
             for (int r = 0; r < RL; ++r) acc[r] = {(T)0, (T)0};
 #pragma unroll
             for (int s = 0; s < NS; ++s) {
-                const int p1 = __builtin_amdgcn_readfirstlane(a.pair_first[s * NZ + z + 1]);
-                for (int p = __builtin_amdgcn_readfirstlane(a.pair_first[s * NZ + z]); p < p1; ++p) {
-                    const int y = __builtin_amdgcn_readfirstlane(a.pair_y[p]);
+                const int p1 = uload(a.pair_first, s * NZ + z + 1);
+                for (int p = uload(a.pair_first, s * NZ + z); p < p1; ++p) {
+                    const int y = uload(a.pair_y, p);
                     const cx<T>* xrow = a.mixc + (size_t)(y * NS + s) * a.n0 + t0 * G;
                     const cx<T>* cy = a.coef + (size_t)y * a.Nc;
                     if (j < NRL && col_ok) {
@@ -1562,28 +1578,31 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_split(PassArgs<T> a) {
     const uint32_t jl = (uint32_t)(tid % 64) / G;
     const size_t tile0 = (size_t)t0 * G + (size_t)outer * a.OS + (size_t)q * a.in_qmul * a.S;
     const uint32_t in_lane = ((col_ok ? i0 : a.n0 - 1) - t0 * G + jl * rstride) * (uint32_t)sizeof(cx<TI>);
-    load_twiddles<T, N>(Wl, a.W, tid, G * SZ::TPL);
+    load_twiddles<T, SZ>(Wl, a.W, tid, G * SZ::TPL);
     lds_barrier();
     constexpr bool PF = true;
     const cx<TI>* src = IO32 ? reinterpret_cast<const cx<TI>*>(a.in) : reinterpret_cast<const cx<TI>*>(a.out);
-    auto load_field = [&](int f, cx<T>(&dst)[1][SZ::RMAX]) {
+    auto load_field = [&](int f, cx<TI>(&dst)[SZ::RMAX]) {
         const cx<TI>* in = src + (size_t)f * a.ncell + tile0;
         if (j < NR0) {
 #pragma unroll
-            for (int r = 0; r < R0; ++r) dst[0][r] = cx_as<T>(load_at(in + (size_t)(jw + r * NR0) * rstride, in_lane));
+            for (int r = 0; r < R0; ++r) dst[r] = load_at(in + (size_t)(jw + r * NR0) * rstride, in_lane);
         }
     };
-    cx<T> nxt[1][SZ::RMAX];
+    cx<TI> nxt[SZ::RMAX];
     if constexpr (PF) load_field(0, nxt);
     int par = 0;
     for (int f = 0; f < a.nfields; ++f) {
         cx<T> v[1][SZ::RMAX];
         if constexpr (PF) {
 #pragma unroll
-            for (int r = 0; r < SZ::RMAX; ++r) v[0][r] = nxt[0][r];
+            for (int r = 0; r < SZ::RMAX; ++r) v[0][r] = cx_as<T>(nxt[r]);
             if (f + 1 < a.nfields) load_field(f + 1, nxt);
         } else {
-            load_field(f, v);
+            cx<TI> cur[SZ::RMAX];
+            load_field(f, cur);
+#pragma unroll
+            for (int r = 0; r < SZ::RMAX; ++r) v[0][r] = cx_as<T>(cur[r]);
         }
         transform<T, SZ, 1, true, LP, false>(par ? bufB : bufA, par ? bufA : bufB, v, j, g, 0, Wl);
         if (ALT) par ^= 1;

@@ -254,9 +254,9 @@ __global__ __launch_bounds__(512) void pass_contig(PassArgs<T> a) {
 #pragma unroll
             for (int o = 0; o < NOUT; ++o) acc[m][o] = {(T)0, (T)0};
 
-        for (int c = a.group_first[g]; c < a.group_first[g + 1]; ++c) {
-            const int src = a.chunk_src[c];
-            const int ct = a.chunk_coef[c];
+        for (int c = uload(a.group_first, g); c < uload(a.group_first, g + 1); ++c) {
+            const int src = uload(a.chunk_src, c);
+            const int ct = uload(a.chunk_coef, c);
             for (int j0 = 0; j0 < a.nin; j0 += a.nf) {
                 const int nfc = min(a.nf, a.nin - j0);
                 const cx<TS>* inbase =
@@ -343,9 +343,9 @@ __global__ __launch_bounds__(512) void pass_strided(PassArgs<T> a) {
 #pragma unroll
             for (int o = 0; o < NOUT; ++o) acc[m][o] = {(T)0, (T)0};
 
-        for (int c = a.group_first[g]; c < a.group_first[g + 1]; ++c) {
-            const int src = a.chunk_src[c];
-            const int ct = a.chunk_coef[c];
+        for (int c = uload(a.group_first, g); c < uload(a.group_first, g + 1); ++c) {
+            const int src = uload(a.chunk_src, c);
+            const int ct = uload(a.chunk_coef, c);
             for (int j0 = 0; j0 < a.nin; j0 += a.nf) {
                 const int nfc = min(a.nf, a.nin - j0);
                 const cx<TI>* inbase = reinterpret_cast<const cx<TI>*>(a.in) + ((size_t)src * a.nin + j0) * a.ncell;
@@ -460,8 +460,8 @@ __global__ __launch_bounds__(512) void pass_strided_mix(PassArgs<T> a) {
                     const int jj = j0 + f;
 #pragma unroll
                     for (int z = 0; z < NZ; ++z) {
-                        for (int p = a.pair_first[s * NZ + z]; p < a.pair_first[s * NZ + z + 1]; ++p) {
-                            const int y = a.pair_y[p];
+                        for (int p = uload(a.pair_first, s * NZ + z); p < uload(a.pair_first, s * NZ + z + 1); ++p) {
+                            const int y = uload(a.pair_y, p);
                             const cx<T> xc = a.mixc[(size_t)(y * a.nsrc + s) * a.n0 + i0];
 #pragma unroll
                             for (int m = 0; m < EPT; ++m) {
