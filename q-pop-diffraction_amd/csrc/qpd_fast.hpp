@@ -1439,7 +1439,8 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided_mix(PassArgs<T> a) {
 // group's T_S is held, the first outputs of all z-classes are formed and stored, and the
 // held values are reused for the F1 C_S.  A thread holds NS RL group values (the last
 // transform of a round runs beside NS - 1 of them), where fast_strided_mix holds NZ 2 RL
-// sums and 2 RL group values; the prefetched line stays in the arrays' own type.
+// sums and 2 RL group values; the prefetched line stays in the arrays' own type, and the
+// wavenumbers are read from LDS (a vector load of them would wait for the prefetch too).
 template <typename T, class SZ, int G, int NZ, int NS, bool IO32 = false, bool ONEBUF = false>
 __global__ __launch_bounds__(G* SZ::TPL) void fast_mix_held(PassArgs<T> a) {
     using TI = io_t<T, IO32>;
@@ -1451,6 +1452,7 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_mix_held(PassArgs<T> a) {
     cx<T>* bufA = reinterpret_cast<cx<T>*>(smem_raw);
     cx<T>* bufB = ONEBUF ? bufA : bufA + TILE;
     cx<T>* Wl = bufB + TILE;
+    T* kl = reinterpret_cast<T*>(Wl + N);  // the wavenumbers along the pass, read from LDS
     const int tid = threadIdx.x;
     const int g = tid % G, j = tid / G;
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
@@ -1466,8 +1468,8 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_mix_held(PassArgs<T> a) {
     const uint32_t in_lane = ((col_ok ? i0 : a.n0 - 1) - t0 * G + jl * a.S) * (uint32_t)sizeof(cx<TI>);
     const uint32_t out_lane = (g + jl * a.S) * (uint32_t)sizeof(cx<TI>);
     const uint32_t coff = jl * (uint32_t)sizeof(cx<T>);
-    const uint32_t koff = jl * (uint32_t)sizeof(T);
     load_twiddles<T, SZ>(Wl, a.W, tid, G * SZ::TPL);
+    for (int m = tid; m < N; m += G * SZ::TPL) kl[m] = a.k[m];
     lds_barrier();
     // item it -> input field: A_s, B_s for s = it / 2 while it < 2 NS, then C_{it - 2 NS}
     auto load_item = [&](int it, cx<TI>(&dst)[SZ::RMAX]) {
@@ -1535,7 +1537,7 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_mix_held(PassArgs<T> a) {
                 } else if (it == 2 * s + 1) {
 #pragma unroll
                     for (int r = 0; r < RL; ++r)
-                        held[s][r] = held[s][r] + cscale(negi(v[0][r]), load_at(a.k + (jw + r * NRL), koff));
+                        held[s][r] = held[s][r] + cscale(negi(v[0][r]), kl[j + r * NRL]);
                 }
             }
         }
