@@ -440,6 +440,171 @@ __device__ __forceinline__ void transform(cx<T>* X, cx<T>* Y, cx<T> (&v)[F][SZ::
     }
 }
 
+// A last stage of prime radix R shared by P lanes per butterfly (the last stage of a line
+// with few butterflies leaves most lanes idle: 68 = 4 x 17 has four radix-17 butterflies on
+// seventeen lanes).  With g a primitive root mod R, the P cosets of the subgroup <g^P> split
+// the outputs 1..R-1; lane q of a butterfly computes outputs g^q t0 and -g^q t0 for the
+// representatives t0 of that subgroup (up to sign), so that every lane runs the same
+// constants: X[g^q t0] = sum_s x[g^-q s] w^(t0 s), i.e. the lane reads its inputs in the
+// order r = g^-q s mod R and forms the standard symmetric-pair sums of X[t0].  Lane 0 also
+// forms X[0].  (R - 1) / P must be even, so that the subgroup holds -1.
+template <int R, int P>
+struct SplitOdd {
+    static constexpr int H = (R - 1) / 2;
+    static constexpr int PPL = (R - 1) / (2 * P);  // output pairs a lane
+    static constexpr int SLOTS = 2 * PPL + 1;      // X[t0], X[-t0] per pair, then X[0] (lane 0)
+    static constexpr int pw(int b, int e) { return e == 0 ? 1 : (pw(b, e - 1) * b) % R; }
+    static constexpr bool primitive(int g) {
+        for (int e = 1; e < R - 1; ++e)
+            if (pw(g, e) == 1) return false;
+        return true;
+    }
+    static constexpr int root() {
+        for (int g = 2; g < R; ++g)
+            if (primitive(g)) return g;
+        return 0;
+    }
+    static constexpr int G = root();
+    static constexpr int GINV = pw(G, R - 2);
+    // the subgroup's representatives up to sign (the members at most H)
+    static constexpr int t0(int i) {
+        int n = 0;
+        for (int e = 0; e < (R - 1) / P; ++e) {
+            const int m = pw(G, P * e);
+            if (m <= H) {
+                if (n == i) return m;
+                ++n;
+            }
+        }
+        return 0;
+    }
+    static_assert(R >= 5 && (R - 1) % (2 * P) == 0, "the subgroup must hold -1");
+};
+
+// The representatives t0(k) and the powers g^q of a split as a compile-time table (indexed by
+// unrolled loop counters, its entries fold into the code).
+template <int R, int P>
+struct SplitTab {
+    int t0[SplitOdd<R, P>::PPL];
+    int gq[P];
+    constexpr SplitTab() : t0(), gq() {
+        for (int k = 0; k < SplitOdd<R, P>::PPL; ++k) t0[k] = SplitOdd<R, P>::t0(k);
+        for (int q = 0; q < P; ++q) gq[q] = SplitOdd<R, P>::pw(SplitOdd<R, P>::G, q);
+    }
+};
+template <int R, int P>
+__device__ constexpr SplitTab<R, P> kSplitTab{};
+
+// Slot i of lane q (q < P): its output index t in 0..R-1 (slot 2k: g^q t0(k), 2k+1: -g^q t0(k),
+// 2 PPL: 0); -1 where the lane has no such output (X[0] on lanes q > 0).
+template <int R, int P>
+__device__ __forceinline__ int split_slot_index(int q, int i) {
+    using SO = SplitOdd<R, P>;
+    if (i == 2 * SO::PPL) return q == 0 ? 0 : -1;
+    int gq = 1;
+#pragma unroll
+    for (int e = 1; e < P; ++e) gq = q == e ? kSplitTab<R, P>.gq[e] : gq;
+    const int t = (gq * kSplitTab<R, P>.t0[i >> 1]) % R;
+    return (i & 1) ? R - t : t;
+}
+
+// Slots 2K, 2K + 1 (X[t0], X[-t0] with t0 = t0(K), a compile-time constant) and the pairs after
+// them, from the symmetric sums av, bv of the lane's inputs v.
+template <typename T, int R, int P, int K>
+__device__ __forceinline__ void split_pairs(cx<T>* v, const cx<T>* av, const cx<T>* bv, std::integral_constant<int, K>) {
+    using SO = SplitOdd<R, P>;
+    if constexpr (K < SO::PPL) {
+        constexpr int H = SO::H, t = SO::t0(K);
+        constexpr bool f32 = std::is_same<T, float>::value;
+        cx<T> A = v[0];
+        cx<T> B = {(T)0, (T)0};
+#pragma unroll
+        for (int s = 1; s <= H; ++s) {
+            const int m = (t * s) % R;
+            const int mm = m <= H ? m : R - m;
+            const T c = f32 ? (T)kOddRoots<R>.c[mm] : (T)kOddRoots64<R>.c[mm];
+            const T s1 = f32 ? (T)kOddRoots<R>.s[mm] : (T)kOddRoots64<R>.s[mm];
+            const T sn = m <= H ? s1 : -s1;
+            A.x += av[s].x * c;
+            A.y += av[s].y * c;
+            B.x += bv[s].x * sn;
+            B.y += bv[s].y * sn;
+        }
+        // v[0] is read by every pair: the later pairs are formed before this one is stored
+        split_pairs<T, R, P>(v, av, bv, std::integral_constant<int, K + 1>{});
+        v[2 * K] = {A.x + B.y, A.y - B.x};
+        v[2 * K + 1] = {A.x - B.y, A.y + B.x};
+    }
+}
+
+// The slots of one lane of a split radix-R DFT whose inputs v[0..R) are already in the lane's
+// order (x[g^-q s] in v[s]): X[g^q t0(k)], X[-g^q t0(k)] in v[2k], v[2k + 1], then X[0] in
+// v[2 PPL] (used by lane 0 only).
+template <typename T, int R, int P, int N>
+__device__ __forceinline__ void dft_split_slots(cx<T> (&v)[N]) {
+    using SO = SplitOdd<R, P>;
+    constexpr int H = SO::H;
+    cx<T> av[H + 1], bv[H + 1];
+    cx<T> y0 = v[0];
+#pragma unroll
+    for (int s = 1; s <= H; ++s) {
+        av[s] = v[s] + v[R - s];
+        bv[s] = v[s] - v[R - s];
+        y0 = y0 + av[s];
+    }
+    split_pairs<T, R, P>(v, av, bv, std::integral_constant<int, 0>{});
+    v[2 * SO::PPL] = y0;
+}
+
+// The last stage of a line, split over P lanes per butterfly: lane (b, q) = (j % NRL, j / NRL)
+// reads the R inputs of butterfly b in its order, twiddles them and leaves its slots in v.
+template <typename T, class SZ, int P>
+__device__ __forceinline__ void last_stage_split(const cx<T>* line, cx<T> (&v)[SZ::RMAX], int b, int q,
+                                                 const cx<T>* tw) {
+    constexpr int L = SZ::NST - 1, R = SZ::rad(L), NR = SZ::nr(L), NS = SZ::ns(L);
+    constexpr int OFF = StageTw<SZ>::off(L);
+    using SO = SplitOdd<R, P>;
+    int gi = 1;  // g^-q mod R
+#pragma unroll
+    for (int e = 0; e + 1 < P; ++e) gi = e < q ? (gi * SO::GINV) % R : gi;
+    v[0] = line[lds_addr<false, SZ, 0>(0, b)];
+    int r = 0;
+#pragma unroll
+    for (int s = 1; s < R; ++s) {
+        r += gi;
+        r = r >= R ? r - R : r;  // r = g^-q s mod R
+        const cx<T> x = line[lds_addr<false, SZ, 0>(0, b + r * NR)];
+        v[s] = NS > 1 ? cmul(x, tw[OFF + (r - 1) * NS + b]) : x;
+    }
+    dft_split_slots<T, R, P>(v);
+}
+
+// Outputs a lane holds after a line transform, at the row's point index
+// out_index(j, i) for slot i < out_slots (i.e. X[j + i N/RL] when the last stage is not split).
+template <class SZ, int P>
+struct LineOut {
+    static constexpr int L = SZ::NST - 1;
+    static constexpr int NRL = SZ::NRL;
+    static constexpr int slots() {
+        if constexpr (P > 1)
+            return SplitOdd<SZ::rad(L), P>::SLOTS;
+        else
+            return SZ::RL;
+    }
+    static constexpr int SLOTS = slots();
+    // the lane holds outputs at all
+    __device__ static bool owns(int j) { return j < P * NRL; }
+    // point index of slot i, or -1
+    __device__ static int index(int j, int i) {
+        if constexpr (P > 1) {
+            const int t = split_slot_index<SZ::rad(L), P>(j / NRL, i);
+            return t < 0 ? -1 : j % NRL + t * NRL;
+        } else {
+            return j + i * NRL;
+        }
+    }
+};
+
 // transform() for nlines contiguous lines whose stage-0 inputs are in X in natural order
 // (written before a barrier), one line per thread group: the stages before the last
 // run with their butterflies packed over the block's threads (thread t takes butterfly
@@ -448,7 +613,47 @@ __device__ __forceinline__ void transform(cx<T>* X, cx<T>* Y, cx<T> (&v)[F][SZ::
 // the rest idle instead of running every wave with a few active lanes; the last stage
 // runs in the thread's own line (line, j) and leaves X[j + r NRL] in v.  Same buffers,
 // writes and barriers as transform<..., LOAD0 = true>.
-template <typename T, class SZ>
+// Stage 0 of transform_packed split over P0 threads per butterfly (P0 > 1; empty otherwise).
+template <typename T, class SZ, int P0>
+__device__ __forceinline__ void packed_stage0_split(const cx<T>* rd, cx<T>* wr, cx<T> (&v)[1][SZ::RMAX], int tid,
+                                                int nlines) {
+    if constexpr (P0 > 1) {
+        constexpr int NR = SZ::nr(0);
+        constexpr int R0 = SZ::rad(0);
+        using SO = SplitOdd<R0, P0>;
+        int ln = tid / (NR * P0);
+        const int rem = tid - ln * (NR * P0);
+        int b = rem % NR, q = rem / NR;
+        // the line, butterfly and coset are re-formed for every use (the empty asm hides their
+        // values), so that the compiler does not hold this stage's eighteen-odd LDS addresses
+        // across the chunks
+        asm volatile("" : "+v"(ln), "+v"(b), "+v"(q));
+        if (ln < nlines) {
+            int gi = 1;  // g^-q mod R0
+#pragma unroll
+            for (int e = 0; e + 1 < P0; ++e) gi = e < q ? (gi * SO::GINV) % R0 : gi;
+            v[0][0] = rd[lds_addr<false, SZ, 0>(ln, b)];
+            int r = 0;
+#pragma unroll
+            for (int s = 1; s < R0; ++s) {
+                r += gi;
+                r = r >= R0 ? r - R0 : r;
+                v[0][s] = rd[lds_addr<false, SZ, 0>(ln, b + r * NR)];
+            }
+            dft_split_slots<T, R0, P0>(v[0]);
+#pragma unroll
+            for (int i = 0; i < SO::SLOTS; ++i) {
+                const int t = split_slot_index<R0, P0>(q, i);
+                if (t >= 0) wr[lds_addr<false, SZ, 0>(ln, b * R0 + t)] = v[0][i];
+            }
+        }
+    }
+}
+
+// P0 > 1: stage 0 (prime radix R0, packed) runs P0 threads per butterfly, thread t taking coset
+// q = (t mod NR0 P0) / NR0 of butterfly b = t mod NR0 of line t / (NR0 P0) (see SplitOdd): it
+// reads the butterfly's inputs in the coset's order and writes its slots to their outputs.
+template <typename T, class SZ, int P0 = 1>
 __device__ __forceinline__ void transform_packed(cx<T>* X, cx<T>* Y, cx<T> (&v)[1][SZ::RMAX], int tid, int nlines,
                                                  int j, int line, const cx<T>* __restrict__ W) {
     constexpr int SL = SZ::NST - 1;
@@ -457,11 +662,15 @@ __device__ __forceinline__ void transform_packed(cx<T>* X, cx<T>* Y, cx<T> (&v)[
     auto packed_stage = [&](auto s_) {
         constexpr int S = decltype(s_)::value;
         constexpr int NR = SZ::nr(S);
-        const int ln = tid / NR, jj = tid - ln * NR;
-        if (ln < nlines) {
-            stage_read<T, SZ, S, 1, false, 0>(rd, v, jj, ln, 0);
-            stage_compute<T, SZ, S, 1>(v, jj, W);
-            stage_write<T, SZ, S, 1, false, 0>(wr, v, jj, ln, 0);
+        if constexpr (S == 0 && P0 > 1) {
+            packed_stage0_split<T, SZ, P0>(rd, wr, v, tid, nlines);
+        } else {
+            const int ln = tid / NR, jj = tid - ln * NR;
+            if (ln < nlines) {
+                stage_read<T, SZ, S, 1, false, 0>(rd, v, jj, ln, 0);
+                stage_compute<T, SZ, S, 1>(v, jj, W);
+                stage_write<T, SZ, S, 1, false, 0>(wr, v, jj, ln, 0);
+            }
         }
         lds_barrier();
         cx<T>* t = rd;
@@ -573,7 +782,8 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_contig(PassArgs<T> a) {
 // contribution of input f to its output: f = 0 and f = 1 (times -i k) both feed
 // output 0 and are summed through LDS once per group; f = 2, 3 feed outputs 1, 2.
 // IO32 (FP64 only): FP64 arithmetic on FP32 P, weights and outputs.
-template <typename T, class SZ, int G, bool IO32 = false>
+// P0 > 1: stage 0 split over P0 threads per butterfly (transform_packed).
+template <typename T, class SZ, int G, bool IO32 = false, int P0 = 1>
 __global__ __launch_bounds__(4 * G * SZ::TPL) void fast_source(PassArgs<T> a) {
     using TS = io_t<T, IO32>;
     constexpr int N = SZ::N, TPL = SZ::TPL, NRL = SZ::NRL, RL = SZ::RL;
@@ -656,7 +866,7 @@ __global__ __launch_bounds__(4 * G * SZ::TPL) void fast_source(PassArgs<T> a) {
             }
             lds_barrier();
             cx<T> v[1][SZ::RMAX];
-            transform_packed<T, SZ>(X, Y, v, tid, 4 * G, j, f * G + g, Wl);
+            transform_packed<T, SZ, P0>(X, Y, v, tid, 4 * G, j, f * G + g, Wl);
             if (j < NRL) {
 #pragma unroll
                 for (int r = 0; r < RL; ++r) {
@@ -724,148 +934,6 @@ __device__ __forceinline__ void stage_compute_tw(cx<T> (&v)[SZ::RMAX], int j, co
     }
     dft<R>(v);
 }
-
-// A last stage of prime radix R shared by P lanes per butterfly (the last stage of a line
-// with few butterflies leaves most lanes idle: 68 = 4 x 17 has four radix-17 butterflies on
-// seventeen lanes).  With g a primitive root mod R, the P cosets of the subgroup <g^P> split
-// the outputs 1..R-1; lane q of a butterfly computes outputs g^q t0 and -g^q t0 for the
-// representatives t0 of that subgroup (up to sign), so that every lane runs the same
-// constants: X[g^q t0] = sum_s x[g^-q s] w^(t0 s), i.e. the lane reads its inputs in the
-// order r = g^-q s mod R and forms the standard symmetric-pair sums of X[t0].  Lane 0 also
-// forms X[0].  (R - 1) / P must be even, so that the subgroup holds -1.
-template <int R, int P>
-struct SplitOdd {
-    static constexpr int H = (R - 1) / 2;
-    static constexpr int PPL = (R - 1) / (2 * P);  // output pairs a lane
-    static constexpr int SLOTS = 2 * PPL + 1;      // X[t0], X[-t0] per pair, then X[0] (lane 0)
-    static constexpr int pw(int b, int e) { return e == 0 ? 1 : (pw(b, e - 1) * b) % R; }
-    static constexpr bool primitive(int g) {
-        for (int e = 1; e < R - 1; ++e)
-            if (pw(g, e) == 1) return false;
-        return true;
-    }
-    static constexpr int root() {
-        for (int g = 2; g < R; ++g)
-            if (primitive(g)) return g;
-        return 0;
-    }
-    static constexpr int G = root();
-    static constexpr int GINV = pw(G, R - 2);
-    // the subgroup's representatives up to sign (the members at most H)
-    static constexpr int t0(int i) {
-        int n = 0;
-        for (int e = 0; e < (R - 1) / P; ++e) {
-            const int m = pw(G, P * e);
-            if (m <= H) {
-                if (n == i) return m;
-                ++n;
-            }
-        }
-        return 0;
-    }
-    static_assert(R >= 5 && (R - 1) % (2 * P) == 0, "the subgroup must hold -1");
-};
-
-// Slot i of lane q (q < P): its output index t in 0..R-1 (slot 2k: g^q t0(k), 2k+1: -g^q t0(k),
-// 2 PPL: 0); -1 where the lane has no such output (X[0] on lanes q > 0).
-template <int R, int P>
-__device__ __forceinline__ int split_slot_index(int q, int i) {
-    using SO = SplitOdd<R, P>;
-    int gq = 1;
-#pragma unroll
-    for (int e = 0; e + 1 < P; ++e) gq = e < q ? (gq * SO::G) % R : gq;
-    if (i == 2 * SO::PPL) return q == 0 ? 0 : -1;
-    const int t = (gq * SO::t0(i >> 1)) % R;
-    return (i & 1) ? R - t : t;
-}
-
-// Slots 2K, 2K + 1 (X[t0], X[-t0] with t0 = t0(K), a compile-time constant) and the pairs after
-// them, from the symmetric sums av, bv of the lane's inputs v.
-template <typename T, int R, int P, int K>
-__device__ __forceinline__ void split_pairs(cx<T>* v, const cx<T>* av, const cx<T>* bv, std::integral_constant<int, K>) {
-    using SO = SplitOdd<R, P>;
-    if constexpr (K < SO::PPL) {
-        constexpr int H = SO::H, t = SO::t0(K);
-        constexpr bool f32 = std::is_same<T, float>::value;
-        cx<T> A = v[0];
-        cx<T> B = {(T)0, (T)0};
-#pragma unroll
-        for (int s = 1; s <= H; ++s) {
-            const int m = (t * s) % R;
-            const int mm = m <= H ? m : R - m;
-            const T c = f32 ? (T)kOddRoots<R>.c[mm] : (T)kOddRoots64<R>.c[mm];
-            const T s1 = f32 ? (T)kOddRoots<R>.s[mm] : (T)kOddRoots64<R>.s[mm];
-            const T sn = m <= H ? s1 : -s1;
-            A.x += av[s].x * c;
-            A.y += av[s].y * c;
-            B.x += bv[s].x * sn;
-            B.y += bv[s].y * sn;
-        }
-        // v[0] is read by every pair: the later pairs are formed before this one is stored
-        split_pairs<T, R, P>(v, av, bv, std::integral_constant<int, K + 1>{});
-        v[2 * K] = {A.x + B.y, A.y - B.x};
-        v[2 * K + 1] = {A.x - B.y, A.y + B.x};
-    }
-}
-
-// The last stage of a line, split over P lanes per butterfly: lane (b, q) = (j % NRL, j / NRL)
-// reads the R inputs of butterfly b in its order, twiddles them and leaves its slots in v.
-template <typename T, class SZ, int P>
-__device__ __forceinline__ void last_stage_split(const cx<T>* line, cx<T> (&v)[SZ::RMAX], int b, int q,
-                                                 const cx<T>* tw) {
-    constexpr int L = SZ::NST - 1, R = SZ::rad(L), NR = SZ::nr(L), NS = SZ::ns(L);
-    constexpr int OFF = StageTw<SZ>::off(L);
-    using SO = SplitOdd<R, P>;
-    int gi = 1;  // g^-q mod R
-#pragma unroll
-    for (int e = 0; e + 1 < P; ++e) gi = e < q ? (gi * SO::GINV) % R : gi;
-    v[0] = line[lds_addr<false, SZ, 0>(0, b)];
-    int r = 0;
-#pragma unroll
-    for (int s = 1; s < R; ++s) {
-        r += gi;
-        r = r >= R ? r - R : r;  // r = g^-q s mod R
-        const cx<T> x = line[lds_addr<false, SZ, 0>(0, b + r * NR)];
-        v[s] = NS > 1 ? cmul(x, tw[OFF + (r - 1) * NS + b]) : x;
-    }
-    constexpr int H = SO::H;
-    cx<T> av[H + 1], bv[H + 1];
-    cx<T> y0 = v[0];
-#pragma unroll
-    for (int s = 1; s <= H; ++s) {
-        av[s] = v[s] + v[R - s];
-        bv[s] = v[s] - v[R - s];
-        y0 = y0 + av[s];
-    }
-    split_pairs<T, R, P>(v, av, bv, std::integral_constant<int, 0>{});
-    v[2 * SO::PPL] = y0;
-}
-
-// Outputs a lane holds after a line transform, at the row's point index
-// out_index(j, i) for slot i < out_slots (i.e. X[j + i N/RL] when the last stage is not split).
-template <class SZ, int P>
-struct LineOut {
-    static constexpr int L = SZ::NST - 1;
-    static constexpr int NRL = SZ::NRL;
-    static constexpr int slots() {
-        if constexpr (P > 1)
-            return SplitOdd<SZ::rad(L), P>::SLOTS;
-        else
-            return SZ::RL;
-    }
-    static constexpr int SLOTS = slots();
-    // the lane holds outputs at all
-    __device__ static bool owns(int j) { return j < P * NRL; }
-    // point index of slot i, or -1
-    __device__ static int index(int j, int i) {
-        if constexpr (P > 1) {
-            const int t = split_slot_index<SZ::rad(L), P>(j / NRL, i);
-            return t < 0 ? -1 : j % NRL + t * NRL;
-        } else {
-            return j + i * NRL;
-        }
-    }
-};
 
 // All stages of one line: the stage-0 inputs x[j + r N/R0] are in v on entry, the outputs
 // X[j + r N/RL] on return.  act: the lane owns a line (inactive lanes touch no memory).

@@ -188,6 +188,64 @@ hipError_t srcw_launch64(const Launch& L, const PassArgs<double>& a, hipStream_t
     return launch_one(fast_source_wave<double, SZ, ROWS, IO32, P>, L, st, a);
 }
 
+// Threads per butterfly of the block-per-row source pass's first stage (a prime radix R0 with
+// nr(0) butterflies a line and TPL threads a line): the largest P0 with P0 nr(0) <= TPL and
+// (R0 - 1) divisible by 2 P0, else 1.
+template <class SZ>
+constexpr int first_split_lanes() {
+    constexpr int R = SZ::rad(0);
+    bool prime = R >= 5;
+    for (int d = 2; d * d <= R; ++d) prime = prime && R % d != 0;
+    if (!prime) return 1;
+    for (int p = SZ::TPL / SZ::nr(0); p >= 2; --p)
+        if ((R - 1) % (2 * p) == 0) return p;
+    return 1;
+}
+
+template <class SZ, int G, int P0>
+hipError_t src_split_launch(const Launch& L, const PassArgs<float>& a, hipStream_t st) {
+    return launch_one(fast_source<float, SZ, G, false, P0>, L, st, a);
+}
+template <class SZ, int G, bool IO32, int P0>
+hipError_t src_split_launch64(const Launch& L, const PassArgs<double>& a, hipStream_t st) {
+    return launch_one(fast_source<double, SZ, G, IO32, P0>, L, st, a);
+}
+
+// The block-per-row source pass with its first stage split over P0 threads per butterfly
+// (first_split_lanes): FP32 at its default rows a block and twice them, FP64 and FP64 on FP32
+// arrays at the rows the 256-thread FP64 blocks allow; listed before the unsplit forms.
+template <class SZ>
+void add_source_split(std::vector<FastEntry>& v, const char* radices) {
+    constexpr int P0 = first_split_lanes<SZ>();
+    if constexpr (P0 > 1) {
+        constexpr int G = fast_tile(kRoleSource, SZ::TPL);
+        auto add32 = [&](auto g_) {
+            constexpr int GG = decltype(g_)::value;
+            constexpr size_t lds = (2 * (size_t)4 * GG * SZ::PITCH + SZ::N) * 8;
+            if constexpr (4 * GG * SZ::TPL <= 512 && lds <= 160 * 1024) {
+                FastEntry e{SZ::N, kRoleSource, GG, 4 * GG * SZ::TPL, SZ::TPL, lds, radices,
+                            &src_split_launch<SZ, GG, P0>, (const void*)fast_source<float, SZ, GG, false, P0>, false};
+                e.split = P0;
+                v.push_back(e);
+            }
+        };
+        add32(std::integral_constant<int, G>{});
+        add32(std::integral_constant<int, 2 * G>{});
+        constexpr size_t lds64 = (2 * (size_t)4 * G * SZ::PITCH + SZ::N) * 16;
+        if constexpr (4 * G * SZ::TPL <= 256 && lds64 <= 160 * 1024) {
+            for (const bool io32 : {true, false}) {
+                FastEntry e{SZ::N, kRoleSource, G, 4 * G * SZ::TPL, SZ::TPL, lds64, radices, nullptr,
+                            io32 ? (const void*)fast_source<double, SZ, G, true, P0>
+                                 : (const void*)fast_source<double, SZ, G, false, P0>,
+                            false, 8, io32 ? &src_split_launch64<SZ, G, true, P0> : &src_split_launch64<SZ, G, false, P0>,
+                            io32};
+                e.split = P0;
+                v.push_back(e);
+            }
+        }
+    }
+}
+
 // The one-wave-per-row source pass (lengths whose lines fit one wave and whose first
 // radix is at most 8, so that the four fields' stage-0 inputs fit in registers): G is the
 // rows of a block; the LDS holds one line per row and the per-stage twiddle table.  Lengths
@@ -258,6 +316,7 @@ template <class SZ, unsigned ROLES>
 void add_fast_roles(std::vector<FastEntry>& v, const char* radices) {
     if constexpr ((ROLES >> kRoleSource) & 1u) {
         add_source_wave<SZ>(v, radices);
+        add_source_split<SZ>(v, radices);
         add_fast_variants<SZ, kRoleSource>(v, radices);
     }
     if constexpr ((ROLES >> kRoleContig) & 1u) add_fast_variants<SZ, kRoleContig>(v, radices);

@@ -112,14 +112,16 @@ def test_fp64_every_length(refl, role, dim, form, monkeypatch):
     assert ran_f64 >= len(lengths) // 2
 
 
+@pytest.mark.parametrize("kind", ["source-wave", "source"])
 @pytest.mark.parametrize("N,P", [(34, 8), (52, 3), (68, 4), (136, 2)])
-def test_fp64_source_wave_split_last_stage(refl, N, P):
-    """The FP64 source pass with its last (prime-radix) stage split over P lanes per butterfly,
-    at the reference's bound."""
+def test_fp64_source_split_stage(refl, N, P, kind):
+    """The FP64 source passes with a prime-radix stage split over P lanes per butterfly (the
+    one-wave-per-row pass's last stage, the block-per-row pass's first), at the reference's
+    bound."""
     from qpd_oracle import diffraction_fp64
     shape = (N, 20, 25)
-    print(f"FP64 split source N={N} P={P}", flush=True)
-    I, snap, plan = run_fp64(refl, shape, pick={0: f" source-wave split{P} f64"})
+    print(f"FP64 split {kind} N={N} P={P}", flush=True)
+    I, snap, plan = run_fp64(refl, shape, pick={0: f" {kind} split{P} f64"})
     ref = diffraction_fp64(snap.params, refl.atoms, *workloads.host_inputs(snap))[0]
     np.testing.assert_allclose(I, ref, rtol=1e-9, atol=1e-12 * ref.max(), err_msg=f"split source N={N}")
 

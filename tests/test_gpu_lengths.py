@@ -162,23 +162,25 @@ def test_volume_every_length(refl, role, dim, form, arith, monkeypatch):
         assert ran_fast >= len(lengths) // 2
 
 
-# lengths whose one-wave-per-row source pass also has a last (prime-radix) stage shared by
-# several lanes per butterfly, with that lane count: 34 = 2 x 17 (eight lanes), 52 = 4 x 13
-# (three), 68 = 4 x 17 (four), 136 = 8 x 17 (two)
+# lengths whose source passes also have a prime-radix stage shared by several lanes per
+# butterfly, with that lane count: 34 = 2 x 17 (eight lanes), 52 = 4 x 13 (three), 68 = 4 x 17
+# (four), 136 = 8 x 17 (two); the one-wave-per-row pass splits its last stage (radix order
+# 2x17, 4x13, ...), the block-per-row pass its first (17x2, 13x4, ...)
 SPLIT_SOURCE = [(34, 8), (52, 3), (68, 4), (136, 2)]
 
 
+@pytest.mark.parametrize("kind", ["source-wave", "source"])
 @pytest.mark.parametrize("arith", ["fp32", "mixed"])
 @pytest.mark.parametrize("N,P", SPLIT_SOURCE)
-def test_source_wave_split_last_stage(refl, N, P, arith):
-    """The source pass with the last stage split over P lanes per butterfly (each lane forms
-    the outputs of one coset of the radix's multiplicative group) against the oracle, FP32 and
+def test_source_split_stage(refl, N, P, arith, kind):
+    """The source passes with a stage split over P lanes per butterfly (each lane forms the
+    outputs of one coset of the radix's multiplicative group) against the oracle, FP32 and
     FP64 transforms on FP32 arrays, on rows that leave a partial last block.  The FP64
     transforms are also held to |dlog10 I| <= 1e-3 (explicit FP32 arithmetic has no such
     bound)."""
-    suffix = f" source-wave split{P}" + (" f64 io32" if arith == "mixed" else "")
+    suffix = f" {kind} split{P}" + (" f64 io32" if arith == "mixed" else "")
     shape = [N, 20, 25]
-    print(f"split source N={N} P={P} {arith}", flush=True)
+    print(f"split {kind} N={N} P={P} {arith}", flush=True)
     err = volume_check(refl, *shape, pick={0: suffix}, arith=arith, strict=arith == "mixed")
     assert err <= FP32_PEAK_TOL, (N, err)
 
