@@ -348,6 +348,52 @@ __device__ __forceinline__ void stage_compute(cx<T> (&v)[F][SZ::RMAX], int j, co
     }
 }
 
+// Stage S (odd radix R >= 5, not the last) of one line held in v: twiddle, then the DFT_R
+// outputs written to their Stockham positions in LDS a symmetric pair at a time, so that the R
+// outputs are never all held in registers beside the pair sums (stage_compute + stage_write
+// hold both: about 2 R complex values).
+template <typename T, class SZ, int S, bool STRIDED, int LP>
+__device__ __forceinline__ void stage_odd_to_lds(cx<T>* tile, cx<T> (&v)[1][SZ::RMAX], int j, int line,
+                                                 const cx<T>* __restrict__ W) {
+    constexpr int R = SZ::rad(S), NS = SZ::ns(S), H = (R - 1) / 2;
+    if constexpr (NS > 1) {
+        constexpr int OFF = StageTw<SZ>::off(S);
+        const int k = j % NS;
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[0][r] = cmul(v[0][r], W[OFF + (r - 1) * NS + k]);
+    }
+    cx<T> a[H + 1], b[H + 1];
+    cx<T> y0 = v[0][0];
+#pragma unroll
+    for (int s = 1; s <= H; ++s) {
+        a[s] = v[0][s] + v[0][R - s];
+        b[s] = v[0][s] - v[0][R - s];
+        y0 = y0 + a[s];
+    }
+    const int o = (j / NS) * NS * R + (j % NS);
+    tile[lds_addr<STRIDED, SZ, LP>(line, o)] = y0;
+    constexpr bool f32 = std::is_same<T, float>::value;
+#pragma unroll
+    for (int t = 1; t <= H; ++t) {
+        cx<T> A = v[0][0];
+        cx<T> B = {(T)0, (T)0};
+#pragma unroll
+        for (int s = 1; s <= H; ++s) {
+            const int m = (t * s) % R;
+            const int mm = m <= H ? m : R - m;
+            const T c = f32 ? (T)kOddRoots<R>.c[mm] : (T)kOddRoots64<R>.c[mm];
+            const T s1 = f32 ? (T)kOddRoots<R>.s[mm] : (T)kOddRoots64<R>.s[mm];
+            const T sn = m <= H ? s1 : -s1;
+            A.x += a[s].x * c;
+            A.y += a[s].y * c;
+            B.x += b[s].x * sn;
+            B.y += b[s].y * sn;
+        }
+        tile[lds_addr<STRIDED, SZ, LP>(line, o + t * NS)] = {A.x + B.y, A.y - B.x};
+        tile[lds_addr<STRIDED, SZ, LP>(line, o + (R - t) * NS)] = {A.x - B.y, A.y + B.x};
+    }
+}
+
 // Write the outputs of stage S (butterfly j) to LDS at their Stockham positions.
 template <typename T, class SZ, int S, int F, bool STRIDED, int LP>
 __device__ __forceinline__ void stage_write(cx<T>* tile, const cx<T> (&v)[F][SZ::RMAX], int j, int line0, int dline) {
@@ -396,10 +442,19 @@ __device__ __forceinline__ void transform(cx<T>* X, cx<T>* Y, cx<T> (&v)[F][SZ::
     if constexpr (LOAD0) {
         if (a0) stage_read<T, SZ, 0, F, STRIDED, LP>(rd, v, j, line0, dline);
     }
-    if (a0) stage_compute<T, SZ, 0, F>(v, j, W);
-    if constexpr (SZ::NST > 1) {
+    // an odd first radix (>= 5) of a multi-stage line writes its outputs as it forms them
+    constexpr bool ODD0 = F == 1 && SZ::NST > 1 && (SZ::rad(0) & 1) == 1 && SZ::rad(0) >= 5;
+    if constexpr (ODD0) {
         if constexpr (ONEBUF) lds_barrier();
-        if (a0) stage_write<T, SZ, 0, F, STRIDED, LP>(wr, v, j, line0, dline);
+        if (a0) stage_odd_to_lds<T, SZ, 0, STRIDED, LP>(wr, v, j, line0, W);
+    } else {
+        if (a0) stage_compute<T, SZ, 0, F>(v, j, W);
+    }
+    if constexpr (SZ::NST > 1) {
+        if constexpr (!ODD0) {
+            if constexpr (ONEBUF) lds_barrier();
+            if (a0) stage_write<T, SZ, 0, F, STRIDED, LP>(wr, v, j, line0, dline);
+        }
         lds_barrier();
         rd = wr;
         wr = (wr == X) ? Y : X;
