@@ -727,7 +727,7 @@ hipError_t launch_pass(const Launch& L, const PassArgs<T>& a, bool source, bool 
 // (an FP32 plan computing in FP64).
 int pass_candidates(const AxisFFT& ax, int d, int md, const int64_t dims[3], int nin, size_t csize, bool io32,
                     bool intensity_last, int atomtab_entries, bool allow_split, std::vector<PassPlan>& out,
-                    int mix_role = -1, int nsrc = 0) {
+                    int mix_role = -1, int nsrc = 0, int ncoef = 0) {
     out.clear();
     const int hard = max_lds_bytes();
     const int soft = std::min(hard, 65536);
@@ -762,7 +762,10 @@ int pass_candidates(const AxisFFT& ax, int d, int md, const int64_t dims[3], int
                     if (e.nsrc != 0 && e.nsrc != nsrc) continue;
                     if (rl == kRoleSourceW && (pos == 0) != (kernel_scratch_bytes(e.kernel) <= 32)) continue;
                     const bool src = rl == kRoleSource || rl == kRoleSourceW;
-                    const size_t lds = e.lds + (src ? csize / 2 * (size_t)atomtab_entries : 0);
+                    // the held pass over source groups stages the pass's coefficient tables and
+                    // the tile's mix factors (ncoef (N + nsrc G) values) in LDS
+                    const size_t lds = e.lds + (src ? csize / 2 * (size_t)atomtab_entries : 0) +
+                                       (e.nsrc ? csize * (size_t)ncoef * (size_t)(N + e.nsrc * e.G) : 0);
                     if ((int)lds > hard) continue;
                     if (!contig && !lane_offsets_fit(e.G, 1, row_stride, csize)) continue;
                     PassPlan pp;
@@ -933,6 +936,7 @@ int run_pass(qpd_plan* p, const qpd_fields* f, double* I, void* work, int d, hip
             a.I = I;
         if (d == 1 && p->grouped) {
             a.nsrc = p->nS;
+            a.ncoef = p->nY;
             a.pair_first = tb.pair_first;
             a.pair_y = tb.pair_y;
             a.mixc = static_cast<const cx<T>*>(tb.mixc);
@@ -1366,7 +1370,7 @@ int qpd_plan_create_opts(qpd_plan** plan, const qpd_grid* grid, const qpd_crysta
     for (int d = 0; d < 3 && rc == QPD_OK; ++d) {
         const bool io32 = p->mixed != 0 && p->csz[d] == 16;
         rc = pass_candidates(p->fft[d], d, p->pd[d], p->dims, nin[d], p->csz[d], io32, true, tab_entries, true,
-                             p->cand[d], p->grouped ? mix_role : -1, p->grouped ? p->nS : 0);
+                             p->cand[d], p->grouped ? mix_role : -1, p->grouped ? p->nS : 0, p->grouped ? p->nY : 0);
         if (d == 1 && p->grouped && rc != QPD_OK) {
             // no source-group kernel (direct or split) for this length: keep the class form
             g_error.clear();

@@ -1563,7 +1563,9 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided_mix(PassArgs<T> a) {
 // held values are reused for the F1 C_S.  A thread holds NS RL group values (the last
 // transform of a round runs beside NS - 1 of them), where fast_strided_mix holds NZ 2 RL
 // sums and 2 RL group values; the prefetched line stays in the arrays' own type, and the
-// wavenumbers are read from LDS (a vector load of them would wait for the prefetch too).
+// wavenumbers, the coefficient tables and the tile's mix factors are read from LDS (vector
+// loads of them in the mix waited for the prefetch too: the mix took 0.4 of 0.87 ms).  The
+// launch adds a.ncoef (N + NS G) values to the kernel's LDS.
 template <typename T, class SZ, int G, int NZ, int NS, bool IO32 = false, bool ONEBUF = false>
 __global__ __launch_bounds__(G* SZ::TPL) void fast_mix_held(PassArgs<T> a) {
     using TI = io_t<T, IO32>;
@@ -1576,6 +1578,10 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_mix_held(PassArgs<T> a) {
     cx<T>* bufB = ONEBUF ? bufA : bufA + TILE;
     cx<T>* Wl = bufB + TILE;
     T* kl = reinterpret_cast<T*>(Wl + N);  // the wavenumbers along the pass, read from LDS
+    // the pass's coefficient tables (a.ncoef x N) and the tile's mix factors (a.ncoef x NS x G),
+    // read from LDS by the mix
+    cx<T>* cyl = Wl + N + (N + 1) / 2;  // after the wavenumbers, aligned to a complex value
+    cx<T>* xcl = cyl + (size_t)a.ncoef * N;
     const int tid = threadIdx.x;
     const int g = tid % G, j = tid / G;
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
@@ -1590,9 +1596,14 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_mix_held(PassArgs<T> a) {
     const size_t tile0 = (size_t)t0 * G + (size_t)outer * a.OS;
     const uint32_t in_lane = ((col_ok ? i0 : a.n0 - 1) - t0 * G + jl * a.S) * (uint32_t)sizeof(cx<TI>);
     const uint32_t out_lane = (g + jl * a.S) * (uint32_t)sizeof(cx<TI>);
-    const uint32_t coff = jl * (uint32_t)sizeof(cx<T>);
     load_twiddles<T, SZ>(Wl, a.W, tid, G * SZ::TPL);
     for (int m = tid; m < N; m += G * SZ::TPL) kl[m] = a.k[m];
+    for (int m = tid; m < a.ncoef * N; m += G * SZ::TPL) cyl[m] = a.coef[m];
+    for (int m = tid; m < a.ncoef * NS * G; m += G * SZ::TPL) {
+        const int ys = m / G, c = m - ys * G;
+        const uint32_t col = t0 * G + (uint32_t)c;
+        xcl[m] = col < a.n0 ? a.mixc[(size_t)ys * a.n0 + col] : cx<T>{(T)0, (T)0};
+    }
     lds_barrier();
     // item it -> input field: A_s, B_s for s = it / 2 while it < 2 NS, then C_{it - 2 NS}
     auto load_item = [&](int it, cx<TI>(&dst)[SZ::RMAX]) {
@@ -1622,13 +1633,11 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_mix_held(PassArgs<T> a) {
                 const int p1 = uload(a.pair_first, s * NZ + z + 1);
                 for (int p = uload(a.pair_first, s * NZ + z); p < p1; ++p) {
                     const int y = uload(a.pair_y, p);
-                    const cx<T>* xrow = a.mixc + (size_t)(y * NS + s) * a.n0 + t0 * G;
-                    const cx<T>* cy = a.coef + (size_t)y * a.Nc;
+                    const cx<T>* cy = cyl + y * N;
                     if (j < NRL && col_ok) {
-                        const cx<T> xc = load_at(xrow, g * (uint32_t)sizeof(cx<T>));
+                        const cx<T> xc = xcl[(y * NS + s) * G + g];
 #pragma unroll
-                        for (int r = 0; r < RL; ++r)
-                            acc[r] = acc[r] + cmul(cmul(load_at(cy + (jw + r * NRL), coff), xc), held[s][r]);
+                        for (int r = 0; r < RL; ++r) acc[r] = acc[r] + cmul(cmul(cy[j + r * NRL], xc), held[s][r]);
                     }
                 }
             }

@@ -146,16 +146,16 @@ void add_held(std::vector<FastEntry>& v, const char* radices) {
     constexpr int NZ = ROLE == kRoleMix1 ? 1 : 2;
     constexpr int G = fast_tile(ROLE, SZ::TPL);
     constexpr size_t tile = (size_t)SZ::strided_tile(G), tile64 = (size_t)SZ::strided_tile(G / 2);
-    if constexpr (G >= 1 && (2 * tile + SZ::N) * 8 + SZ::N * 4 <= 160 * 1024) {
-        FastEntry e{SZ::N, ROLE, G, G * SZ::TPL, SZ::TPL, (2 * tile + SZ::N) * 8 + SZ::N * 4, radices,
+    if constexpr (G >= 1 && (2 * tile + SZ::N + (SZ::N + 1) / 2) * 8 <= 160 * 1024) {
+        FastEntry e{SZ::N, ROLE, G, G * SZ::TPL, SZ::TPL, (2 * tile + SZ::N + (SZ::N + 1) / 2) * 8, radices,
                     &held_launch<SZ, NZ, NS, G>, (const void*)fast_mix_held<float, SZ, G, NZ, NS>, false};
         e.nsrc = NS;
         v.push_back(e);
     }
-    if constexpr (G / 2 >= 4 && (G / 2) * SZ::TPL <= 256 && (2 * tile64 + SZ::N) * 16 + SZ::N * 8 <= 160 * 1024) {
+    if constexpr (G / 2 >= 4 && (G / 2) * SZ::TPL <= 256 && (2 * tile64 + SZ::N + (SZ::N + 1) / 2) * 16 <= 160 * 1024) {
         for (const bool io32 : {true, false}) {
             if (!io32 && NS > 2) continue;
-            FastEntry e{SZ::N, ROLE, G / 2, (G / 2) * SZ::TPL, SZ::TPL, (2 * tile64 + SZ::N) * 16 + SZ::N * 8, radices, nullptr,
+            FastEntry e{SZ::N, ROLE, G / 2, (G / 2) * SZ::TPL, SZ::TPL, (2 * tile64 + SZ::N + (SZ::N + 1) / 2) * 16, radices, nullptr,
                         io32 ? (const void*)fast_mix_held<double, SZ, G / 2, NZ, NS, true>
                              : (const void*)fast_mix_held<double, SZ, G / 2, NZ, NS, false>,
                         false, 8, io32 ? &held_launch64<SZ, NZ, NS, G / 2, true> : &held_launch64<SZ, NZ, NS, G / 2, false>,

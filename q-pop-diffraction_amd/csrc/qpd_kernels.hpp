@@ -71,6 +71,7 @@ struct PassArgs {
     // and z-class z the y-classes pair_y[pair_first[s NZ + z] .. pair_first[s NZ + z + 1])
     // contribute with column factor mixc[(y nsrc + s) n0 + i0]
     int nsrc;
+    int ncoef;                // coefficient tables of the pass (the y-classes), staged in LDS by fast_mix_held
     const int* pair_first;
     const int* pair_y;
     const cx<T>* mixc;
