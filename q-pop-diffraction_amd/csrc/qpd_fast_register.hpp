@@ -164,6 +164,15 @@ void add_held(std::vector<FastEntry>& v, const char* radices) {
             v.push_back(e);
         }
     }
+    // FP64 on FP32 arrays at the FP32 tile (rows of 128 bytes read and written, blocks up to
+    // 256 threads)
+    if constexpr (G >= 1 && G * SZ::TPL <= 256 && (2 * tile + SZ::N + (SZ::N + 1) / 2) * 16 <= 160 * 1024) {
+        FastEntry e{SZ::N, ROLE, G, G * SZ::TPL, SZ::TPL, (2 * tile + SZ::N + (SZ::N + 1) / 2) * 16, radices, nullptr,
+                    (const void*)fast_mix_held<double, SZ, G, NZ, NS, true>, false, 8,
+                    &held_launch64<SZ, NZ, NS, G, true>, true};
+        e.nsrc = NS;
+        v.push_back(e);
+    }
 }
 
 // The largest P <= TPL / NRL for which the last stage of a two-stage length splits over P
