@@ -136,9 +136,9 @@ template <class SZ, int NZ, int NS, int G>
 hipError_t held_launch(const Launch& L, const PassArgs<float>& a, hipStream_t st) {
     return launch_one(fast_mix_held<float, SZ, G, NZ, NS>, L, st, a);
 }
-template <class SZ, int NZ, int NS, int G, bool IO32>
+template <class SZ, int NZ, int NS, int G, bool IO32, bool ONEBUF = false>
 hipError_t held_launch64(const Launch& L, const PassArgs<double>& a, hipStream_t st) {
-    return launch_one(fast_mix_held<double, SZ, G, NZ, NS, IO32>, L, st, a);
+    return launch_one(fast_mix_held<double, SZ, G, NZ, NS, IO32, ONEBUF>, L, st, a);
 }
 
 template <class SZ, int ROLE, int NS>
@@ -170,6 +170,16 @@ void add_held(std::vector<FastEntry>& v, const char* radices) {
         FastEntry e{SZ::N, ROLE, G, G * SZ::TPL, SZ::TPL, (2 * tile + SZ::N + (SZ::N + 1) / 2) * 16, radices, nullptr,
                     (const void*)fast_mix_held<double, SZ, G, NZ, NS, true>, false, 8,
                     &held_launch64<SZ, NZ, NS, G, true>, true};
+        e.nsrc = NS;
+        v.push_back(e);
+    }
+    // the same with one LDS stage buffer (a barrier before each stage write too): half the
+    // LDS, for lengths whose blocks LDS rather than registers limits per CU (the 104-point
+    // pass of a 512 x 512 x 104 volume: 0.774 against 0.784 ms, the same blocks per CU)
+    if constexpr (G >= 1 && G * SZ::TPL <= 256 && (tile + SZ::N + (SZ::N + 1) / 2) * 16 <= 160 * 1024) {
+        FastEntry e{SZ::N, ROLE, G, G * SZ::TPL, SZ::TPL, (tile + SZ::N + (SZ::N + 1) / 2) * 16, radices, nullptr,
+                    (const void*)fast_mix_held<double, SZ, G, NZ, NS, true, true>, false, 8,
+                    &held_launch64<SZ, NZ, NS, G, true, true>, true, true};
         e.nsrc = NS;
         v.push_back(e);
     }

@@ -231,6 +231,25 @@ def test_every_schedule(refl, shape):
             assert err <= FP32_PEAK_TOL, (d, name, err)
 
 
+@pytest.mark.parametrize("arith", ["mixed", "fp32"])
+def test_every_schedule_held_pass(refl, arith):
+    """Every schedule of every pass of a plan with pass 1 along 104 points (the held pass over
+    two source groups: eight and sixteen columns a tile, two LDS stage buffers and one; 48
+    columns leave a partial last tile), in FP64 transforms on FP32 arrays (held to the
+    FP32 bounds including |dlog10 I|) and in FP32 transforms (1e-5 of the peak)."""
+    shape = (48, 104, 12)
+    snap = workloads.twin_snapshot(("len",) + shape, *shape, seed=1)
+    plan = PlanCache(32, order="memory", arith=arith).get(snap, 0, refl)
+    names = [plan.schedules(d) for d in range(3)]
+    if arith == "mixed":
+        assert any("G16 groups held2 f64 io32 1buf" in nm for nm in names[1]), names[1]
+    for d in range(3):
+        for idx, name in enumerate(names[d]):
+            print(f"  pass {d} schedule {idx}: {name}", flush=True)
+            err = volume_check(refl, *shape, seed=1, schedules={d: idx}, arith=arith, strict=arith == "mixed")
+            assert err <= FP32_PEAK_TOL, (d, name, err)
+
+
 @pytest.mark.parametrize("shape,axis", [((48, 40, 24), (0, 1, 2)), ((24, 40, 48), (2, 1, 0)), ((104, 64, 30), (0, 1, 2)),
                                         ((24, 1000, 12), (0, 1, 2))])
 def test_source_groups_match_class_form(refl, shape, axis, monkeypatch):
