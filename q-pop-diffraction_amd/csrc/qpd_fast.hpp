@@ -1565,9 +1565,12 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided_mix(PassArgs<T> a) {
 // sums and 2 RL group values; the prefetched line stays in the arrays' own type, and the
 // wavenumbers, the coefficient tables and the tile's mix factors are read from LDS (vector
 // loads of them in the mix waited for the prefetch too: the mix took 0.4 of 0.87 ms).  The
-// launch adds a.ncoef (N + NS G) values to the kernel's LDS.
-template <typename T, class SZ, int G, int NZ, int NS, bool IO32 = false, bool ONEBUF = false>
-__global__ __launch_bounds__(G* SZ::TPL) void fast_mix_held(PassArgs<T> a) {
+// launch adds a.ncoef (N + NS G) values to the kernel's LDS.  PF: the next input is
+// prefetched into registers during the current transform; without it the kernel holds fewer
+// registers and is built for at least three waves per SIMD.
+template <typename T, class SZ, int G, int NZ, int NS, bool IO32 = false, bool ONEBUF = false, bool PF = true>
+__global__ __launch_bounds__(G* SZ::TPL) __attribute__((amdgpu_waves_per_eu(PF ? 1 : 3))) void fast_mix_held(
+    PassArgs<T> a) {
     using TI = io_t<T, IO32>;
     constexpr int N = SZ::N, R0 = SZ::rad(0), NR0 = SZ::nr(0), LP = SZ::strided_lp(G), RL = SZ::RL, NRL = SZ::NRL;
     constexpr int TILE = SZ::strided_tile(G);
@@ -1649,15 +1652,22 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_mix_held(PassArgs<T> a) {
         }
     };
     cx<T> held[NS][RL];
-    cx<TI> nxt[SZ::RMAX];
-    load_item(0, nxt);
+    cx<TI> nxt[PF ? SZ::RMAX : 1];
+    if constexpr (PF) load_item(0, nxt);
     int par = 0;
 #pragma unroll 1
     for (int it = 0; it < NITEMS; ++it) {
         cx<T> v[1][SZ::RMAX];
+        if constexpr (PF) {
 #pragma unroll
-        for (int r = 0; r < SZ::RMAX; ++r) v[0][r] = cx_as<T>(nxt[r]);
-        if (it + 1 < NITEMS) load_item(it + 1, nxt);
+            for (int r = 0; r < SZ::RMAX; ++r) v[0][r] = cx_as<T>(nxt[r]);
+            if (it + 1 < NITEMS) load_item(it + 1, nxt);
+        } else {
+            cx<TI> cur[SZ::RMAX];
+            load_item(it, cur);
+#pragma unroll
+            for (int r = 0; r < SZ::RMAX; ++r) v[0][r] = cx_as<T>(cur[r]);
+        }
         transform<T, SZ, 1, true, LP, false, ONEBUF>(par ? bufB : bufA, par ? bufA : bufB, v, j, g, 0, Wl);
         if (ALT) par ^= 1;
         if (j < NRL) {

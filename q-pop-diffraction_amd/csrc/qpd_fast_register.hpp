@@ -132,13 +132,13 @@ void add_fast64_variants(std::vector<FastEntry>& v, const char* radices) {
 // Pass 1 over exactly NS source groups with the group spectra held (fast_mix_held), listed
 // before the general pass over source groups: FP32 at the strided tile, FP64 (and FP64 on
 // FP32 arrays) at half of it.
-template <class SZ, int NZ, int NS, int G>
+template <class SZ, int NZ, int NS, int G, bool PF = true>
 hipError_t held_launch(const Launch& L, const PassArgs<float>& a, hipStream_t st) {
-    return launch_one(fast_mix_held<float, SZ, G, NZ, NS>, L, st, a);
+    return launch_one(fast_mix_held<float, SZ, G, NZ, NS, false, false, PF>, L, st, a);
 }
-template <class SZ, int NZ, int NS, int G, bool IO32, bool ONEBUF = false>
+template <class SZ, int NZ, int NS, int G, bool IO32, bool ONEBUF = false, bool PF = true>
 hipError_t held_launch64(const Launch& L, const PassArgs<double>& a, hipStream_t st) {
-    return launch_one(fast_mix_held<double, SZ, G, NZ, NS, IO32, ONEBUF>, L, st, a);
+    return launch_one(fast_mix_held<double, SZ, G, NZ, NS, IO32, ONEBUF, PF>, L, st, a);
 }
 
 template <class SZ, int ROLE, int NS>
@@ -180,6 +180,16 @@ void add_held(std::vector<FastEntry>& v, const char* radices) {
         FastEntry e{SZ::N, ROLE, G, G * SZ::TPL, SZ::TPL, (tile + SZ::N + (SZ::N + 1) / 2) * 16, radices, nullptr,
                     (const void*)fast_mix_held<double, SZ, G, NZ, NS, true, true>, false, 8,
                     &held_launch64<SZ, NZ, NS, G, true, true>, true, true};
+        e.nsrc = NS;
+        v.push_back(e);
+    }
+    // FP32 without the register prefetch: 106 registers, four waves per SIMD instead of three
+    // (0.588 against 0.603 ms for the 104-point pass of a 512 x 512 x 104 volume; the FP64 form
+    // at three waves spills and runs 1.07 against 0.77 ms, so it is not listed)
+    if constexpr (G >= 1 && (2 * tile + SZ::N + (SZ::N + 1) / 2) * 8 <= 160 * 1024) {
+        FastEntry e{SZ::N, ROLE, G, G * SZ::TPL, SZ::TPL, (2 * tile + SZ::N + (SZ::N + 1) / 2) * 8, radices,
+                    &held_launch<SZ, NZ, NS, G, false>, (const void*)fast_mix_held<float, SZ, G, NZ, NS, false, false, false>,
+                    true};
         e.nsrc = NS;
         v.push_back(e);
     }
