@@ -768,6 +768,10 @@ int pass_candidates(const AxisFFT& ax, int d, int md, const int64_t dims[3], int
                                        (e.nsrc ? csize * (size_t)ncoef * (size_t)(N + e.nsrc * e.G) : 0);
                     if ((int)lds > hard) continue;
                     if (!contig && !lane_offsets_fit(e.G, 1, row_stride, csize)) continue;
+                    // sixteen-byte staged loads: whole tiles, 16-byte aligned rows and fields
+                    if (e.lds_stage && (dims[0] % e.G != 0 || row_stride % 2 != 0 ||
+                                        (dims[0] * dims[1] * dims[2]) % 2 != 0))
+                        continue;
                     PassPlan pp;
                     set_fast(pp.L[0], &e, lds);
                     pp.L[0].blocks = contig ? (lines + e.G - 1) / e.G : (dims[0] + e.G - 1) / e.G * n_other;
@@ -1142,13 +1146,14 @@ std::string describe_launch(const Launch& L) {
     static const char* role_name[kNumRoles] = {"source", "contiguous", "strided", "intensity", "strided1",
                                                "split-step", "strided", "intensity", "groups", "groups",
                                                "groups", "groups", "source-wave"};
-    if (L.fe)
+    if (L.fe) {
         std::snprintf(buf, sizeof(buf), "fast %d (%s) G%d %s%s%s%s%s%s%s", L.fe->N, L.fe->radices, L.fe->G,
                       role_name[L.fe->role], L.fe->nsrc ? (L.fe->nsrc == 2 ? " held2" : " held3") : "",
                       L.fe->split > 1 ? (" split" + std::to_string(L.fe->split)).c_str() : "",
                       L.fe->no_prefetch ? " np" : "", L.fe->real_bytes == 8 ? " f64" : "", L.fe->io32 ? " io32" : "",
                       L.fe->one_buffer ? " 1buf" : "");
-    else
+        if (L.fe->lds_stage) std::strncat(buf, " ldsin", sizeof(buf) - std::strlen(buf) - 1);
+    } else
         std::snprintf(buf, sizeof(buf), "generic %s%s nl%d nf%d nthr%d ept%d%s%s", L.contiguous ? "contiguous" : "strided",
                       L.mix ? " groups" : "", L.nl, L.nf, L.nthr, L.ept,
                       L.rset == kRadixPow2 ? " pow2" : (L.rset == kRadixSmall ? " small" : ""), L.io32 ? " io32" : "");

@@ -262,6 +262,22 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// The same ordering of LDS accesses without the memory-model fence, whose release part also
+// waits for LDS writes of global_load_lds still in flight (vmcnt): this wave's LDS accesses
+// complete (lgkmcnt(0)), then the workgroup barrier; the memory clobber keeps the compiler
+// from moving memory accesses across it.
+__device__ __forceinline__ void lds_barrier_nofence() {
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+template <bool NOFENCE>
+__device__ __forceinline__ void stage_barrier() {
+    if constexpr (NOFENCE)
+        lds_barrier_nofence();
+    else
+        lds_barrier();
+}
+
 // p[byte_off] with p wave-uniform and byte_off a 32-bit lane offset, as a buffer access: a
 // resource (scalar registers) over the 4 GiB from p and the lane offset as its 32-bit
 // vector operand.  A plain p + byte_off leaves the optimiser free to regroup the sum as
@@ -432,7 +448,7 @@ constexpr int chunk_writes() {
     return SZ::NST - 1 + (LOAD0 ? 1 : 0);
 }
 
-template <typename T, class SZ, int F, bool STRIDED, int LP, bool LOAD0, bool ONEBUF = false>
+template <typename T, class SZ, int F, bool STRIDED, int LP, bool LOAD0, bool ONEBUF = false, bool NOFENCE = false>
 __device__ __forceinline__ void transform(cx<T>* X, cx<T>* Y, cx<T> (&v)[F][SZ::RMAX], int j, int line0, int dline,
                                           const cx<T>* __restrict__ W) {
     static_assert(!(ONEBUF && LOAD0), "one buffer holds the stage-0 inputs only when they are in registers");
@@ -445,17 +461,17 @@ __device__ __forceinline__ void transform(cx<T>* X, cx<T>* Y, cx<T> (&v)[F][SZ::
     // an odd first radix (>= 5) of a multi-stage line writes its outputs as it forms them
     constexpr bool ODD0 = F == 1 && SZ::NST > 1 && (SZ::rad(0) & 1) == 1 && SZ::rad(0) >= 5;
     if constexpr (ODD0) {
-        if constexpr (ONEBUF) lds_barrier();
+        if constexpr (ONEBUF) stage_barrier<NOFENCE>();
         if (a0) stage_odd_to_lds<T, SZ, 0, STRIDED, LP>(wr, v, j, line0, W);
     } else {
         if (a0) stage_compute<T, SZ, 0, F>(v, j, W);
     }
     if constexpr (SZ::NST > 1) {
         if constexpr (!ODD0) {
-            if constexpr (ONEBUF) lds_barrier();
+            if constexpr (ONEBUF) stage_barrier<NOFENCE>();
             if (a0) stage_write<T, SZ, 0, F, STRIDED, LP>(wr, v, j, line0, dline);
         }
-        lds_barrier();
+        stage_barrier<NOFENCE>();
         rd = wr;
         wr = (wr == X) ? Y : X;
         const bool a1 = j < SZ::nr(1);
@@ -465,11 +481,11 @@ __device__ __forceinline__ void transform(cx<T>* X, cx<T>* Y, cx<T> (&v)[F][SZ::
         }
         if constexpr (SZ::NST > 2) {
             if constexpr (ONEBUF) {
-                lds_barrier();
+                stage_barrier<NOFENCE>();
                 wr = X;
             }
             if (a1) stage_write<T, SZ, 1, F, STRIDED, LP>(wr, v, j, line0, dline);
-            lds_barrier();
+            stage_barrier<NOFENCE>();
             rd = wr;
             wr = (wr == X) ? Y : X;
             const bool a2 = j < SZ::nr(2);
@@ -479,11 +495,11 @@ __device__ __forceinline__ void transform(cx<T>* X, cx<T>* Y, cx<T> (&v)[F][SZ::
             }
             if constexpr (SZ::NST > 3) {
                 if constexpr (ONEBUF) {
-                    lds_barrier();
+                    stage_barrier<NOFENCE>();
                     wr = X;
                 }
                 if (a2) stage_write<T, SZ, 2, F, STRIDED, LP>(wr, v, j, line0, dline);
-                lds_barrier();
+                stage_barrier<NOFENCE>();
                 rd = wr;
                 const bool a3 = j < SZ::nr(3);
                 if (a3) {
@@ -1686,6 +1702,137 @@ __global__ __launch_bounds__(G* SZ::TPL) __attribute__((amdgpu_waves_per_eu(PF ?
         if (it == 2 * NS - 1) mix_store(0, held);
         if (it == NITEMS - 1) mix_store(1, held);
     }
+}
+
+// s_waitcnt immediate that waits until at most n vector-memory operations are outstanding
+// (expcnt and lgkmcnt left at their maxima, which wait for nothing)
+constexpr int vmcnt_imm(int n) {
+    return (n & 0xF) | (0x7 << 4) | (0xF << 8) | ((n >> 4) << 14);
+}
+
+// ---------------------------------------------------------------------------
+// Pass 1 over NS source groups as fast_mix_held (FP64 transforms on FP32 spectra, one LDS
+// stage buffer), with the next input staged in LDS instead of registers: each wave loads its
+// rows of the tile with global_load_lds_dwordx4 (sixteen bytes a lane, two columns of a row;
+// a wave's load fills 1 KB of LDS in lane order, eight rows of sixteen columns), so the load
+// of input it + 1 stays in flight across the transform of input it without holding registers.
+// Only whole waves issue these loads (a wave's load writes all 64 lanes' slots; the partial
+// last wave of a 13-row tile has 16 lanes).
+// Every wave waits for its own loads (vmcnt(0)) before the barrier after which the tile is
+// read, and the next loads are issued only after a second barrier, once every wave has read
+// its stage-0 inputs.  Tiles must be whole (n0 a multiple of G) and the row stride even.
+template <class SZ, int G, int NZ, int NS, int WPE>
+__global__ __launch_bounds__(G* SZ::TPL) __attribute__((amdgpu_waves_per_eu(WPE))) void fast_mix_held_lds(
+    PassArgs<double> a) {
+    using T = double;
+    using TI = float;
+    constexpr int N = SZ::N, R0 = SZ::rad(0), NR0 = SZ::nr(0), LP = SZ::strided_lp(G), RL = SZ::RL, NRL = SZ::NRL;
+    constexpr int TILE = SZ::strided_tile(G);
+    constexpr int NITEMS = 3 * NS;
+    static_assert(G == 16, "a staged row is 128 bytes: eight lanes of sixteen bytes");
+    constexpr int RPI = 8;                          // rows per wave load
+    constexpr int NLD = (N + RPI - 1) / RPI;        // wave loads per input
+    constexpr int NWAVES = G * SZ::TPL / 64;         // whole waves, which issue the loads
+    static_assert(NWAVES >= 1, "a whole wave loads the tile");
+    __shared__ __align__(16) cx<TI> stg[NLD * RPI * G];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T>* bufA = reinterpret_cast<cx<T>*>(smem_raw);
+    cx<T>* Wl = bufA + TILE;
+    T* kl = reinterpret_cast<T*>(Wl + N);
+    cx<T>* cyl = Wl + N + (N + 1) / 2;
+    cx<T>* xcl = cyl + (size_t)a.ncoef * N;
+    const int tid = threadIdx.x;
+    const int g = tid % G, j = tid / G;
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t t0 = blk % a.ntiles0;
+    const uint32_t outer = blk / a.ntiles0;
+    static_assert(64 % G == 0, "a wave holds whole rows of the tile");
+    const uint32_t jw = (uint32_t)__builtin_amdgcn_readfirstlane(tid / 64) * (64 / G);
+    const uint32_t jl = (uint32_t)(tid % 64) / G;
+    const size_t tile0 = (size_t)t0 * G + (size_t)outer * a.OS;
+    const uint32_t out_lane = (g + jl * a.S) * (uint32_t)sizeof(cx<TI>);
+    load_twiddles<T, SZ>(Wl, a.W, tid, G * SZ::TPL);
+    for (int m = tid; m < N; m += G * SZ::TPL) kl[m] = a.k[m];
+    for (int m = tid; m < a.ncoef * N; m += G * SZ::TPL) cyl[m] = a.coef[m];
+    for (int m = tid; m < a.ncoef * NS * G; m += G * SZ::TPL) {
+        const int ys = m / G, c = m - ys * G;
+        xcl[m] = a.mixc[(size_t)ys * a.n0 + t0 * G + (uint32_t)c];
+    }
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int ln = tid & 63;
+    // input it -> field: A_s, B_s for s = it / 2 while it < 2 NS, then C_{it - 2 NS}
+    auto issue = [&](int it, cx<TI>* buf) {
+        const int fld = it < 2 * NS ? (it >> 1) * 3 + (it & 1) : (it - 2 * NS) * 3 + 2;
+        const float* in = reinterpret_cast<const float*>(a.in) + 2 * ((size_t)fld * a.ncell + tile0);
+#pragma unroll
+        for (int k0 = 0; k0 < NLD; k0 += NWAVES) {
+            const int k = k0 + wv;
+            const int row = k * RPI + (ln >> 3);
+            if (wv < NWAVES && k < NLD && row < N)
+                __builtin_amdgcn_global_load_lds(
+                    (const void*)(in + 2 * ((size_t)row * a.S + (size_t)((ln & 7) * 2))),
+                    (__attribute__((address_space(3))) void*)(buf + k * RPI * G), 16, 0, 0);
+        }
+    };
+    auto mix_store = [&](int oq, const cx<T>(&held)[NS][RL]) {
+#pragma unroll
+        for (int z = 0; z < NZ; ++z) {
+            cx<T> acc[RL];
+#pragma unroll
+            for (int r = 0; r < RL; ++r) acc[r] = {(T)0, (T)0};
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                const int p1 = uload(a.pair_first, s * NZ + z + 1);
+                for (int p = uload(a.pair_first, s * NZ + z); p < p1; ++p) {
+                    const int y = uload(a.pair_y, p);
+                    const cx<T>* cy = cyl + y * N;
+                    if (j < NRL) {
+                        const cx<T> xc = xcl[(y * NS + s) * G + g];
+#pragma unroll
+                        for (int r = 0; r < RL; ++r) acc[r] = acc[r] + cmul(cmul(cy[j + r * NRL], xc), held[s][r]);
+                    }
+                }
+            }
+            if (j < NRL) {
+                cx<TI>* o = reinterpret_cast<cx<TI>*>(a.out) + (size_t)(2 * z + oq) * a.ncell + tile0;
+#pragma unroll
+                for (int r = 0; r < RL; ++r) store_at(o + (size_t)((jw + r * NRL) * a.S), out_lane, cx_as<TI>(acc[r]));
+            }
+        }
+    };
+    cx<T> held[NS][RL];
+    // input it: wait for this wave's loads of it (vmcnt(0)), then a barrier for every wave's;
+    // read the stage-0 inputs; a barrier before the buffer is loaded with input it + 1;
+    // transform, update the held values, mix and store
+    auto step = [&](int it, cx<TI>* buf) {
+        __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
+        lds_barrier_nofence();
+        cx<T> v[1][SZ::RMAX];
+        if (j < NR0) {
+#pragma unroll
+            for (int r = 0; r < R0; ++r) v[0][r] = cx_as<T>(buf[(j + r * NR0) * G + g]);
+        }
+        lds_barrier_nofence();  // every wave has read the staged tile
+        if (it + 1 < NITEMS) issue(it + 1, buf);
+        transform<T, SZ, 1, true, LP, false, true, true>(bufA, bufA, v, j, g, 0, Wl);
+        if (j < NRL) {
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                if (it == 2 * s || it == 2 * NS + s) {
+#pragma unroll
+                    for (int r = 0; r < RL; ++r) held[s][r] = v[0][r];
+                } else if (it == 2 * s + 1) {
+#pragma unroll
+                    for (int r = 0; r < RL; ++r) held[s][r] = held[s][r] + cscale(negi(v[0][r]), kl[j + r * NRL]);
+                }
+            }
+        }
+        if (it == 2 * NS - 1) mix_store(0, held);
+        if (it == NITEMS - 1) mix_store(1, held);
+    };
+    issue(0, stg);
+#pragma unroll 1
+    for (int it = 0; it < NITEMS; ++it) step(it, stg);
 }
 
 // ---------------------------------------------------------------------------

@@ -141,21 +141,64 @@ hipError_t held_launch64(const Launch& L, const PassArgs<double>& a, hipStream_t
     return launch_one(fast_mix_held<double, SZ, G, NZ, NS, IO32, ONEBUF, PF>, L, st, a);
 }
 
+template <class SZ, int NZ, int NS, int G, int WPE>
+hipError_t held_lds_launch(const Launch& L, const PassArgs<double>& a, hipStream_t st) {
+    return launch_one(fast_mix_held_lds<SZ, G, NZ, NS, WPE>, L, st, a);
+}
+
 template <class SZ, int ROLE, int NS>
 void add_held(std::vector<FastEntry>& v, const char* radices) {
     constexpr int NZ = ROLE == kRoleMix1 ? 1 : 2;
     constexpr int G = fast_tile(ROLE, SZ::TPL);
     constexpr size_t tile = (size_t)SZ::strided_tile(G), tile64 = (size_t)SZ::strided_tile(G / 2);
-    if constexpr (G >= 1 && (2 * tile + SZ::N + (SZ::N + 1) / 2) * 8 <= 160 * 1024) {
-        FastEntry e{SZ::N, ROLE, G, G * SZ::TPL, SZ::TPL, (2 * tile + SZ::N + (SZ::N + 1) / 2) * 8, radices,
+    constexpr size_t tabs = SZ::N + (SZ::N + 1) / 2;  // twiddles and wavenumbers
+    if constexpr (G >= 1 && (2 * tile + tabs) * 8 <= 160 * 1024) {
+        FastEntry e{SZ::N, ROLE, G, G * SZ::TPL, SZ::TPL, (2 * tile + tabs) * 8, radices,
                     &held_launch<SZ, NZ, NS, G>, (const void*)fast_mix_held<float, SZ, G, NZ, NS>, false};
         e.nsrc = NS;
         v.push_back(e);
+        // without the register prefetch: 106 registers, four waves per SIMD instead of three
+        // (0.588 against 0.603 ms for the 104-point pass of a 512 x 512 x 104 volume)
+        FastEntry f{SZ::N, ROLE, G, G * SZ::TPL, SZ::TPL, (2 * tile + tabs) * 8, radices,
+                    &held_launch<SZ, NZ, NS, G, false>, (const void*)fast_mix_held<float, SZ, G, NZ, NS, false, false, false>,
+                    true};
+        f.nsrc = NS;
+        v.push_back(f);
     }
-    if constexpr (G / 2 >= 4 && (G / 2) * SZ::TPL <= 256 && (2 * tile64 + SZ::N + (SZ::N + 1) / 2) * 16 <= 160 * 1024) {
+    // FP64 on FP32 arrays at the FP32 tile (rows of 128 bytes read and written, blocks up to
+    // 256 threads), first with the next input staged in LDS (two groups, whose held values fit
+    // two waves per SIMD; whole tiles only; the 104-point
+    // pass of a 512 x 512 x 104 volume: 0.708 ms against 0.773 with one LDS buffer and a register
+    // prefetch, 0.785 with two buffers, 0.827 at eight columns)
+    if constexpr (NS == 2 && G == 16 && G * SZ::TPL >= 64 && G * SZ::TPL <= 256 && (tile + tabs) * 16 <= 160 * 1024) {
+        FastEntry e{SZ::N, ROLE, G, G * SZ::TPL, SZ::TPL, (tile + tabs) * 16, radices, nullptr,
+                    (const void*)fast_mix_held_lds<SZ, G, NZ, NS, 2>, false, 8, &held_lds_launch<SZ, NZ, NS, G, 2>, true,
+                    true};
+        e.nsrc = NS;
+        e.lds_stage = true;
+        v.push_back(e);
+    }
+    // one LDS stage buffer (a barrier before each stage write too): half the LDS, for lengths
+    // whose blocks LDS rather than registers limits per CU
+    if constexpr (G >= 1 && G * SZ::TPL <= 256 && (tile + tabs) * 16 <= 160 * 1024) {
+        FastEntry e{SZ::N, ROLE, G, G * SZ::TPL, SZ::TPL, (tile + tabs) * 16, radices, nullptr,
+                    (const void*)fast_mix_held<double, SZ, G, NZ, NS, true, true>, false, 8,
+                    &held_launch64<SZ, NZ, NS, G, true, true>, true, true};
+        e.nsrc = NS;
+        v.push_back(e);
+    }
+    if constexpr (G >= 1 && G * SZ::TPL <= 256 && (2 * tile + tabs) * 16 <= 160 * 1024) {
+        FastEntry e{SZ::N, ROLE, G, G * SZ::TPL, SZ::TPL, (2 * tile + tabs) * 16, radices, nullptr,
+                    (const void*)fast_mix_held<double, SZ, G, NZ, NS, true>, false, 8,
+                    &held_launch64<SZ, NZ, NS, G, true>, true};
+        e.nsrc = NS;
+        v.push_back(e);
+    }
+    // FP64 (and FP64 on FP32 arrays) at half the FP32 tile
+    if constexpr (G / 2 >= 4 && (G / 2) * SZ::TPL <= 256 && (2 * tile64 + tabs) * 16 <= 160 * 1024) {
         for (const bool io32 : {true, false}) {
             if (!io32 && NS > 2) continue;
-            FastEntry e{SZ::N, ROLE, G / 2, (G / 2) * SZ::TPL, SZ::TPL, (2 * tile64 + SZ::N + (SZ::N + 1) / 2) * 16, radices, nullptr,
+            FastEntry e{SZ::N, ROLE, G / 2, (G / 2) * SZ::TPL, SZ::TPL, (2 * tile64 + tabs) * 16, radices, nullptr,
                         io32 ? (const void*)fast_mix_held<double, SZ, G / 2, NZ, NS, true>
                              : (const void*)fast_mix_held<double, SZ, G / 2, NZ, NS, false>,
                         false, 8, io32 ? &held_launch64<SZ, NZ, NS, G / 2, true> : &held_launch64<SZ, NZ, NS, G / 2, false>,
@@ -163,35 +206,6 @@ void add_held(std::vector<FastEntry>& v, const char* radices) {
             e.nsrc = NS;
             v.push_back(e);
         }
-    }
-    // FP64 on FP32 arrays at the FP32 tile (rows of 128 bytes read and written, blocks up to
-    // 256 threads)
-    if constexpr (G >= 1 && G * SZ::TPL <= 256 && (2 * tile + SZ::N + (SZ::N + 1) / 2) * 16 <= 160 * 1024) {
-        FastEntry e{SZ::N, ROLE, G, G * SZ::TPL, SZ::TPL, (2 * tile + SZ::N + (SZ::N + 1) / 2) * 16, radices, nullptr,
-                    (const void*)fast_mix_held<double, SZ, G, NZ, NS, true>, false, 8,
-                    &held_launch64<SZ, NZ, NS, G, true>, true};
-        e.nsrc = NS;
-        v.push_back(e);
-    }
-    // the same with one LDS stage buffer (a barrier before each stage write too): half the
-    // LDS, for lengths whose blocks LDS rather than registers limits per CU (the 104-point
-    // pass of a 512 x 512 x 104 volume: 0.774 against 0.784 ms, the same blocks per CU)
-    if constexpr (G >= 1 && G * SZ::TPL <= 256 && (tile + SZ::N + (SZ::N + 1) / 2) * 16 <= 160 * 1024) {
-        FastEntry e{SZ::N, ROLE, G, G * SZ::TPL, SZ::TPL, (tile + SZ::N + (SZ::N + 1) / 2) * 16, radices, nullptr,
-                    (const void*)fast_mix_held<double, SZ, G, NZ, NS, true, true>, false, 8,
-                    &held_launch64<SZ, NZ, NS, G, true, true>, true, true};
-        e.nsrc = NS;
-        v.push_back(e);
-    }
-    // FP32 without the register prefetch: 106 registers, four waves per SIMD instead of three
-    // (0.588 against 0.603 ms for the 104-point pass of a 512 x 512 x 104 volume; the FP64 form
-    // at three waves spills and runs 1.07 against 0.77 ms, so it is not listed)
-    if constexpr (G >= 1 && (2 * tile + SZ::N + (SZ::N + 1) / 2) * 8 <= 160 * 1024) {
-        FastEntry e{SZ::N, ROLE, G, G * SZ::TPL, SZ::TPL, (2 * tile + SZ::N + (SZ::N + 1) / 2) * 8, radices,
-                    &held_launch<SZ, NZ, NS, G, false>, (const void*)fast_mix_held<float, SZ, G, NZ, NS, false, false, false>,
-                    true};
-        e.nsrc = NS;
-        v.push_back(e);
     }
 }
 

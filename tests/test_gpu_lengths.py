@@ -162,6 +162,30 @@ def test_volume_every_length(refl, role, dim, form, arith, monkeypatch):
         assert ran_fast >= len(lengths) // 2
 
 
+def test_held_staged_every_length(refl):
+    """The held pass 1 with its next input staged in LDS (sixteen-byte loads of whole 16-column
+    tiles, FP64 transforms on FP32 spectra) on every pass-1 length that has it, against the
+    oracle at the FP32 bounds including |dlog10 I|; 32 columns make two tiles.  Grids whose
+    columns are not whole tiles do not list it."""
+    ran = 0
+    for N in fast_lengths("strided_groups2"):
+        shape = (32, N, 4)
+        names = PlanCache(32, order="memory", arith="mixed").get(
+            workloads.twin_snapshot(("names",) + shape, *shape, seed=0), 0, refl).schedules(1)
+        if not any(nm.endswith(" ldsin") for nm in names):
+            continue
+        print(f"held staged N={N}", flush=True)
+        err = volume_check(refl, *shape, pick={1: " ldsin"}, arith="mixed", strict=True)
+        assert err <= FP32_PEAK_TOL, (N, err)
+        ran += 1
+        partial = (24, N, 4)
+        pnames = PlanCache(32, order="memory", arith="mixed").get(
+            workloads.twin_snapshot(("names",) + partial, *partial, seed=0), 0, refl).schedules(1)
+        assert not any(nm.endswith(" ldsin") for nm in pnames), (N, pnames)
+    print(f"{ran} lengths with a staged held pass")
+    assert ran >= 4
+
+
 # lengths whose source passes also have a prime-radix stage shared by several lanes per
 # butterfly, with that lane count: 34 = 2 x 17 (eight lanes), 52 = 4 x 13 (three), 68 = 4 x 17
 # (four), 136 = 8 x 17 (two); the one-wave-per-row pass splits its last stage (radix order
