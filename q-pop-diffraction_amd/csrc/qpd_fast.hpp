@@ -1721,10 +1721,10 @@ constexpr int vmcnt_imm(int n) {
 // Every wave waits for its own loads (vmcnt(0)) before the barrier after which the tile is
 // read, and the next loads are issued only after a second barrier, once every wave has read
 // its stage-0 inputs.  Tiles must be whole (n0 a multiple of G) and the row stride even.
-template <class SZ, int G, int NZ, int NS, int WPE>
+// T = float: the same in FP32 throughout.
+template <class SZ, int G, int NZ, int NS, int WPE, typename T = double>
 __global__ __launch_bounds__(G* SZ::TPL) __attribute__((amdgpu_waves_per_eu(WPE))) void fast_mix_held_lds(
-    PassArgs<double> a) {
-    using T = double;
+    PassArgs<T> a) {
     using TI = float;
     constexpr int N = SZ::N, R0 = SZ::rad(0), NR0 = SZ::nr(0), LP = SZ::strided_lp(G), RL = SZ::RL, NRL = SZ::NRL;
     constexpr int TILE = SZ::strided_tile(G);

@@ -145,6 +145,10 @@ template <class SZ, int NZ, int NS, int G, int WPE>
 hipError_t held_lds_launch(const Launch& L, const PassArgs<double>& a, hipStream_t st) {
     return launch_one(fast_mix_held_lds<SZ, G, NZ, NS, WPE>, L, st, a);
 }
+template <class SZ, int NZ, int NS, int G>
+hipError_t held_lds_launch32(const Launch& L, const PassArgs<float>& a, hipStream_t st) {
+    return launch_one(fast_mix_held_lds<SZ, G, NZ, NS, 1, float>, L, st, a);
+}
 
 template <class SZ, int ROLE, int NS>
 void add_held(std::vector<FastEntry>& v, const char* radices) {
@@ -152,6 +156,17 @@ void add_held(std::vector<FastEntry>& v, const char* radices) {
     constexpr int G = fast_tile(ROLE, SZ::TPL);
     constexpr size_t tile = (size_t)SZ::strided_tile(G), tile64 = (size_t)SZ::strided_tile(G / 2);
     constexpr size_t tabs = SZ::N + (SZ::N + 1) / 2;  // twiddles and wavenumbers
+    // FP32, first with the next input staged in LDS (one LDS stage buffer; two groups, whole
+    // tiles only): 0.546 ms against 0.582 without a prefetch and 0.601 with a register prefetch
+    // for the 104-point pass of a 512 x 512 x 104 volume
+    if constexpr (NS == 2 && G == 16 && G * SZ::TPL >= 64 && (tile + tabs) * 8 <= 160 * 1024) {
+        FastEntry e{SZ::N, ROLE, G, G * SZ::TPL, SZ::TPL, (tile + tabs) * 8, radices, &held_lds_launch32<SZ, NZ, NS, G>,
+                    (const void*)fast_mix_held_lds<SZ, G, NZ, NS, 1, float>, false};
+        e.nsrc = NS;
+        e.one_buffer = true;
+        e.lds_stage = true;
+        v.push_back(e);
+    }
     if constexpr (G >= 1 && (2 * tile + tabs) * 8 <= 160 * 1024) {
         FastEntry e{SZ::N, ROLE, G, G * SZ::TPL, SZ::TPL, (2 * tile + tabs) * 8, radices,
                     &held_launch<SZ, NZ, NS, G>, (const void*)fast_mix_held<float, SZ, G, NZ, NS>, false};
@@ -167,9 +182,9 @@ void add_held(std::vector<FastEntry>& v, const char* radices) {
     }
     // FP64 on FP32 arrays at the FP32 tile (rows of 128 bytes read and written, blocks up to
     // 256 threads), first with the next input staged in LDS (two groups, whose held values fit
-    // two waves per SIMD; whole tiles only; the 104-point
-    // pass of a 512 x 512 x 104 volume: 0.708 ms against 0.773 with one LDS buffer and a register
-    // prefetch, 0.785 with two buffers, 0.827 at eight columns)
+    // two waves per SIMD; whole tiles only; the 104-point pass of a 512 x 512 x 104 volume:
+    // 0.708 ms against 0.773 with one LDS buffer and a register prefetch, 0.785 with two
+    // buffers, 0.827 at eight columns)
     if constexpr (NS == 2 && G == 16 && G * SZ::TPL >= 64 && G * SZ::TPL <= 256 && (tile + tabs) * 16 <= 160 * 1024) {
         FastEntry e{SZ::N, ROLE, G, G * SZ::TPL, SZ::TPL, (tile + tabs) * 16, radices, nullptr,
                     (const void*)fast_mix_held_lds<SZ, G, NZ, NS, 2>, false, 8, &held_lds_launch<SZ, NZ, NS, G, 2>, true,
