@@ -768,8 +768,8 @@ int pass_candidates(const AxisFFT& ax, int d, int md, const int64_t dims[3], int
                                        (e.nsrc ? csize * (size_t)ncoef * (size_t)(N + e.nsrc * e.G) : 0);
                     if ((int)lds > hard) continue;
                     if (!contig && !lane_offsets_fit(e.G, 1, row_stride, csize)) continue;
-                    // sixteen-byte staged loads: whole tiles, 16-byte aligned rows and fields
-                    if (e.lds_stage && (dims[0] % e.G != 0 || row_stride % 2 != 0 ||
+                    // sixteen-byte staged loads: 16-byte aligned column pairs, rows and fields
+                    if (e.lds_stage && (dims[0] % 2 != 0 || row_stride % 2 != 0 ||
                                         (dims[0] * dims[1] * dims[2]) % 2 != 0))
                         continue;
                     PassPlan pp;

@@ -1720,7 +1720,7 @@ constexpr int vmcnt_imm(int n) {
 // last wave of a 13-row tile has 16 lanes).
 // Every wave waits for its own loads (vmcnt(0)) before the barrier after which the tile is
 // read, and the next loads are issued only after a second barrier, once every wave has read
-// its stage-0 inputs.  Tiles must be whole (n0 a multiple of G) and the row stride even.
+// its stage-0 inputs.  n0 and the row stride must be even (16-byte aligned column pairs).
 // T = float: the same in FP32 throughout.
 template <class SZ, int G, int NZ, int NS, int WPE, typename T = double>
 __global__ __launch_bounds__(G* SZ::TPL) __attribute__((amdgpu_waves_per_eu(WPE))) void fast_mix_held_lds(
@@ -1751,12 +1751,14 @@ __global__ __launch_bounds__(G* SZ::TPL) __attribute__((amdgpu_waves_per_eu(WPE)
     const uint32_t jl = (uint32_t)(tid % 64) / G;
     const size_t tile0 = (size_t)t0 * G + (size_t)outer * a.OS;
     const uint32_t out_lane = (g + jl * a.S) * (uint32_t)sizeof(cx<TI>);
+    const bool col_ok = t0 * G + (uint32_t)g < a.n0;
     load_twiddles<T, SZ>(Wl, a.W, tid, G * SZ::TPL);
     for (int m = tid; m < N; m += G * SZ::TPL) kl[m] = a.k[m];
     for (int m = tid; m < a.ncoef * N; m += G * SZ::TPL) cyl[m] = a.coef[m];
     for (int m = tid; m < a.ncoef * NS * G; m += G * SZ::TPL) {
         const int ys = m / G, c = m - ys * G;
-        xcl[m] = a.mixc[(size_t)ys * a.n0 + t0 * G + (uint32_t)c];
+        const uint32_t col = t0 * G + (uint32_t)c;
+        xcl[m] = col < a.n0 ? a.mixc[(size_t)ys * a.n0 + col] : cx<T>{(T)0, (T)0};
     }
     const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int ln = tid & 63;
@@ -1768,10 +1770,12 @@ __global__ __launch_bounds__(G* SZ::TPL) __attribute__((amdgpu_waves_per_eu(WPE)
         for (int k0 = 0; k0 < NLD; k0 += NWAVES) {
             const int k = k0 + wv;
             const int row = k * RPI + (ln >> 3);
-            if (wv < NWAVES && k < NLD && row < N)
-                __builtin_amdgcn_global_load_lds(
-                    (const void*)(in + 2 * ((size_t)row * a.S + (size_t)((ln & 7) * 2))),
-                    (__attribute__((address_space(3))) void*)(buf + k * RPI * G), 16, 0, 0);
+            // the columns of a partial last tile beyond n0 (even) are not loaded: their slots
+            // keep what they held, their results are not stored
+            const uint32_t c2 = (uint32_t)(ln & 7) * 2;
+            if (wv < NWAVES && k < NLD && row < N && t0 * G + c2 < a.n0)
+                __builtin_amdgcn_global_load_lds((const void*)(in + 2 * ((size_t)row * a.S + (size_t)c2)),
+                                                 (__attribute__((address_space(3))) void*)(buf + k * RPI * G), 16, 0, 0);
         }
     };
     auto mix_store = [&](int oq, const cx<T>(&held)[NS][RL]) {
@@ -1793,7 +1797,7 @@ __global__ __launch_bounds__(G* SZ::TPL) __attribute__((amdgpu_waves_per_eu(WPE)
                     }
                 }
             }
-            if (j < NRL) {
+            if (j < NRL && col_ok) {
                 cx<TI>* o = reinterpret_cast<cx<TI>*>(a.out) + (size_t)(2 * z + oq) * a.ncell + tile0;
 #pragma unroll
                 for (int r = 0; r < RL; ++r) store_at(o + (size_t)((jw + r * NRL) * a.S), out_lane, cx_as<TI>(acc[r]));

@@ -156,8 +156,8 @@ void add_held(std::vector<FastEntry>& v, const char* radices) {
     constexpr int G = fast_tile(ROLE, SZ::TPL);
     constexpr size_t tile = (size_t)SZ::strided_tile(G), tile64 = (size_t)SZ::strided_tile(G / 2);
     constexpr size_t tabs = SZ::N + (SZ::N + 1) / 2;  // twiddles and wavenumbers
-    // FP32, first with the next input staged in LDS (one LDS stage buffer; two groups, whole
-    // tiles only): 0.546 ms against 0.582 without a prefetch and 0.601 with a register prefetch
+    // FP32, first with the next input staged in LDS (one LDS stage buffer; two groups, an even
+    // number of columns): 0.546 ms against 0.582 without a prefetch and 0.601 with a register prefetch
     // for the 104-point pass of a 512 x 512 x 104 volume
     if constexpr (NS == 2 && G == 16 && G * SZ::TPL >= 64 && (tile + tabs) * 8 <= 160 * 1024) {
         FastEntry e{SZ::N, ROLE, G, G * SZ::TPL, SZ::TPL, (tile + tabs) * 8, radices, &held_lds_launch32<SZ, NZ, NS, G>,
@@ -182,7 +182,7 @@ void add_held(std::vector<FastEntry>& v, const char* radices) {
     }
     // FP64 on FP32 arrays at the FP32 tile (rows of 128 bytes read and written, blocks up to
     // 256 threads), first with the next input staged in LDS (two groups, whose held values fit
-    // two waves per SIMD; whole tiles only; the 104-point pass of a 512 x 512 x 104 volume:
+    // two waves per SIMD; an even number of columns; the 104-point pass of a 512 x 512 x 104 volume:
     // 0.708 ms against 0.773 with one LDS buffer and a register prefetch, 0.785 with two
     // buffers, 0.827 at eight columns)
     if constexpr (NS == 2 && G == 16 && G * SZ::TPL >= 64 && G * SZ::TPL <= 256 && (tile + tabs) * 16 <= 160 * 1024) {

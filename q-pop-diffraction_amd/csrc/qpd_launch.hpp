@@ -80,7 +80,7 @@ struct FastEntry {
     bool one_buffer = false;  // strided FP64 kernel with one LDS stage buffer
     int nsrc = 0;             // pass 1 over source groups: the group count it requires (0: any)
     int split = 1;            // one-wave-per-row source pass: lanes per last-stage butterfly
-    bool lds_stage = false;   // inputs staged in LDS by sixteen-byte loads: whole tiles, even strides
+    bool lds_stage = false;   // inputs staged in LDS by sixteen-byte loads: even n0 and strides
 };
 
 // Raise a kernel's dynamic LDS limit once per (kernel, device) and size.
