@@ -792,6 +792,9 @@ int pass_candidates(const AxisFFT& ax, int d, int md, const int64_t dims[3], int
                     for (const FastEntry& ea : fast_registry()) {
                         if (ea.N != n2 || ea.role != kRoleSplit || (int)ea.lds > hard) continue;
                         if ((size_t)ea.real_bytes * 2 != csize || ea.io32 != split64) continue;
+                        if (ea.lds_stage && (dims[0] % 2 != 0 || row_stride % 2 != 0 ||
+                                             (dims[0] * dims[1] * dims[2]) % 2 != 0))
+                            continue;
                         if (!lane_offsets_fit(ea.G, n1, row_stride, csize)) continue;
                         const int role_b = role == kRoleStrided     ? kRoleStridedB
                                            : role == kRoleIntensity ? kRoleIntensityB

@@ -1847,16 +1847,27 @@ __global__ __launch_bounds__(G* SZ::TPL) __attribute__((amdgpu_waves_per_eu(WPE)
 // Each block reads all of a line before it writes it, and blocks own disjoint lines.
 // IO32 (FP64 only): reads the FP32 fields at a.in and writes the FP64 results at the same
 // places of a.out (a separate buffer), so the intermediate keeps FP64 precision.
-template <typename T, class SZ, int G, bool IO32 = false>
+// LDSIN (FP32 arrays in, G >= 2): the next field is staged in LDS by sixteen-byte
+// global_load_lds (two columns of a row a lane, as fast_mix_held_lds does) instead of
+// registers, and the block's twiddles W^{q k2} are read from LDS; n0 and the strides even.
+template <typename T, class SZ, int G, bool IO32 = false, bool LDSIN = false>
 __global__ __launch_bounds__(G* SZ::TPL) void fast_split(PassArgs<T> a) {
     using TI = io_t<T, IO32>;
     constexpr int N = SZ::N, R0 = SZ::rad(0), NR0 = SZ::nr(0), LP = SZ::strided_lp(G);
     constexpr int TILE = SZ::strided_tile(G);
-    constexpr bool ALT = (chunk_writes<SZ, false>() & 1) == 1;
+    constexpr bool ALT = !LDSIN && (chunk_writes<SZ, false>() & 1) == 1;
+    static_assert(!LDSIN || (std::is_same<TI, float>::value && G >= 2 && G <= 16), "staged FP32 rows");
+    constexpr int LPR = G / 2 > 0 ? G / 2 : 1;   // lanes per staged row (sixteen bytes each)
+    constexpr int RPI = 64 / LPR;                // staged rows per wave load
+    constexpr int NLD = (N + RPI - 1) / RPI;     // wave loads per field
+    constexpr int NWL = G * SZ::TPL / 64;        // whole waves, which issue them
+    static_assert(!LDSIN || NWL >= 1, "a whole wave loads the tile");
+    __shared__ __align__(16) cx<float> stg[LDSIN ? NLD * RPI * G : 1];
+    __shared__ __align__(16) cx<T> wsl[LDSIN ? N : 1];
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* bufA = reinterpret_cast<cx<T>*>(smem_raw);
-    cx<T>* bufB = bufA + TILE;
-    cx<T>* Wl = bufB + TILE;
+    cx<T>* bufB = LDSIN ? bufA : bufA + TILE;
+    cx<T>* Wl = bufA + (LDSIN ? 1 : 2) * TILE;
     const int tid = threadIdx.x;
     const int g = tid % G, j = tid / G;
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
@@ -1874,8 +1885,11 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_split(PassArgs<T> a) {
     const size_t tile0 = (size_t)t0 * G + (size_t)outer * a.OS + (size_t)q * a.in_qmul * a.S;
     const uint32_t in_lane = ((col_ok ? i0 : a.n0 - 1) - t0 * G + jl * rstride) * (uint32_t)sizeof(cx<TI>);
     load_twiddles<T, SZ>(Wl, a.W, tid, G * SZ::TPL);
+    if constexpr (LDSIN) {
+        for (int m = tid; m < N; m += G * SZ::TPL) wsl[m] = a.Wsplit[q * (uint32_t)m];
+    }
     lds_barrier();
-    constexpr bool PF = true;
+    constexpr bool PF = !LDSIN;
     const cx<TI>* src = IO32 ? reinterpret_cast<const cx<TI>*>(a.in) : reinterpret_cast<const cx<TI>*>(a.out);
     auto load_field = [&](int f, cx<TI>(&dst)[SZ::RMAX]) {
         const cx<TI>* in = src + (size_t)f * a.ncell + tile0;
@@ -1884,12 +1898,38 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_split(PassArgs<T> a) {
             for (int r = 0; r < R0; ++r) dst[r] = load_at(in + (size_t)(jw + r * NR0) * rstride, in_lane);
         }
     };
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int ln = tid & 63;
+    // LDSIN: the wave loads of field f into stg (rows k RPI + ln / LPR, columns 2 (ln % LPR));
+    // the columns of a partial last tile beyond n0 (even) are not loaded, nor stored
+    auto issue = [&](int f) {
+        const float* in = reinterpret_cast<const float*>(src) + 2 * ((size_t)f * a.ncell + tile0);
+#pragma unroll
+        for (int k0 = 0; k0 < NLD; k0 += NWL) {
+            const int k = k0 + wv;
+            const int row = k * RPI + ln / LPR;
+            const uint32_t c2 = (uint32_t)(ln % LPR) * 2;
+            if (wv < NWL && k < NLD && row < N && t0 * G + c2 < a.n0)
+                __builtin_amdgcn_global_load_lds((const void*)(in + 2 * ((size_t)row * rstride + (size_t)c2)),
+                                                 (__attribute__((address_space(3))) void*)(stg + k * RPI * G), 16, 0, 0);
+        }
+    };
     cx<TI> nxt[SZ::RMAX];
     if constexpr (PF) load_field(0, nxt);
+    if constexpr (LDSIN) issue(0);
     int par = 0;
     for (int f = 0; f < a.nfields; ++f) {
         cx<T> v[1][SZ::RMAX];
-        if constexpr (PF) {
+        if constexpr (LDSIN) {
+            __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));  // this wave's loads of field f (and its stores)
+            lds_barrier_nofence();                      // and every wave's
+            if (j < NR0) {
+#pragma unroll
+                for (int r = 0; r < R0; ++r) v[0][r] = cx_as<T>(stg[(j + r * NR0) * G + g]);
+            }
+            lds_barrier_nofence();  // every wave has read the staged tile
+            if (f + 1 < a.nfields) issue(f + 1);
+        } else if constexpr (PF) {
 #pragma unroll
             for (int r = 0; r < SZ::RMAX; ++r) v[0][r] = cx_as<T>(nxt[r]);
             if (f + 1 < a.nfields) load_field(f + 1, nxt);
@@ -1899,7 +1939,7 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_split(PassArgs<T> a) {
 #pragma unroll
             for (int r = 0; r < SZ::RMAX; ++r) v[0][r] = cx_as<T>(cur[r]);
         }
-        transform<T, SZ, 1, true, LP, false>(par ? bufB : bufA, par ? bufA : bufB, v, j, g, 0, Wl);
+        transform<T, SZ, 1, true, LP, false, LDSIN, LDSIN>(par ? bufB : bufA, par ? bufA : bufB, v, j, g, 0, Wl);
         if (ALT) par ^= 1;
         if (col_ok && j < SZ::NRL) {
             cx<T>* o = a.out + (size_t)f * a.ncell + tile0;
@@ -1909,7 +1949,8 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_split(PassArgs<T> a) {
             for (int r = 0; r < SZ::RL; ++r) {
                 // row k2 = j + r NRL: its twiddle W^{q k2} and its output row
                 const uint32_t k2u = jw + (uint32_t)(r * SZ::NRL);
-                store_at(o + (size_t)k2u * rstride, out_lane, cmul(v[0][r], load_at(a.Wsplit + q * k2u, w_lane)));
+                const cx<T> w = LDSIN ? wsl[j + r * SZ::NRL] : load_at(a.Wsplit + q * k2u, w_lane);
+                store_at(o + (size_t)k2u * rstride, out_lane, cmul(v[0][r], w));
             }
         }
     }

@@ -47,6 +47,34 @@ hipError_t fast_launch(const Launch& L, const PassArgs<float>& a, hipStream_t st
         return launch_one(fast_strided_mix<float, SZ, G, 2, true>, L, st, a);
 }
 
+// The first step of a split strided pass with the next field staged in LDS (fast_split
+// LDSIN): FP32, and FP64 on FP32 arrays; listed before the register-prefetch kernels.
+template <class SZ, int G>
+hipError_t split_lds_launch(const Launch& L, const PassArgs<float>& a, hipStream_t st) {
+    return launch_one(fast_split<float, SZ, G, false, true>, L, st, a);
+}
+template <class SZ, int G>
+hipError_t split_lds_launch64(const Launch& L, const PassArgs<double>& a, hipStream_t st) {
+    return launch_one(fast_split<double, SZ, G, true, true>, L, st, a);
+}
+template <class SZ, int G, bool F64>
+void add_split_lds(std::vector<FastEntry>& v, const char* radices) {
+    if constexpr (G >= 2 && G <= 16 && G * SZ::TPL >= 64 && G * SZ::TPL <= (F64 ? 256 : 512)) {
+        constexpr size_t cs = F64 ? 16 : 8;
+        constexpr size_t dyn = ((size_t)SZ::strided_tile(G) + SZ::N) * cs;
+        constexpr int RPI = 64 / (G / 2);
+        constexpr size_t stat = (size_t)((SZ::N + RPI - 1) / RPI) * RPI * G * 8 + (size_t)SZ::N * cs;
+        if constexpr (dyn + stat <= 160 * 1024) {
+            FastEntry e{SZ::N, kRoleSplit, G, G * SZ::TPL, SZ::TPL, dyn, radices,
+                        F64 ? nullptr : &split_lds_launch<SZ, G>,
+                        F64 ? (const void*)fast_split<double, SZ, G, true, true> : (const void*)fast_split<float, SZ, G, false, true>,
+                        false, F64 ? 8 : 4, F64 ? &split_lds_launch64<SZ, G> : nullptr, F64, true};
+            e.lds_stage = true;
+            v.push_back(e);
+        }
+    }
+}
+
 template <class SZ, int ROLE, int G, bool PF = true>
 void add_fast(std::vector<FastEntry>& v, const char* radices) {
     if constexpr (G >= 1) {
@@ -382,7 +410,10 @@ void add_fast_roles(std::vector<FastEntry>& v, const char* radices) {
     if constexpr ((ROLES >> kRoleIntensity) & 1u) add_fast_variants<SZ, kRoleIntensity>(v, radices);
     if constexpr ((ROLES >> kRoleStrided1) & 1u)
         add_fast<SZ, kRoleStrided1, fast_tile(kRoleStrided1, SZ::TPL)>(v, radices);
-    if constexpr ((ROLES >> kRoleSplit) & 1u) add_fast<SZ, kRoleSplit, fast_tile(kRoleSplit, SZ::TPL)>(v, radices);
+    if constexpr ((ROLES >> kRoleSplit) & 1u) {
+        add_split_lds<SZ, fast_tile(kRoleSplit, SZ::TPL), false>(v, radices);
+        add_fast<SZ, kRoleSplit, fast_tile(kRoleSplit, SZ::TPL)>(v, radices);
+    }
     if constexpr ((ROLES >> kRoleStridedB) & 1u)
         add_fast<SZ, kRoleStridedB, fast_tile(kRoleStridedB, SZ::TPL)>(v, radices);
     if constexpr ((ROLES >> kRoleIntensityB) & 1u)
@@ -404,8 +435,10 @@ void add_fast_roles(std::vector<FastEntry>& v, const char* radices) {
     if constexpr ((ROLES >> kRoleMix2) & 1u) add_fast64_variants<SZ, kRoleMix2>(v, radices);
     // FP64 split steps of the intensity pass of an FP32 plan computing in FP64: the first
     // reads the FP32 spectra and writes an FP64 intermediate (io32), the second reads that
-    if constexpr ((ROLES >> kRoleSplit) & 1u)
+    if constexpr ((ROLES >> kRoleSplit) & 1u) {
+        add_split_lds<SZ, fast_tile(kRoleSplit, SZ::TPL) / 2, true>(v, radices);
         add_fast64<SZ, kRoleSplit, fast_tile(kRoleSplit, SZ::TPL) / 2, true, true>(v, radices);
+    }
     if constexpr ((ROLES >> kRoleIntensityB) & 1u)
         add_fast64<SZ, kRoleIntensityB, fast_tile(kRoleIntensityB, SZ::TPL) / 2, true, false>(v, radices);
     if constexpr ((ROLES >> kRoleMix1B) & 1u) add_fast<SZ, kRoleMix1B, fast_tile(kRoleMix1B, SZ::TPL)>(v, radices);

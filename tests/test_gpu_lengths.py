@@ -223,6 +223,26 @@ def test_split_long_strided(refl, N, dim):
     assert err <= FP32_PEAK_TOL
 
 
+@pytest.mark.parametrize("arith,dim,N", [("fp32", 1, 1000), ("fp32", 1, 4000), ("fp32", 2, 4000),
+                                         ("mixed", 2, 2000), ("mixed", 2, 4000)])
+@pytest.mark.parametrize("nx", [6, 18])
+def test_split_staged(refl, arith, dim, N, nx):
+    """Split long strided passes whose first step stages its next field in LDS (an even number
+    of columns: 6 is one partial tile, 18 a whole one and a partial one for the FP64 step's
+    eight columns), in FP32 and as the FP64 intensity split of FP64 transforms on FP32 spectra,
+    against the oracle (FP64 transforms also at |dlog10 I| <= 1e-3)."""
+    shape = [nx, 4, 3]
+    shape[dim] = N
+    print(f"staged split N={N} dim {dim} {arith} grid {shape}", flush=True)
+    err = volume_check(refl, *shape, expect={dim: "split"}, arith=arith, strict=arith == "mixed")
+    snap = workloads.twin_snapshot(("len",) + tuple(shape), *shape, seed=0)
+    plan = PlanCache(32, order="memory", arith=arith).get(snap, 0, refl)
+    name = plan.schedules(dim)[plan.schedule(dim)]
+    print(f"  {name}: max|dI|/peak {err:.2e}")
+    assert "ldsin" in name, name
+    assert err <= FP32_PEAK_TOL
+
+
 @pytest.mark.parametrize("N", [2000, 4000])
 def test_split_intensity_fp64(refl, N):
     """The intensity pass of a plan computing in FP64 on FP32 spectra, split in two launches
