@@ -803,6 +803,9 @@ int pass_candidates(const AxisFFT& ax, int d, int md, const int64_t dims[3], int
                         for (const FastEntry& eb : fast_registry()) {
                             if (eb.N != n1 || eb.role != role_b || (int)eb.lds > hard) continue;
                             if ((size_t)eb.real_bytes * 2 != csize || eb.io32) continue;
+                            if (eb.lds_stage && (dims[0] % 2 != 0 || row_stride % 2 != 0 ||
+                                                 (dims[0] * dims[1] * dims[2]) % 2 != 0))
+                                continue;
                             if (!lane_offsets_fit(eb.G, n2, row_stride, csize)) continue;
                             PassPlan pp;
                             pp.nlaunch = 2;

@@ -270,6 +270,12 @@ __device__ __forceinline__ void lds_barrier_nofence() {
     asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
+// s_waitcnt immediate that waits until at most n vector-memory operations are outstanding
+// (expcnt and lgkmcnt left at their maxima, which wait for nothing)
+constexpr int vmcnt_imm(int n) {
+    return (n & 0xF) | (0x7 << 4) | (0xF << 8) | ((n >> 4) << 14);
+}
+
 template <bool NOFENCE>
 __device__ __forceinline__ void stage_barrier() {
     if constexpr (NOFENCE)
@@ -1315,10 +1321,22 @@ __global__ __launch_bounds__(64 * kSrcWaves) __attribute__((amdgpu_waves_per_eu(
 // without it the kernel needs fewer registers and more blocks fit on a CU.
 // IO32 (FP64 only): FP64 arithmetic on complex FP32 inputs and outputs.
 // ONEBUF: one LDS stage buffer (transform's ONEBUF), for more blocks per CU.
+// LDSIN (FP64, one LDS buffer): the next input is staged in LDS by sixteen-byte
+// global_load_lds (as fast_mix_held_lds does) instead of registers; the item's coefficients
+// and wavenumbers are loaded into registers before the next input's loads are issued, so
+// that waiting for them does not wait for those.  n0 and the strides even.
 template <typename T, class SZ, int G, int NIN, bool INTENSITY, bool SPLIT, bool PF = true, bool IO32 = false,
-          bool ONEBUF = false>
+          bool ONEBUF = false, bool LDSIN = false>
 __global__ __launch_bounds__(G* SZ::TPL) void fast_strided(PassArgs<T> a) {
     using TI = io_t<T, IO32>;
+    static_assert(!LDSIN || (std::is_same<T, double>::value && ONEBUF && !PF), "staged inputs: FP64, one buffer");
+    constexpr int EPL = 16 / (int)sizeof(cx<TI>);          // elements a staged lane loads
+    constexpr int LPR = G / EPL > 0 ? G / EPL : 1;          // lanes per staged row
+    constexpr int RPI = 64 / LPR;                           // staged rows per wave load
+    constexpr int NLD = (SZ::N + RPI - 1) / RPI;            // wave loads per input
+    constexpr int NWL = G * SZ::TPL / 64;                   // whole waves, which issue them
+    static_assert(!LDSIN || (G % EPL == 0 && NWL >= 1), "staged rows of whole lanes, loaded by whole waves");
+    __shared__ __align__(16) cx<TI> stg[LDSIN ? NLD * RPI * G : 1];
     // the NIN inputs of a chunk are transformed one after another (one line per thread)
     constexpr int N = SZ::N, R0 = SZ::rad(0), NR0 = SZ::nr(0), LP = SZ::strided_lp(G);
     constexpr int NOUT = NIN > 1 ? NIN - 1 : 1;
@@ -1370,6 +1388,25 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided(PassArgs<T> a) {
     };
     cx<TI> nxt[SZ::RMAX];
     if constexpr (PF) load_item(0, nxt);
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int ln = tid & 63;
+    // LDSIN: the wave loads of input it2 into stg (rows k RPI + ln / LPR, EPL columns from
+    // EPL (ln % LPR)); the columns of a partial last tile beyond n0 are not loaded
+    auto issue = [&](int it2) {
+        const int c = it2 / NIN, f = it2 - (it2 / NIN) * NIN;
+        const TI* in = reinterpret_cast<const TI*>(a.in) + 2 * (((size_t)uload(a.chunk_src, c) * a.nin + f) * a.ncell + tile0);
+#pragma unroll
+        for (int k0 = 0; k0 < NLD; k0 += NWL) {
+            const int k = k0 + wv;
+            const int row = k * RPI + ln / LPR;
+            const uint32_t col = (uint32_t)(ln % LPR) * EPL;
+            if (wv < NWL && k < NLD && row < SZ::N && t0 * G + col < a.n0)
+                __builtin_amdgcn_global_load_lds(
+                    (const void*)(in + 2 * ((size_t)((in0 + (uint32_t)row * in_step) * a.S) + (size_t)col)),
+                    (__attribute__((address_space(3))) void*)(stg + k * RPI * G), 16, 0, 0);
+        }
+    };
+    if constexpr (LDSIN) issue(0);
     int it = 0;
     int par = 0;
     for (int grp = 0; grp < a.ngroups; ++grp) {
@@ -1384,7 +1421,28 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided(PassArgs<T> a) {
 #pragma unroll
             for (int f = 0; f < NIN; ++f, ++it) {
                 cx<T> v[1][SZ::RMAX];
-                if constexpr (PF) {
+                cx<T> cfv[LDSIN ? SZ::RL : 1];
+                T kv[LDSIN ? SZ::RL : 1];
+                if constexpr (LDSIN) {
+                    __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));  // this wave's loads of input it
+                    lds_barrier_nofence();                      // and every wave's
+                    if (j < NR0) {
+#pragma unroll
+                        for (int r = 0; r < R0; ++r) v[0][r] = cx_as<T>(stg[(j + r * NR0) * G + g]);
+                    }
+                    if (j < SZ::NRL) {
+                        const uint32_t coff = c_lane * (uint32_t)sizeof(cx<T>);
+                        const uint32_t koff = c_lane * (uint32_t)sizeof(T);
+#pragma unroll
+                        for (int r = 0; r < SZ::RL; ++r) {
+                            const uint32_t iu = c0 + (jw + r * SZ::NRL) * c_step;
+                            cfv[r] = ct >= 0 ? load_at(coef + iu, coff) : cx<T>{(T)1, (T)0};
+                            kv[r] = f == 1 ? load_at(a.k + iu, koff) : (T)0;
+                        }
+                    }
+                    lds_barrier_nofence();  // every wave has read the staged tile
+                    if (it + 1 < nitems) issue(it + 1);
+                } else if constexpr (PF) {
 #pragma unroll
                     for (int r = 0; r < SZ::RMAX; ++r) v[0][r] = cx_as<T>(nxt[r]);
                     if (it + 1 < nitems) load_item(it + 1, nxt);
@@ -1397,7 +1455,7 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided(PassArgs<T> a) {
                             v[0][r] = cx_as<T>(load_at(in + (size_t)((in0 + (jw + r * NR0) * in_step) * a.S), in_lane));
                     }
                 }
-                transform<T, SZ, 1, true, LP, false, ONEBUF>(par ? bufB : bufA, par ? bufA : bufB, v, j, g, 0, Wl);
+                transform<T, SZ, 1, true, LP, false, ONEBUF, LDSIN>(par ? bufB : bufA, par ? bufA : bufB, v, j, g, 0, Wl);
                 if (ALT) par ^= 1;
                 if (j < SZ::NRL) {
                     const uint32_t coff = c_lane * (uint32_t)sizeof(cx<T>);
@@ -1406,9 +1464,9 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided(PassArgs<T> a) {
                     for (int r = 0; r < SZ::RL; ++r) {
                         // coefficient index c0 + (j + r NRL) c_step, uniform part + lane part
                         const uint32_t iu = c0 + (jw + r * SZ::NRL) * c_step;
-                        const cx<T> cf = ct >= 0 ? load_at(coef + iu, coff) : cx<T>{(T)1, (T)0};
+                        const cx<T> cf = LDSIN ? cfv[r] : (ct >= 0 ? load_at(coef + iu, coff) : cx<T>{(T)1, (T)0});
                         cx<T> t = cmul(cf, v[0][r]);
-                        if (f == 1) t = cscale(negi(t), load_at(a.k + iu, koff));
+                        if (f == 1) t = cscale(negi(t), LDSIN ? kv[r] : load_at(a.k + iu, koff));
                         const int oq = f <= 1 ? 0 : f - 1;
                         acc[oq][r] = acc[oq][r] + t;
                     }
@@ -1702,12 +1760,6 @@ __global__ __launch_bounds__(G* SZ::TPL) __attribute__((amdgpu_waves_per_eu(PF ?
         if (it == 2 * NS - 1) mix_store(0, held);
         if (it == NITEMS - 1) mix_store(1, held);
     }
-}
-
-// s_waitcnt immediate that waits until at most n vector-memory operations are outstanding
-// (expcnt and lgkmcnt left at their maxima, which wait for nothing)
-constexpr int vmcnt_imm(int n) {
-    return (n & 0xF) | (0x7 << 4) | (0xF << 8) | ((n >> 4) << 14);
 }
 
 // ---------------------------------------------------------------------------

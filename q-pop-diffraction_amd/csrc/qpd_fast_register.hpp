@@ -114,6 +114,27 @@ hipError_t fast_launch64(const Launch& L, const PassArgs<double>& a, hipStream_t
         return launch_one(fast_strided_mix<double, SZ, G, 2, false, IO32, ONEBUF>, L, st, a);
 }
 
+// The FP64 intensity pass (IO32: on FP32 spectra) or the second step of its split (on the
+// FP64 intermediate) with the next input staged in LDS (fast_strided LDSIN, one LDS buffer).
+template <class SZ, int ROLE, int G, bool IO32>
+hipError_t strided_lds_launch64(const Launch& L, const PassArgs<double>& a, hipStream_t st) {
+    return launch_one(fast_strided<double, SZ, G, 2, true, ROLE == kRoleIntensityB, false, IO32, true, true>, L, st, a);
+}
+template <class SZ, int ROLE, int G, bool IO32>
+void add_strided_lds(std::vector<FastEntry>& v, const char* radices) {
+    constexpr size_t ce = IO32 ? 8 : 16;                       // bytes of a staged element
+    constexpr int EPL = 16 / (int)ce, LPR = G / EPL > 0 ? G / EPL : 1, RPI = 64 / LPR;
+    constexpr size_t dyn = ((size_t)SZ::strided_tile(G) + SZ::N) * 16;
+    constexpr size_t stat = (size_t)((SZ::N + RPI - 1) / RPI) * RPI * G * ce;
+    if constexpr (G >= 1 && G % EPL == 0 && G * SZ::TPL >= 64 && G * SZ::TPL <= 256 && dyn + stat <= 160 * 1024) {
+        FastEntry e{SZ::N, ROLE, G, G * SZ::TPL, SZ::TPL, dyn, radices, nullptr,
+                    (const void*)fast_strided<double, SZ, G, 2, true, ROLE == kRoleIntensityB, false, IO32, true, true>,
+                    true, 8, &strided_lds_launch64<SZ, ROLE, G, IO32>, IO32, true};
+        e.lds_stage = true;
+        v.push_back(e);
+    }
+}
+
 // ONEBUF (strided roles): one LDS stage buffer instead of two
 template <class SZ, int ROLE, int G, bool PF = true, bool IO32 = false, bool ONEBUF = false>
 void add_fast64(std::vector<FastEntry>& v, const char* radices) {
@@ -138,6 +159,7 @@ void add_fast64_tiles(std::vector<FastEntry>& v, const char* radices) {
     if constexpr (ROLE == kRoleIntensity) {
         if constexpr (IO32) {
             add_fast64<SZ, ROLE, G, false, IO32, true>(v, radices);
+            add_strided_lds<SZ, ROLE, G, true>(v, radices);
             add_fast64<SZ, ROLE, G, true, IO32, true>(v, radices);
         }
         add_fast64<SZ, ROLE, G, true, IO32>(v, radices);
@@ -439,8 +461,10 @@ void add_fast_roles(std::vector<FastEntry>& v, const char* radices) {
         add_split_lds<SZ, fast_tile(kRoleSplit, SZ::TPL) / 2, true>(v, radices);
         add_fast64<SZ, kRoleSplit, fast_tile(kRoleSplit, SZ::TPL) / 2, true, true>(v, radices);
     }
-    if constexpr ((ROLES >> kRoleIntensityB) & 1u)
+    if constexpr ((ROLES >> kRoleIntensityB) & 1u) {
+        add_strided_lds<SZ, kRoleIntensityB, fast_tile(kRoleIntensityB, SZ::TPL) / 2, false>(v, radices);
         add_fast64<SZ, kRoleIntensityB, fast_tile(kRoleIntensityB, SZ::TPL) / 2, true, false>(v, radices);
+    }
     if constexpr ((ROLES >> kRoleMix1B) & 1u) add_fast<SZ, kRoleMix1B, fast_tile(kRoleMix1B, SZ::TPL)>(v, radices);
     if constexpr ((ROLES >> kRoleMix2B) & 1u) add_fast<SZ, kRoleMix2B, fast_tile(kRoleMix2B, SZ::TPL)>(v, radices);
 }
