@@ -223,6 +223,25 @@ def test_split_long_strided(refl, N, dim):
     assert err <= FP32_PEAK_TOL
 
 
+def test_intensity_staged_every_length(refl):
+    """The FP64 intensity pass on FP32 spectra with its next input staged in LDS (an autotuner
+    alternative) on every length that lists it, on 6 columns (one partial tile), against the
+    oracle at the FP32 bounds including |dlog10 I|."""
+    ran = 0
+    for N in fast_lengths("intensity"):
+        shape = (6, 5, N)
+        names = PlanCache(32, order="memory", arith="mixed").get(
+            workloads.twin_snapshot(("names",) + shape, *shape, seed=0), 0, refl).schedules(2)
+        if not any(nm.endswith(" ldsin") for nm in names):
+            continue
+        print(f"intensity staged N={N}", flush=True)
+        err = volume_check(refl, *shape, pick={2: " ldsin"}, arith="mixed", strict=True)
+        assert err <= FP32_PEAK_TOL, (N, err)
+        ran += 1
+    print(f"{ran} lengths with a staged intensity pass")
+    assert ran >= 4
+
+
 @pytest.mark.parametrize("arith,dim,N", [("fp32", 1, 1000), ("fp32", 1, 4000), ("fp32", 2, 4000),
                                          ("mixed", 2, 2000), ("mixed", 2, 4000)])
 @pytest.mark.parametrize("nx", [6, 18])
