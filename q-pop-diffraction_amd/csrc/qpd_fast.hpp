@@ -1781,8 +1781,9 @@ __global__ __launch_bounds__(G* SZ::TPL) __attribute__((amdgpu_waves_per_eu(WPE)
     constexpr int N = SZ::N, R0 = SZ::rad(0), NR0 = SZ::nr(0), LP = SZ::strided_lp(G), RL = SZ::RL, NRL = SZ::NRL;
     constexpr int TILE = SZ::strided_tile(G);
     constexpr int NITEMS = 3 * NS;
-    static_assert(G == 16, "a staged row is 128 bytes: eight lanes of sixteen bytes");
-    constexpr int RPI = 8;                          // rows per wave load
+    static_assert(G >= 2 && G <= 16 && G % 2 == 0, "a staged row is G / 2 lanes of sixteen bytes");
+    constexpr int LPR = G / 2;                      // lanes per staged row
+    constexpr int RPI = 64 / LPR;                   // rows per wave load
     constexpr int NLD = (N + RPI - 1) / RPI;        // wave loads per input
     constexpr int NWAVES = G * SZ::TPL / 64;         // whole waves, which issue the loads
     static_assert(NWAVES >= 1, "a whole wave loads the tile");
@@ -1821,10 +1822,10 @@ __global__ __launch_bounds__(G* SZ::TPL) __attribute__((amdgpu_waves_per_eu(WPE)
 #pragma unroll
         for (int k0 = 0; k0 < NLD; k0 += NWAVES) {
             const int k = k0 + wv;
-            const int row = k * RPI + (ln >> 3);
+            const int row = k * RPI + ln / LPR;
             // the columns of a partial last tile beyond n0 (even) are not loaded: their slots
             // keep what they held, their results are not stored
-            const uint32_t c2 = (uint32_t)(ln & 7) * 2;
+            const uint32_t c2 = (uint32_t)(ln % LPR) * 2;
             if (wv < NWAVES && k < NLD && row < N && t0 * G + c2 < a.n0)
                 __builtin_amdgcn_global_load_lds((const void*)(in + 2 * ((size_t)row * a.S + (size_t)c2)),
                                                  (__attribute__((address_space(3))) void*)(buf + k * RPI * G), 16, 0, 0);
