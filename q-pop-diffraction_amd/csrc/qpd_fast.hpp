@@ -1773,16 +1773,18 @@ __global__ __launch_bounds__(G* SZ::TPL) __attribute__((amdgpu_waves_per_eu(PF ?
 // Every wave waits for its own loads (vmcnt(0)) before the barrier after which the tile is
 // read, and the next loads are issued only after a second barrier, once every wave has read
 // its stage-0 inputs.  n0 and the row stride must be even (16-byte aligned column pairs).
-// T = float: the same in FP32 throughout.
-template <class SZ, int G, int NZ, int NS, int WPE, typename T = double>
+// T = float: the same in FP32 throughout.  IO32 false (FP64 plans): FP64 spectra, one value a
+// staged lane (a row of G = 8 columns is 128 bytes).
+template <class SZ, int G, int NZ, int NS, int WPE, typename T = double, bool IO32 = true>
 __global__ __launch_bounds__(G* SZ::TPL) __attribute__((amdgpu_waves_per_eu(WPE))) void fast_mix_held_lds(
     PassArgs<T> a) {
-    using TI = float;
+    using TI = std::conditional_t<IO32, float, T>;
     constexpr int N = SZ::N, R0 = SZ::rad(0), NR0 = SZ::nr(0), LP = SZ::strided_lp(G), RL = SZ::RL, NRL = SZ::NRL;
     constexpr int TILE = SZ::strided_tile(G);
     constexpr int NITEMS = 3 * NS;
-    static_assert(G >= 2 && G <= 16 && G % 2 == 0, "a staged row is G / 2 lanes of sixteen bytes");
-    constexpr int LPR = G / 2;                      // lanes per staged row
+    constexpr int EPL = 16 / (int)sizeof(cx<TI>);   // values a staged lane loads
+    static_assert(G % EPL == 0 && G / EPL <= 8, "a staged row is G / EPL lanes of sixteen bytes");
+    constexpr int LPR = G / EPL;                    // lanes per staged row
     constexpr int RPI = 64 / LPR;                   // rows per wave load
     constexpr int NLD = (N + RPI - 1) / RPI;        // wave loads per input
     constexpr int NWAVES = G * SZ::TPL / 64;         // whole waves, which issue the loads
@@ -1818,14 +1820,14 @@ __global__ __launch_bounds__(G* SZ::TPL) __attribute__((amdgpu_waves_per_eu(WPE)
     // input it -> field: A_s, B_s for s = it / 2 while it < 2 NS, then C_{it - 2 NS}
     auto issue = [&](int it, cx<TI>* buf) {
         const int fld = it < 2 * NS ? (it >> 1) * 3 + (it & 1) : (it - 2 * NS) * 3 + 2;
-        const float* in = reinterpret_cast<const float*>(a.in) + 2 * ((size_t)fld * a.ncell + tile0);
+        const TI* in = reinterpret_cast<const TI*>(a.in) + 2 * ((size_t)fld * a.ncell + tile0);
 #pragma unroll
         for (int k0 = 0; k0 < NLD; k0 += NWAVES) {
             const int k = k0 + wv;
             const int row = k * RPI + ln / LPR;
             // the columns of a partial last tile beyond n0 (even) are not loaded: their slots
             // keep what they held, their results are not stored
-            const uint32_t c2 = (uint32_t)(ln % LPR) * 2;
+            const uint32_t c2 = (uint32_t)(ln % LPR) * EPL;
             if (wv < NWAVES && k < NLD && row < N && t0 * G + c2 < a.n0)
                 __builtin_amdgcn_global_load_lds((const void*)(in + 2 * ((size_t)row * a.S + (size_t)c2)),
                                                  (__attribute__((address_space(3))) void*)(buf + k * RPI * G), 16, 0, 0);

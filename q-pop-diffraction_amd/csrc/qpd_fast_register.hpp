@@ -196,6 +196,10 @@ hipError_t held_lds_launch(const Launch& L, const PassArgs<double>& a, hipStream
     return launch_one(fast_mix_held_lds<SZ, G, NZ, NS, WPE>, L, st, a);
 }
 template <class SZ, int NZ, int NS, int G>
+hipError_t held_lds_launch64w(const Launch& L, const PassArgs<double>& a, hipStream_t st) {
+    return launch_one(fast_mix_held_lds<SZ, G, NZ, NS, 2, double, false>, L, st, a);
+}
+template <class SZ, int NZ, int NS, int G>
 hipError_t held_lds_launch32(const Launch& L, const PassArgs<float>& a, hipStream_t st) {
     return launch_one(fast_mix_held_lds<SZ, G, NZ, NS, 1, float>, L, st, a);
 }
@@ -257,6 +261,19 @@ void add_held(std::vector<FastEntry>& v, const char* radices) {
                     (const void*)fast_mix_held<double, SZ, G, NZ, NS, true>, false, 8,
                     &held_launch64<SZ, NZ, NS, G, true>, true};
         e.nsrc = NS;
+        v.push_back(e);
+    }
+    // FP64 plans, first with the next input staged in LDS (FP64 spectra: one value a lane, rows
+    // of 128 bytes at half the FP32 tile; 192 registers against 244): 1.033 against 1.035 ms for
+    // the 104-point pass of a 512 x 512 x 104 volume, 0.147 against 0.159 for the 64-point pass
+    // of a 256 x 256 x 64 volume
+    if constexpr (NS == 2 && G == 16 && (G / 2) * SZ::TPL >= 64 && (G / 2) * SZ::TPL <= 256 &&
+                  (tile64 + tabs) * 16 <= 160 * 1024) {
+        FastEntry e{SZ::N, ROLE, G / 2, (G / 2) * SZ::TPL, SZ::TPL, (tile64 + tabs) * 16, radices, nullptr,
+                    (const void*)fast_mix_held_lds<SZ, G / 2, NZ, NS, 2, double, false>, false, 8,
+                    &held_lds_launch64w<SZ, NZ, NS, G / 2>, false, true};
+        e.nsrc = NS;
+        e.lds_stage = true;
         v.push_back(e);
     }
     // FP64 (and FP64 on FP32 arrays) at half the FP32 tile
