@@ -1099,7 +1099,10 @@ __global__ __launch_bounds__(64 * kSrcWaves) __attribute__((amdgpu_waves_per_eu(
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* lines = reinterpret_cast<cx<T>*>(smem_raw);
     cx<T>* tw = lines + kSrcWaves * ROWS * SZ::PITCH;
-    TT* tabl = reinterpret_cast<TT*>(tw + NTW);
+    // the wavenumbers along the row, read from LDS where the outputs are formed: a global load
+    // there, after the other fields' stores, would wait for those stores
+    T* kl = reinterpret_cast<T*>(tw + NTW);
+    TT* tabl = reinterpret_cast<TT*>(kl + N);
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
     const int g = lane / TPL, j = lane - g * TPL;
@@ -1111,6 +1114,7 @@ __global__ __launch_bounds__(64 * kSrcWaves) __attribute__((amdgpu_waves_per_eu(
     fill_stage_tw<T, SZ, 1>(tw, a.W, tid, NTHR);
     fill_stage_tw<T, SZ, 2>(tw, a.W, tid, NTHR);
     fill_stage_tw<T, SZ, 3>(tw, a.W, tid, NTHR);
+    for (int m = tid; m < N; m += NTHR) kl[m] = a.k[m];
     // FP32 source: the atom table in FP32 with the su rows in memory-dim order (entry
     // 2 + 3 m + k <- 2 + 3 ax_m + k); FP64 keeps the table's own order (cell_source picks the axes)
     for (int m = tid; m < a.n_atom_tab; m += NTHR) {
@@ -1146,7 +1150,7 @@ __global__ __launch_bounds__(64 * kSrcWaves) __attribute__((amdgpu_waves_per_eu(
         if (has_out) {
 #pragma unroll
             for (int i = 0; i < NOS; ++i)
-                if (P == 1 || out_at[i] >= 0) oA[out_at[i]] = cx_as<TS>(v[0][i] + cscale(negi(v[1][i]), a.k[out_at[i]]));
+                if (P == 1 || out_at[i] >= 0) oA[out_at[i]] = cx_as<TS>(v[0][i] + cscale(negi(v[1][i]), kl[out_at[i]]));
         }
         wave_line_transform<T, SZ, P>(line, v[2], j, inrow, tw);
         if (has_out) {
@@ -1197,7 +1201,7 @@ __global__ __launch_bounds__(64 * kSrcWaves) __attribute__((amdgpu_waves_per_eu(
         if (has_out) {
 #pragma unroll
             for (int i = 0; i < NOS; ++i)
-                if (P == 1 || out_at[i] >= 0) oA[out_at[i]] = cx_as<TS>(w[i] + cscale(negi(f0[i]), a.k[out_at[i]]));
+                if (P == 1 || out_at[i] >= 0) oA[out_at[i]] = cx_as<TS>(w[i] + cscale(negi(f0[i]), kl[out_at[i]]));
         }
     };
     if constexpr (!SRC32) {

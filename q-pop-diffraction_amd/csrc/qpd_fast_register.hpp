@@ -379,7 +379,8 @@ template <class SZ>
 void add_source_wave(std::vector<FastEntry>& v, const char* radices) {
     if constexpr (SZ::TPL <= 64 && SZ::rad(0) <= 8) {
         constexpr int rows = 64 / SZ::TPL;
-        constexpr size_t cells = (size_t)kSrcWaves * rows * SZ::PITCH + StageTw<SZ>::total();
+        // line buffers and twiddles (complex), then the row's wavenumbers (real)
+        constexpr size_t cells = (size_t)kSrcWaves * rows * SZ::PITCH + StageTw<SZ>::total() + (SZ::N + 1) / 2;
         constexpr int P = split_lanes<SZ>();
         if constexpr (P > 1) {
             FastEntry e32{SZ::N, kRoleSourceW, kSrcWaves * rows, 64 * kSrcWaves, SZ::TPL, cells * 8, radices,
