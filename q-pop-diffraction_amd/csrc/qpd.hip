@@ -765,7 +765,9 @@ int pass_candidates(const AxisFFT& ax, int d, int md, const int64_t dims[3], int
                     // the held pass over source groups stages the pass's coefficient tables and
                     // the tile's mix factors (ncoef (N + nsrc G) values) in LDS
                     const size_t lds = e.lds + (src ? csize / 2 * (size_t)atomtab_entries : 0) +
-                                       (e.nsrc ? csize * (size_t)ncoef * (size_t)(N + e.nsrc * e.G) : 0);
+                                       (e.nsrc || e.tab_lds
+                                            ? csize * (size_t)ncoef * (size_t)(N + (e.nsrc ? e.nsrc : nsrc) * e.G)
+                                            : 0);
                     if ((int)lds > hard) continue;
                     if (!contig && !lane_offsets_fit(e.G, 1, row_stride, csize)) continue;
                     // sixteen-byte staged loads: 16-byte aligned column pairs, rows and fields
@@ -803,6 +805,9 @@ int pass_candidates(const AxisFFT& ax, int d, int md, const int64_t dims[3], int
                         for (const FastEntry& eb : fast_registry()) {
                             if (eb.N != n1 || eb.role != role_b || (int)eb.lds > hard) continue;
                             if ((size_t)eb.real_bytes * 2 != csize || eb.io32) continue;
+                            // tables of the pass over source groups staged in LDS (step 2: N1 points)
+                            const size_t ldsb = eb.lds + (eb.tab_lds ? csize * (size_t)ncoef * (size_t)(n1 + nsrc * eb.G) : 0);
+                            if ((int)ldsb > hard) continue;
                             if (eb.lds_stage && (dims[0] % 2 != 0 || row_stride % 2 != 0 ||
                                                  (dims[0] * dims[1] * dims[2]) % 2 != 0))
                                 continue;
@@ -812,7 +817,7 @@ int pass_candidates(const AxisFFT& ax, int d, int md, const int64_t dims[3], int
                             pp.N1 = (int)n1;
                             pp.N2 = (int)n2;
                             set_fast(pp.L[0], &ea, ea.lds);
-                            set_fast(pp.L[1], &eb, eb.lds);
+                            set_fast(pp.L[1], &eb, ldsb);
                             pp.L[0].blocks = (dims[0] + ea.G - 1) / ea.G * n1 * n_other;
                             pp.L[1].blocks = (dims[0] + eb.G - 1) / eb.G * n2 * n_other;
                             if (pp.L[0].blocks > 0x7fffffffLL || pp.L[1].blocks > 0x7fffffffLL) continue;
@@ -1159,6 +1164,7 @@ std::string describe_launch(const Launch& L) {
                       L.fe->no_prefetch ? " np" : "", L.fe->real_bytes == 8 ? " f64" : "", L.fe->io32 ? " io32" : "",
                       L.fe->one_buffer ? " 1buf" : "");
         if (L.fe->lds_stage) std::strncat(buf, " ldsin", sizeof(buf) - std::strlen(buf) - 1);
+        if (L.fe->tab_lds) std::strncat(buf, " tabl", sizeof(buf) - std::strlen(buf) - 1);
     } else
         std::snprintf(buf, sizeof(buf), "generic %s%s nl%d nf%d nthr%d ept%d%s%s", L.contiguous ? "contiguous" : "strided",
                       L.mix ? " groups" : "", L.nl, L.nf, L.nthr, L.ept,

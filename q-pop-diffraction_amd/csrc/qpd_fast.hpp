@@ -1518,7 +1518,10 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided(PassArgs<T> a) {
 // the same outputs as the class form.  Each group's three inputs are transformed once.
 // IO32 (FP64 only): FP64 arithmetic on complex FP32 inputs and outputs.
 // ONEBUF: one LDS stage buffer (transform's ONEBUF), for more blocks per CU.
-template <typename T, class SZ, int G, int NZ, bool SPLIT, bool IO32 = false, bool ONEBUF = false>
+// TABL: the block's wavenumbers, coefficient rows (a.ncoef of them) and mix factors are staged
+// in LDS once (a.ncoef (N + a.nsrc G) values and N reals after the twiddles): read from global
+// memory after the next input's prefetch is issued, each of them waited for that prefetch.
+template <typename T, class SZ, int G, int NZ, bool SPLIT, bool IO32 = false, bool ONEBUF = false, bool TABL = false>
 __global__ __launch_bounds__(G* SZ::TPL) void fast_strided_mix(PassArgs<T> a) {
     using TI = io_t<T, IO32>;
     constexpr int N = SZ::N, R0 = SZ::rad(0), NR0 = SZ::nr(0), LP = SZ::strided_lp(G), RL = SZ::RL, NRL = SZ::NRL;
@@ -1549,6 +1552,22 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided_mix(PassArgs<T> a) {
     const uint32_t in_lane = ((col_ok ? i0 : a.n0 - 1) - t0 * G + jl * in_step * a.S) * (uint32_t)sizeof(cx<TI>);
     const uint32_t c_lane = jl * c_step;
     load_twiddles<T, SZ>(Wl, a.W, tid, G * SZ::TPL);
+    T* kl = reinterpret_cast<T*>(Wl + N);
+    cx<T>* cyl = Wl + N + (N + 1) / 2;
+    cx<T>* xcl = cyl + (size_t)a.ncoef * N;
+    if constexpr (TABL) {
+        // row idx of the line: wavenumber and coefficient index c0 + idx c_step
+        for (int m = tid; m < N; m += G * SZ::TPL) kl[m] = a.k[c0 + (uint32_t)m * c_step];
+        for (int m = tid; m < a.ncoef * N; m += G * SZ::TPL) {
+            const int y = m / N, idx = m - y * N;
+            cyl[m] = a.coef[(size_t)y * a.Nc + c0 + (uint32_t)idx * c_step];
+        }
+        for (int m = tid; m < a.ncoef * a.nsrc * G; m += G * SZ::TPL) {
+            const int ys = m / G, c = m - ys * G;
+            const uint32_t col = t0 * G + (uint32_t)c;
+            xcl[m] = col < a.n0 ? a.mixc[(size_t)ys * a.n0 + col] : cx<T>{(T)0, (T)0};
+        }
+    }
     lds_barrier();
     constexpr bool PF = true;
     const int nitems = 3 * a.nsrc;
@@ -1590,7 +1609,8 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided_mix(PassArgs<T> a) {
 #pragma unroll
                 for (int r = 0; r < RL; ++r) {
                     if (f == 0) tA[r] = v[0][r];
-                    if (f == 1) tA[r] = tA[r] + cscale(negi(v[0][r]), load_at(a.k + c0 + (jw + r * NRL) * c_step, koff));
+                    if (f == 1)
+                        tA[r] = tA[r] + cscale(negi(v[0][r]), TABL ? kl[j + r * NRL] : load_at(a.k + c0 + (jw + r * NRL) * c_step, koff));
                     if (f == 2) tC[r] = v[0][r];
                 }
             }
@@ -1605,11 +1625,11 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided_mix(PassArgs<T> a) {
                 const cx<T>* xrow = a.mixc + (size_t)(y * a.nsrc + s) * a.n0 + t0 * G;
                 const cx<T>* cy = a.coef + (size_t)y * a.Nc + c0;
                 if (j < NRL && col_ok) {
-                    const cx<T> xc = load_at(xrow, g * (uint32_t)sizeof(cx<T>));
+                    const cx<T> xc = TABL ? xcl[(y * a.nsrc + s) * G + g] : load_at(xrow, g * (uint32_t)sizeof(cx<T>));
                     const uint32_t coff = c_lane * (uint32_t)sizeof(cx<T>);
 #pragma unroll
                     for (int r = 0; r < RL; ++r) {
-                        const cx<T> cf = cmul(load_at(cy + (jw + r * NRL) * c_step, coff), xc);
+                        const cx<T> cf = cmul(TABL ? cyl[y * N + j + r * NRL] : load_at(cy + (jw + r * NRL) * c_step, coff), xc);
                         acc[z][0][r] = acc[z][0][r] + cmul(cf, tA[r]);
                         acc[z][1][r] = acc[z][1][r] + cmul(cf, tC[r]);
                     }

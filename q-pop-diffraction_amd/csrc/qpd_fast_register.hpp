@@ -75,6 +75,27 @@ void add_split_lds(std::vector<FastEntry>& v, const char* radices) {
     }
 }
 
+// The pass over source groups (unsplit or the second step of a split) with its tables in LDS
+// (fast_strided_mix TABL), FP32; listed before the kernel that reads them from global memory.
+template <class SZ, int ROLE, int G>
+hipError_t mix_tabl_launch(const Launch& L, const PassArgs<float>& a, hipStream_t st) {
+    constexpr int NZ = (ROLE == kRoleMix1 || ROLE == kRoleMix1B) ? 1 : 2;
+    constexpr bool SPLIT = ROLE == kRoleMix1B || ROLE == kRoleMix2B;
+    return launch_one(fast_strided_mix<float, SZ, G, NZ, SPLIT, false, false, true>, L, st, a);
+}
+template <class SZ, int ROLE, int G>
+void add_mix_tabl(std::vector<FastEntry>& v, const char* radices) {
+    constexpr int NZ = (ROLE == kRoleMix1 || ROLE == kRoleMix1B) ? 1 : 2;
+    constexpr bool SPLIT = ROLE == kRoleMix1B || ROLE == kRoleMix2B;
+    constexpr size_t lds = (2 * (size_t)SZ::strided_tile(G) + SZ::N + (SZ::N + 1) / 2) * 8;
+    if constexpr (G >= 1 && G * SZ::TPL <= 1024 && lds <= 160 * 1024) {
+        FastEntry e{SZ::N, ROLE, G, G * SZ::TPL, SZ::TPL, lds, radices, &mix_tabl_launch<SZ, ROLE, G>,
+                    (const void*)fast_strided_mix<float, SZ, G, NZ, SPLIT, false, false, true>, false};
+        e.tab_lds = true;
+        v.push_back(e);
+    }
+}
+
 template <class SZ, int ROLE, int G, bool PF = true>
 void add_fast(std::vector<FastEntry>& v, const char* radices) {
     if constexpr (G >= 1) {
@@ -483,8 +504,14 @@ void add_fast_roles(std::vector<FastEntry>& v, const char* radices) {
         add_strided_lds<SZ, kRoleIntensityB, fast_tile(kRoleIntensityB, SZ::TPL) / 2, false>(v, radices);
         add_fast64<SZ, kRoleIntensityB, fast_tile(kRoleIntensityB, SZ::TPL) / 2, true, false>(v, radices);
     }
-    if constexpr ((ROLES >> kRoleMix1B) & 1u) add_fast<SZ, kRoleMix1B, fast_tile(kRoleMix1B, SZ::TPL)>(v, radices);
-    if constexpr ((ROLES >> kRoleMix2B) & 1u) add_fast<SZ, kRoleMix2B, fast_tile(kRoleMix2B, SZ::TPL)>(v, radices);
+    if constexpr ((ROLES >> kRoleMix1B) & 1u) {
+        add_mix_tabl<SZ, kRoleMix1B, fast_tile(kRoleMix1B, SZ::TPL)>(v, radices);
+        add_fast<SZ, kRoleMix1B, fast_tile(kRoleMix1B, SZ::TPL)>(v, radices);
+    }
+    if constexpr ((ROLES >> kRoleMix2B) & 1u) {
+        add_mix_tabl<SZ, kRoleMix2B, fast_tile(kRoleMix2B, SZ::TPL)>(v, radices);
+        add_fast<SZ, kRoleMix2B, fast_tile(kRoleMix2B, SZ::TPL)>(v, radices);
+    }
 }
 
 }  // namespace qpd

@@ -81,6 +81,7 @@ struct FastEntry {
     int nsrc = 0;             // pass 1 over source groups: the group count it requires (0: any)
     int split = 1;            // one-wave-per-row source pass: lanes per last-stage butterfly
     bool lds_stage = false;   // inputs staged in LDS by sixteen-byte loads: even n0 and strides
+    bool tab_lds = false;     // pass over source groups with its tables in LDS (a.ncoef (N + a.nsrc G) values more)
 };
 
 // Raise a kernel's dynamic LDS limit once per (kernel, device) and size.
