@@ -620,7 +620,9 @@ def run():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
+        # the arithmetic of the default plan: FP64 transforms where the plan runs them, FP32 spectra
+        "dtype": ("f64 transforms, f32 spectra" if all(b == 64 for b in plan.pass_bits)
+                  else "f32/f64 transforms (" + "/".join(map(str, plan.pass_bits)) + "-bit), f32 spectra"),
         "data": "synthetic (seeded, generated on the GPU)",
         "config": {"workload": f"two-phase Voronoi volume {nx}x{ny}x{nz}, 40 grains, 5-atom PbTiO3-like cell, "
                                f"hkl 002, one volume per step", "grid": [nx, ny, nz], "precision": "fp32",
