@@ -454,7 +454,10 @@ constexpr int chunk_writes() {
     return SZ::NST - 1 + (LOAD0 ? 1 : 0);
 }
 
-template <typename T, class SZ, int F, bool STRIDED, int LP, bool LOAD0, bool ONEBUF = false, bool NOFENCE = false>
+// FIRSTBAR false (ONEBUF): the caller has already put a barrier between the previous reads of
+// the buffer and this call, so the one before the first stage's write is left out.
+template <typename T, class SZ, int F, bool STRIDED, int LP, bool LOAD0, bool ONEBUF = false, bool NOFENCE = false,
+          bool FIRSTBAR = true>
 __device__ __forceinline__ void transform(cx<T>* X, cx<T>* Y, cx<T> (&v)[F][SZ::RMAX], int j, int line0, int dline,
                                           const cx<T>* __restrict__ W) {
     static_assert(!(ONEBUF && LOAD0), "one buffer holds the stage-0 inputs only when they are in registers");
@@ -467,14 +470,14 @@ __device__ __forceinline__ void transform(cx<T>* X, cx<T>* Y, cx<T> (&v)[F][SZ::
     // an odd first radix (>= 5) of a multi-stage line writes its outputs as it forms them
     constexpr bool ODD0 = F == 1 && SZ::NST > 1 && (SZ::rad(0) & 1) == 1 && SZ::rad(0) >= 5;
     if constexpr (ODD0) {
-        if constexpr (ONEBUF) stage_barrier<NOFENCE>();
+        if constexpr (ONEBUF && FIRSTBAR) stage_barrier<NOFENCE>();
         if (a0) stage_odd_to_lds<T, SZ, 0, STRIDED, LP>(wr, v, j, line0, W);
     } else {
         if (a0) stage_compute<T, SZ, 0, F>(v, j, W);
     }
     if constexpr (SZ::NST > 1) {
         if constexpr (!ODD0) {
-            if constexpr (ONEBUF) stage_barrier<NOFENCE>();
+            if constexpr (ONEBUF && FIRSTBAR) stage_barrier<NOFENCE>();
             if (a0) stage_write<T, SZ, 0, F, STRIDED, LP>(wr, v, j, line0, dline);
         }
         stage_barrier<NOFENCE>();
@@ -1897,7 +1900,9 @@ __global__ __launch_bounds__(G* SZ::TPL) __attribute__((amdgpu_waves_per_eu(WPE)
         }
         lds_barrier_nofence();  // every wave has read the staged tile
         if (it + 1 < NITEMS) issue(it + 1, buf);
-        transform<T, SZ, 1, true, LP, false, true, true>(bufA, bufA, v, j, g, 0, Wl);
+        // the barrier after the staged tile was read also orders the previous input's last reads
+        // of bufA before this input's first write
+        transform<T, SZ, 1, true, LP, false, true, true, false>(bufA, bufA, v, j, g, 0, Wl);
         if (j < NRL) {
 #pragma unroll
             for (int s = 0; s < NS; ++s) {
