@@ -1462,7 +1462,7 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_strided(PassArgs<T> a) {
                             v[0][r] = cx_as<T>(load_at(in + (size_t)((in0 + (jw + r * NR0) * in_step) * a.S), in_lane));
                     }
                 }
-                transform<T, SZ, 1, true, LP, false, ONEBUF, LDSIN>(par ? bufB : bufA, par ? bufA : bufB, v, j, g, 0, Wl);
+                transform<T, SZ, 1, true, LP, false, ONEBUF, LDSIN, !LDSIN>(par ? bufB : bufA, par ? bufA : bufB, v, j, g, 0, Wl);
                 if (ALT) par ^= 1;
                 if (j < SZ::NRL) {
                     const uint32_t coff = c_lane * (uint32_t)sizeof(cx<T>);
@@ -2023,7 +2023,9 @@ __global__ __launch_bounds__(G* SZ::TPL) void fast_split(PassArgs<T> a) {
 #pragma unroll
             for (int r = 0; r < SZ::RMAX; ++r) v[0][r] = cx_as<T>(cur[r]);
         }
-        transform<T, SZ, 1, true, LP, false, LDSIN, LDSIN>(par ? bufB : bufA, par ? bufA : bufB, v, j, g, 0, Wl);
+        // LDSIN: the barrier after the staged tile is read also orders the previous field's last
+        // reads of the stage buffer before this field's first write
+        transform<T, SZ, 1, true, LP, false, LDSIN, LDSIN, !LDSIN>(par ? bufB : bufA, par ? bufA : bufB, v, j, g, 0, Wl);
         if (ALT) par ^= 1;
         if (col_ok && j < SZ::NRL) {
             cx<T>* o = a.out + (size_t)f * a.ncell + tile0;
