@@ -598,6 +598,21 @@ void fill_common(PassArgs<T>& a, const AxisFFT& ax, const Stage* stages_dev, con
 constexpr int kMaxAtomTab = 1024;  // atom-table entries the source pass keeps in LDS
 
 // Private (scratch) bytes per lane of a kernel, 0 when unknown.
+// Static LDS of a kernel (its __shared__ arrays: the staging buffers of the staged-input
+// kernels), which the device adds to the dynamic LDS of a launch.
+size_t kernel_static_lds(const void* kernel) {
+    if (!kernel) return 0;
+    static std::mutex mu;
+    static std::map<const void*, size_t> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find(kernel);
+    if (it != cache.end()) return it->second;
+    hipFuncAttributes attr{};
+    const size_t bytes = hipFuncGetAttributes(&attr, kernel) == hipSuccess ? attr.sharedSizeBytes : 0;
+    cache[kernel] = bytes;
+    return bytes;
+}
+
 size_t kernel_scratch_bytes(const void* kernel) {
     if (!kernel) return 0;
     static std::mutex mu;
@@ -768,7 +783,7 @@ int pass_candidates(const AxisFFT& ax, int d, int md, const int64_t dims[3], int
                                        (e.nsrc || e.tab_lds
                                             ? csize * (size_t)ncoef * (size_t)(N + (e.nsrc ? e.nsrc : nsrc) * e.G)
                                             : 0);
-                    if ((int)lds > hard) continue;
+                    if ((int)(lds + (e.lds_stage ? kernel_static_lds(e.kernel) : 0)) > hard) continue;
                     if (!contig && !lane_offsets_fit(e.G, 1, row_stride, csize)) continue;
                     // sixteen-byte staged loads: 16-byte aligned column pairs, rows and fields
                     if (e.lds_stage && (dims[0] % 2 != 0 || row_stride % 2 != 0 ||
@@ -792,7 +807,9 @@ int pass_candidates(const AxisFFT& ax, int d, int md, const int64_t dims[3], int
                     if (N % n1) continue;
                     const int64_t n2 = N / n1;
                     for (const FastEntry& ea : fast_registry()) {
-                        if (ea.N != n2 || ea.role != kRoleSplit || (int)ea.lds > hard) continue;
+                        if (ea.N != n2 || ea.role != kRoleSplit ||
+                            (int)(ea.lds + (ea.lds_stage ? kernel_static_lds(ea.kernel) : 0)) > hard)
+                            continue;
                         if ((size_t)ea.real_bytes * 2 != csize || ea.io32 != split64) continue;
                         if (ea.lds_stage && (dims[0] % 2 != 0 || row_stride % 2 != 0 ||
                                              (dims[0] * dims[1] * dims[2]) % 2 != 0))
@@ -807,7 +824,7 @@ int pass_candidates(const AxisFFT& ax, int d, int md, const int64_t dims[3], int
                             if ((size_t)eb.real_bytes * 2 != csize || eb.io32) continue;
                             // tables of the pass over source groups staged in LDS (step 2: N1 points)
                             const size_t ldsb = eb.lds + (eb.tab_lds ? csize * (size_t)ncoef * (size_t)(n1 + nsrc * eb.G) : 0);
-                            if ((int)ldsb > hard) continue;
+                            if ((int)(ldsb + (eb.lds_stage ? kernel_static_lds(eb.kernel) : 0)) > hard) continue;
                             if (eb.lds_stage && (dims[0] % 2 != 0 || row_stride % 2 != 0 ||
                                                  (dims[0] * dims[1] * dims[2]) % 2 != 0))
                                 continue;
